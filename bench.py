@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Throughput of the Field Engine's MI355X backend on its headline workload: the fused 7-point conduction step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
+
+One "step" is one fused substep of `d T = diffuse(k, T)` over the whole lattice (ping-pong buffers, contracts off),
+on a seeded uniform [0,1) field already resident in HBM; the unit is cell-updates/s. Beside it: the copy kernel (the
+bandwidth roof), the handwritten kernel it must be bit-identical to, the same step at 512^3 (a working set larger than
+the 256 MB Infinity Cache), and the reference's own OpenMP CPU engine on this host's cores on a bounded sample. Prints
+ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "online-physics-cuda_amd"))
+sys.path.insert(0, ROOT)
+
+import field_hip  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+BYTES_PER_CELL = 8.0   # one float read + one float written per cell-update: the algorithmic minimum
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_launches(launch, steps: int, warmup: int):
+    """Whole-run time of `steps` back-to-back launches (ms per launch) and the mean of per-launch event times."""
+    for _ in range(warmup):
+        launch()
+    field_hip.sync()
+    ev = field_hip.Events()
+    ev.begin()
+    for _ in range(steps):
+        launch()
+    whole = ev.end() / steps
+    per = []
+    for _ in range(min(steps, 50)):
+        ev.begin()
+        launch()
+        per.append(ev.end())
+    return whole, float(np.mean(per)), float(np.median(per))
+
+
+def conduction(n: int, steps: int, warmup: int, seed: int = 2026):
+    cells = n ** 3
+    k, h = 0.1, 1.0 / n
+    sub_dt = 0.4 * h * h / k / 3  # one of the three substeps a step of dt = 0.4 h^2 / k takes under max_dt 0.15 h^2 / k
+    rng = np.random.default_rng(seed)
+    t0 = rng.random(cells, dtype=np.float32)
+    a = field_hip.DeviceArray.from_host(t0)
+    b = field_hip.DeviceArray(cells)
+    bufs = [a, b]
+
+    def hand():
+        field_hip.heat_handwritten(bufs[0], bufs[1], n, n, n, k, h, sub_dt)
+        bufs.reverse()
+
+    def copy():
+        field_hip.copy_f32(bufs[0], bufs[1])
+
+    copy_whole, copy_ev, _ = time_launches(copy, steps, warmup)
+    hand_whole, hand_ev, hand_med = time_launches(hand, steps, warmup)
+    return {
+        "n": n, "cells": cells,
+        "copy_ms": copy_ev, "copy_gbs": BYTES_PER_CELL * cells / (copy_ev * 1e6),
+        "hand_ms_per_step": hand_whole, "hand_kernel_ms": hand_ev, "hand_kernel_median_ms": hand_med,
+        "hand_gbs": BYTES_PER_CELL * cells / (hand_ev * 1e6),
+        "hand_cell_updates_per_s": cells / (hand_whole * 1e-3),
+    }
+
+
+HEAT_LAW = """field T init 0 range -1000000 1000000
+param k = 0.1
+term conduction
+  d T = diffuse(k, T)
+  conserves T
+  max_dt = 0.15 * h * h / k
+"""
+
+
+def cpu_baseline(n: int, budget_s: float):
+    """The reference's own CPU engine (oracle/_ref/lawc, built from its sources) on this host, a bounded sample of the
+    same step; the C restatement (oracle/) when the reference binary is absent."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    lawc = os.path.join(ROOT, "oracle", "_ref", "lawc")
+    h = 1.0 / n
+    dt = 0.4 * h * h / 0.1 / 3
+    if os.path.exists(lawc):
+        tmp = tempfile.mkdtemp()
+        try:
+            law = os.path.join(tmp, "heat.law")
+            open(law, "w").write(HEAT_LAW)
+            scene = os.path.join(tmp, "s.scene")
+            open(scene, "w").write(f"grid {n} {n} {n} {h!r}\nnoise T 1.0 7\n")
+            out = os.path.join(tmp, "t.json")
+            steps = 3
+            env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+            t = time.time()
+            subprocess.run([lawc, "--scene", scene, "--steps", str(steps + 1), "--warmup", "1", "--dt", repr(dt), "--json",
+                            out, law], check=True, capture_output=True, env=env, timeout=max(60, 10 * budget_s))
+            wall = time.time() - t
+            res = json.load(open(out))
+            ms = res["median_ms"]
+            return {"value": n ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "reference",
+                    "sample": f"reference lawc (CPU, OpenMP {cores} threads), d T = diffuse(k,T) with its contracts, "
+                              f"{n}^3, {steps} timed steps of one substep after 1 warm-up, median {ms:.1f} ms/step "
+                              f"({wall:.1f} s wall)"}
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    from oracle import heat
+    rng = np.random.default_rng(7)
+    t0 = rng.random(n ** 3, dtype=np.float32)
+    times = []
+    t_start = time.time()
+    cur = t0
+    while time.time() - t_start < budget_s or len(times) < 2:
+        t = time.time()
+        cur = heat.substep(cur, (n, n, n), 0.1, h, dt)
+        times.append(time.time() - t)
+    ms = 1e3 * float(np.median(times))
+    return {"value": n ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle C restatement (OpenMP), {n}^3, {len(times)} substeps, median {ms:.1f} ms"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-large", action="store_true", help="skip the 512^3 case")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        return 2
+    info = field_hip.require_device()
+    log(f"device {info.name.decode()} ({info.arch.decode()}, {info.compute_units} CUs)")
+
+    main_case = conduction(args.size, args.steps, args.warmup)
+    log(f"{args.size}^3: {json.dumps(main_case)}")
+    large = None if args.no_large else conduction(512, max(20, args.steps // 8), max(3, args.warmup // 4))
+    if large:
+        log(f"512^3: {json.dumps(large)}")
+    cpu = None if args.no_cpu else cpu_baseline(args.size, args.cpu_seconds)
+    if cpu:
+        log(f"cpu baseline: {json.dumps(cpu)}")
+
+    value = main_case["hand_cell_updates_per_s"]
+    achieved = BYTES_PER_CELL * main_case["cells"] / (main_case["hand_kernel_ms"] * 1e6)
+    line = {
+        "metric": "cell-updates/s (fused 7-point conduction step)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_case["hand_ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage, f64 per-cell arithmetic (the reference CPU's)",
+        "data": "synthetic (seeded uniform [0,1))",
+        "config": {"workload": f"d T = diffuse(k, T), {args.size}^3 fp32, h=1/{args.size}, k=0.1, one substep of "
+                               f"dt=0.4h^2/k/3, ping-pong, contracts off", "kernel": "handwritten heat_exact",
+                   "large_512": large, "copy_roof_gbs": main_case["copy_gbs"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,36 @@
+/* fe_field.h - the Field Engine's MI355X backend as a C ABI: handwritten kernels, the law compiler and the solver.
+ *
+ * The reference's engine API is C++ (src/field/Solver.hpp, src/field/Backend.hpp); this header is the same surface with
+ * plain pointers and sizes, for bindings (ctypes here; cgo/JNI would bind the same symbols) and for the tests. All device
+ * work is queued on the library's one stream (fe_hip.h). Entry points return 0 on success, a positive HIP status for a
+ * device failure, or -1 for arguments they do not take; none throws.
+ */
+#ifndef FE_FIELD_H
+#define FE_FIELD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- handwritten kernels ------------------------------------------------------------------------------------------ */
+
+/* One explicit substep of `d T = diffuse(k, T)` (one k everywhere), in -> out, device arrays of nx*ny*nz floats (x
+ * fastest), with the reference CPU's arithmetic (rate in double per cell, rounded to float; float store; denormals
+ * flushed). The reference's bit-identity kernel `heat` (bench/stencil_bench.cpp:47-65), whose result its engine is held
+ * to cell by cell (bench/stencil_bench.cpp:286-294). */
+int fe_heat_handwritten(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt);
+/* The same with the launch shape given: ty rows of 64 columns a block (1, 2, 4, 8), zchunk planes a block, remap = 1 for
+ * the XCD-aware block order. Every shape gives the same bits. */
+int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt, int ty,
+    int zchunk, int remap);
+/* b[i] = a[i], n floats: the bandwidth roof the reference measures beside its stencil (bench/stencil_bench.cpp:41-45). */
+int fe_copy_f32(const float* a, float* b, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,74 @@
+"""Writes the golden vectors under tests/golden/ by running the REFERENCE's own CPU engine.
+
+The reference's `lawc` is built from its sources where they lie (make -f oracle/Makefile.ref, outputs in oracle/_ref/,
+git-ignored) and run here, in the build container, once per case: a law (text written for this project), a seeded
+initial state, N steps of dt. Each case's inputs and the reference's outputs are committed as one small .npz; the
+tests hold the CPU restatement (oracle/) and the HIP backend to them. Nothing here runs on the GPU box.
+
+    python tests/golden/make_golden.py            (from the repository root)
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+HERE = os.path.dirname(os.path.abspath(__file__))
+LAWC = os.path.join(ROOT, "oracle", "_ref", "lawc")
+sys.path.insert(0, HERE)
+
+from cases import CASES  # noqa: E402
+
+
+def run_reference(case: dict) -> dict:
+    nx, ny, nz, h = case["grid"]
+    with tempfile.TemporaryDirectory() as tmp:
+        law_path = os.path.join(tmp, "case.law")
+        with open(law_path, "w") as f:
+            f.write(case["law"])
+        init_dir = os.path.join(tmp, "init")
+        os.makedirs(init_dir)
+        for name, values in case["init"].items():
+            np.asarray(values, dtype=np.float32).tofile(os.path.join(init_dir, name + ".bin"))
+        out_dir = os.path.join(tmp, "out")
+        cmd = [LAWC, "--grid", str(nx), str(ny), str(nz), repr(float(h)), "--init", init_dir, "--steps",
+               str(case["steps"]), "--dt", repr(float(case["dt"])), "--dump", out_dir]
+        for name, value in case.get("params", {}).items():
+            cmd += ["--param", name, repr(float(value))]
+        cmd.append(law_path)
+        res = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        if res.returncode != 0:
+            raise RuntimeError(f"lawc failed for {case['name']}: {res.stdout}\n{res.stderr}")
+        final_dir = os.path.join(out_dir, f"step_{case['steps']}")
+        meta = json.load(open(os.path.join(final_dir, "meta.json")))
+        final = {name: np.fromfile(os.path.join(final_dir, name + ".bin"), dtype=np.float32) for name in meta["fields"]}
+        return {"final": final, "stdout": res.stdout, "stderr": res.stderr}
+
+
+def main() -> None:
+    if not os.path.exists(LAWC):
+        sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    for case in CASES:
+        out = run_reference(case)
+        arrays = {}
+        for name, values in case["init"].items():
+            arrays["init_" + name] = np.asarray(values, dtype=np.float32)
+        for name, values in out["final"].items():
+            arrays["final_" + name] = values
+        meta = {k: case[k] for k in ("name", "law", "grid", "dt", "steps")}
+        meta["params"] = case.get("params", {})
+        meta["reference_stdout"] = out["stdout"]
+        meta["reference_stderr"] = out["stderr"]
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        path = os.path.join(HERE, case["name"] + ".npz")
+        np.savez_compressed(path, **arrays)
+        print(f"{case['name']}: {os.path.getsize(path)} bytes; {out['stdout'].strip().splitlines()[-1]}")
+
+
+if __name__ == "__main__":
+    main()
