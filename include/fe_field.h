@@ -22,10 +22,11 @@ extern "C" {
  * flushed). The reference's bit-identity kernel `heat` (bench/stencil_bench.cpp:47-65), whose result its engine is held
  * to cell by cell (bench/stencil_bench.cpp:286-294). */
 int fe_heat_handwritten(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt);
-/* The same with the launch shape given: ty rows of 64 columns a block (1, 2, 4, 8), zchunk planes a block, remap = 1 for
- * the XCD-aware block order. Every shape gives the same bits. */
+/* The same with the launch shape given. variant 1: a lane per column, ty rows of 64 columns a block (1, 2, 4, 8);
+ * variant 2 (nx a multiple of 4): four cells a lane, ty rows of 256 columns a block (2, 6, 14); 0 picks. zchunk planes a
+ * block; remap = 1 for the XCD-aware block order. Every shape and variant gives the same bits. */
 int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt, int ty,
-    int zchunk, int remap);
+    int zchunk, int remap, int variant);
 /* b[i] = a[i], n floats: the bandwidth roof the reference measures beside its stencil (bench/stencil_bench.cpp:41-45). */
 int fe_copy_f32(const float* a, float* b, size_t n);
 

@@ -121,6 +121,151 @@ __global__ __launch_bounds__(TX * TY) void heat_exact(HeatArgs a)
     }
 }
 
+// Four cells a lane: the kernel the headline step runs when nx is a multiple of 4.
+//
+// A block is TY + 2 waves over a 256 x TY tile of (x, y): waves 1..TY own a row each (64 lanes x 4 cells, one 128-bit
+// load and store a plane), waves 0 and TY + 1 only load the halo rows. Every wave keeps its column's next two planes
+// in flight (loads issued two planes ahead), so the card holds enough bytes in flight to cover HBM latency. Face
+// fluxes are shared where IEEE arithmetic allows it exactly: kf * (a - b) = -(kf * (b - a)) to the bit, so the flux a
+// cell sends through its +x (or +z) face is the one its neighbour receives, negated. A face past the lattice's edge is
+// computed against the cell itself: kf * (c - c) = +0, and adding +0 to a sum that starts at +0 changes no bit, so the
+// edges need no branches. The sums keep the CPU's order: -x, +x, -y, +y, -z, +z.
+template <int TY, bool POW2>
+__global__ __launch_bounds__(64 * (TY + 2)) void heat_exact_x4(HeatArgs a)
+{
+    constexpr int W = 256;
+    constexpr int ROW = W + 8; // [3] the left halo, [4, 4 + W) the tile's cells, [4 + W] the right halo
+    __shared__ __attribute__((aligned(16))) float tile[2][TY + 2][ROW];
+    const int total = a.tiles_x * a.tiles_y * a.tiles_z;
+    const int t = a.remap ? xcd_tile(static_cast<int>(blockIdx.x), total) : static_cast<int>(blockIdx.x);
+    const int txi = t % a.tiles_x;
+    const int tyi = (t / a.tiles_x) % a.tiles_y;
+    const int tzi = t / (a.tiles_x * a.tiles_y);
+    const int lane = static_cast<int>(threadIdx.x);
+    const int r = static_cast<int>(threadIdx.y);
+    const int x0 = txi * W + 4 * lane;
+    const int y = tyi * TY + r - 1;
+    const int z0 = tzi * a.zchunk;
+    const int z1 = min(z0 + a.zchunk, a.nz);
+    const bool live = x0 < a.nx && r >= 1 && r <= TY && y < a.ny;
+    const int xs = min(x0, a.nx - 4);
+    const int ys = min(max(y, 0), a.ny - 1);
+    const std::size_t plane = static_cast<std::size_t>(a.nx) * static_cast<std::size_t>(a.ny);
+    const float* base = a.in + static_cast<std::size_t>(ys) * static_cast<std::size_t>(a.nx) + static_cast<std::size_t>(xs);
+    const bool want_l = lane == 0 && xs > 0;
+    const bool want_r = lane == 63 && xs + 4 < a.nx;
+    auto ld4 = [&](int z) { return *reinterpret_cast<const float4*>(base + static_cast<std::size_t>(z) * plane); };
+    auto ldl = [&](int z) { return want_l ? base[static_cast<std::size_t>(z) * plane - 1] : 0.0f; };
+    auto ldr = [&](int z) { return want_r ? base[static_cast<std::size_t>(z) * plane + 4] : 0.0f; };
+
+    float4 c = ld4(z0);
+    float cl = ldl(z0), cr = ldr(z0);
+    const bool has1 = z0 + 1 < a.nz;
+    float4 h1 = has1 ? ld4(z0 + 1) : c;
+    float h1l = has1 ? ldl(z0 + 1) : 0.0f, h1r = has1 ? ldr(z0 + 1) : 0.0f;
+    const bool has2 = z0 + 2 < a.nz && z0 + 1 < z1;
+    float4 h2 = has2 ? ld4(z0 + 2) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float h2l = has2 ? ldl(z0 + 2) : 0.0f, h2r = has2 ? ldr(z0 + 2) : 0.0f;
+    const float4 lo = z0 > 0 ? ld4(z0 - 1) : c;
+
+    const double kf = a.kf;
+    double c0 = static_cast<double>(c.x), c1 = static_cast<double>(c.y), c2 = static_cast<double>(c.z),
+           c3 = static_cast<double>(c.w);
+    // The flux each cell receives through its -z face at the chunk's first plane.
+    double fzm0 = kf * (static_cast<double>(lo.x) - c0), fzm1 = kf * (static_cast<double>(lo.y) - c1),
+           fzm2 = kf * (static_cast<double>(lo.z) - c2), fzm3 = kf * (static_cast<double>(lo.w) - c3);
+    float* outp = a.out + static_cast<std::size_t>(ys) * static_cast<std::size_t>(a.nx) + static_cast<std::size_t>(xs);
+    int buf = 0;
+    for (int z = z0; z < z1; ++z) {
+        const bool more = z + 3 < a.nz && z + 2 < z1;
+        const float4 h3 = more ? ld4(z + 3) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float h3l = more ? ldl(z + 3) : 0.0f, h3r = more ? ldr(z + 3) : 0.0f;
+        *reinterpret_cast<float4*>(&tile[buf][r][4 + 4 * lane]) = c;
+        if (lane == 0) {
+            tile[buf][r][3] = cl;
+        }
+        if (lane == 63) {
+            tile[buf][r][4 + W] = cr;
+        }
+        __syncthreads();
+        const bool up = z + 1 < a.nz;
+        const float4 hi = up ? h1 : c;
+        const double d0 = static_cast<double>(hi.x), d1 = static_cast<double>(hi.y), d2 = static_cast<double>(hi.z),
+                     d3 = static_cast<double>(hi.w);
+        if (live) {
+            const float4 ym = y > 0 ? *reinterpret_cast<const float4*>(&tile[buf][r - 1][4 + 4 * lane]) : c;
+            const float4 yp = y + 1 < a.ny ? *reinterpret_cast<const float4*>(&tile[buf][r + 1][4 + 4 * lane]) : c;
+            const float xme = x0 > 0 ? tile[buf][r][3 + 4 * lane] : c.x;
+            const float xpe = x0 + 4 < a.nx ? tile[buf][r][8 + 4 * lane] : c.w;
+            const double fxm = kf * (static_cast<double>(xme) - c0);
+            const double f01 = kf * (c1 - c0);
+            const double f12 = kf * (c2 - c1);
+            const double f23 = kf * (c3 - c2);
+            const double fxp = kf * (static_cast<double>(xpe) - c3);
+            const double fzp0 = kf * (d0 - c0), fzp1 = kf * (d1 - c1), fzp2 = kf * (d2 - c2), fzp3 = kf * (d3 - c3);
+            double q0 = 0.0 + fxm;
+            q0 += f01;
+            q0 += kf * (static_cast<double>(ym.x) - c0);
+            q0 += kf * (static_cast<double>(yp.x) - c0);
+            q0 += fzm0;
+            q0 += fzp0;
+            double q1 = 0.0 + (-f01);
+            q1 += f12;
+            q1 += kf * (static_cast<double>(ym.y) - c1);
+            q1 += kf * (static_cast<double>(yp.y) - c1);
+            q1 += fzm1;
+            q1 += fzp1;
+            double q2 = 0.0 + (-f12);
+            q2 += f23;
+            q2 += kf * (static_cast<double>(ym.z) - c2);
+            q2 += kf * (static_cast<double>(yp.z) - c2);
+            q2 += fzm2;
+            q2 += fzp2;
+            double q3 = 0.0 + (-f23);
+            q3 += fxp;
+            q3 += kf * (static_cast<double>(ym.w) - c3);
+            q3 += kf * (static_cast<double>(yp.w) - c3);
+            q3 += fzm3;
+            q3 += fzp3;
+            if (POW2) {
+                q0 *= a.inv_hh;
+                q1 *= a.inv_hh;
+                q2 *= a.inv_hh;
+                q3 *= a.inv_hh;
+            } else {
+                q0 /= a.hh;
+                q1 /= a.hh;
+                q2 /= a.hh;
+                q3 /= a.hh;
+            }
+            float4 o;
+            o.x = fe_ftz(c.x + fe_rf(static_cast<double>(fe_rf(q0)) * a.sub_dt));
+            o.y = fe_ftz(c.y + fe_rf(static_cast<double>(fe_rf(q1)) * a.sub_dt));
+            o.z = fe_ftz(c.z + fe_rf(static_cast<double>(fe_rf(q2)) * a.sub_dt));
+            o.w = fe_ftz(c.w + fe_rf(static_cast<double>(fe_rf(q3)) * a.sub_dt));
+            *reinterpret_cast<float4*>(outp + static_cast<std::size_t>(z) * plane) = o;
+            fzm0 = -fzp0;
+            fzm1 = -fzp1;
+            fzm2 = -fzp2;
+            fzm3 = -fzp3;
+        }
+        buf ^= 1;
+        c = hi;
+        cl = h1l;
+        cr = h1r;
+        c0 = d0;
+        c1 = d1;
+        c2 = d2;
+        c3 = d3;
+        h1 = h2;
+        h1l = h2l;
+        h1r = h2r;
+        h2 = h3;
+        h2l = h3l;
+        h2r = h3r;
+    }
+}
+
 __global__ __launch_bounds__(256) void copy_f32x4(const float4* __restrict__ a, float4* __restrict__ b, std::size_t n4)
 {
     for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n4;
@@ -150,10 +295,11 @@ extern "C" {
 void* fe_hip_stream(void);
 
 // One substep of the handwritten conduction kernel on the library's stream. in/out are device arrays of nx*ny*nz
-// floats (x fastest), distinct. ty: rows per block (1, 2, 4 or 8; 64 columns each); zchunk: planes per block;
-// remap: XCD-aware block order. Returns a HIP status, or -1 for arguments the kernel does not take.
+// floats (x fastest), distinct. variant 1: a lane a column, ty rows of 64 columns a block (1, 2, 4 or 8); variant 2
+// (nx a multiple of 4): four cells a lane, ty rows of 256 columns a block (2, 6 or 14); variant 0 picks. zchunk:
+// planes per block; remap: XCD-aware block order. Returns a HIP status, or -1 for arguments the kernel does not take.
 int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt, int ty,
-    int zchunk, int remap)
+    int zchunk, int remap, int variant)
 {
     if (in == nullptr || out == nullptr || in == out || nx < 1 || ny < 1 || nz < 1 || zchunk < 1 || !(h > 0.0)) {
         return -1;
@@ -180,6 +326,32 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
         return -1;
     }
     hipStream_t s = static_cast<hipStream_t>(fe_hip_stream());
+    if (variant == 2 || (variant == 0 && nx % 4 == 0)) {
+        if (nx % 4 != 0) {
+            return -1;
+        }
+        a.tiles_x = (nx + 255) / 256;
+        a.tiles_y = (ny + ty - 1) / ty;
+        const long long total4 = static_cast<long long>(a.tiles_x) * a.tiles_y * a.tiles_z;
+        if (total4 > 0x7fffffffLL) {
+            return -1;
+        }
+        const dim3 grid4(static_cast<unsigned>(total4));
+#define FE_X4(TYV)                                                                                                     \
+    if (a.hh_pow2) {                                                                                                   \
+        hipLaunchKernelGGL((heat_exact_x4<TYV, true>), grid4, dim3(64, TYV + 2), 0, s, a);                             \
+    } else {                                                                                                           \
+        hipLaunchKernelGGL((heat_exact_x4<TYV, false>), grid4, dim3(64, TYV + 2), 0, s, a);                            \
+    }
+        switch (ty) {
+        case 2: FE_X4(2) break;
+        case 6: FE_X4(6) break;
+        case 14: FE_X4(14) break;
+        default: return -1;
+        }
+#undef FE_X4
+        return static_cast<int>(hipGetLastError());
+    }
     const dim3 grid(static_cast<unsigned>(total));
     switch (ty) {
     case 1: hipLaunchKernelGGL((heat_exact<TX, 1>), grid, dim3(TX, 1), 0, s, a); break;
@@ -193,7 +365,10 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
 
 int fe_heat_handwritten(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt)
 {
-    return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 4, 32, 1);
+    if (nx % 4 == 0) {
+        return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 6, 8, 1, 2);
+    }
+    return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 4, 32, 1, 1);
 }
 
 // b[i] = a[i] for n floats: the bandwidth a plain copy reaches, the roof a stencil's step is measured against.

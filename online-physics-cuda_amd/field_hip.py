@@ -57,7 +57,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_hip_to_bf16": (ctypes.c_uint16, [c_float]),
         "fe_heat_handwritten": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double]),
         "fe_heat_handwritten_cfg": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double,
-                                            c_int, c_int, c_int]),
+                                            c_int, c_int, c_int, c_int]),
         "fe_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t]),
     }
     for name, (res, args) in sig.items():
@@ -171,12 +171,21 @@ class Events:
 
 
 def heat_handwritten(src: DeviceArray, dst: DeviceArray, nx: int, ny: int, nz: int, k: float, h: float, sub_dt: float,
-                     ty: int = 4, zchunk: int = 32, remap: int = 1) -> None:
-    """One substep of the handwritten 7-point conduction kernel, src -> dst (queued on the library's stream)."""
+                     ty: Optional[int] = None, zchunk: Optional[int] = None, remap: int = 1, variant: int = 0) -> None:
+    """One substep of the handwritten 7-point conduction kernel, src -> dst (queued on the library's stream).
+
+    variant 0 picks (four cells a lane when nx % 4 == 0), 1 a lane a column, 2 four cells a lane."""
     n = nx * ny * nz
     if src.count != n or dst.count != n or src.dtype != np.float32 or dst.dtype != np.float32:
         raise FieldHipError("heat_handwritten: arrays must hold nx*ny*nz floats")
-    check(lib().fe_heat_handwritten_cfg(src.ptr, dst.ptr, nx, ny, nz, k, h, sub_dt, ty, zchunk, remap), "heat_handwritten")
+    if ty is None and zchunk is None and variant == 0:
+        check(lib().fe_heat_handwritten(src.ptr, dst.ptr, nx, ny, nz, k, h, sub_dt), "heat_handwritten")
+        return
+    x4 = variant == 2 or (variant == 0 and nx % 4 == 0)
+    ty = ty if ty is not None else (6 if x4 else 4)
+    zchunk = zchunk if zchunk is not None else (8 if x4 else 32)
+    check(lib().fe_heat_handwritten_cfg(src.ptr, dst.ptr, nx, ny, nz, k, h, sub_dt, ty, zchunk, remap,
+                                        2 if x4 else 1), "heat_handwritten")
 
 
 def copy_f32(src: DeviceArray, dst: DeviceArray) -> None:

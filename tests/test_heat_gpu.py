@@ -31,20 +31,46 @@ def test_handwritten_matches_reference_bits(device, name):
         f"{name}: max |diff| {np.max(np.abs(out - final['T']))}"
 
 
-@pytest.mark.parametrize("ty,zchunk,remap", [(1, 1, 0), (2, 7, 1), (4, 32, 1), (8, 64, 0), (4, 1000, 1)])
-def test_every_launch_shape_gives_the_oracle_bits(device, ty, zchunk, remap):
+SHAPES = [
+    # variant 1: a lane a column (any nx)
+    (1, (130, 37, 19), 1, 1, 0), (1, (130, 37, 19), 2, 7, 1), (1, (130, 37, 19), 4, 32, 1), (1, (130, 37, 19), 8, 64, 0),
+    (1, (130, 37, 19), 4, 1000, 1),
+    # variant 2: four cells a lane (nx % 4 == 0), tiles of 256 columns: nx below, at and above one tile
+    (2, (132, 37, 19), 2, 1, 0), (2, (132, 37, 19), 6, 3, 1), (2, (132, 37, 19), 14, 8, 1), (2, (256, 9, 11), 6, 4, 1),
+    (2, (516, 13, 10), 6, 5, 0), (2, (8, 3, 2), 2, 1, 1), (2, (260, 20, 33), 14, 64, 1),
+]
+
+
+@pytest.mark.parametrize("variant,shape,ty,zchunk,remap", SHAPES)
+def test_every_launch_shape_gives_the_oracle_bits(device, variant, shape, ty, zchunk, remap):
     import field_hip
 
-    shape = (130, 37, 19)  # not a multiple of the tile on any axis
+    print(f"variant {variant} shape {shape} ty {ty} zchunk {zchunk} remap {remap}", flush=True)
     rng = np.random.default_rng(11)
     t = rng.random(int(np.prod(shape))).astype(np.float32)
-    k, h, sub = 0.2, 0.37, 0.05
-    ref = heat.substep(t, shape, k, h, sub)
-    a = field_hip.DeviceArray.from_host(t)
-    b = field_hip.DeviceArray(a.count)
-    field_hip.heat_handwritten(a, b, *shape, k, h, sub, ty=ty, zchunk=zchunk, remap=remap)
-    out = b.download()
-    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    for k, h, sub in ((0.2, 0.37, 0.05), (0.1, 0.25, 0.1)):  # h*h not a power of two, then one
+        ref = heat.substep(t, shape, k, h, sub)
+        a = field_hip.DeviceArray.from_host(t)
+        b = field_hip.DeviceArray(a.count)
+        field_hip.heat_handwritten(a, b, *shape, k, h, sub, ty=ty, zchunk=zchunk, remap=remap, variant=variant)
+        out = b.download()
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), f"max |diff| {np.max(np.abs(out - ref))}"
+
+
+def test_denormals_flush_as_on_the_cpu(device):
+    import field_hip
+
+    # values near the smallest normal float: the CPU's FTZ/DAZ threads flush results below it to signed zero
+    shape = (64, 8, 4)
+    rng = np.random.default_rng(3)
+    t = (rng.random(int(np.prod(shape))) * 4e-38 - 2e-38).astype(np.float32)
+    for variant in (1, 2):
+        ref = heat.substep(t, shape, 0.3, 1.0, 0.5)
+        a = field_hip.DeviceArray.from_host(t)
+        b = field_hip.DeviceArray(a.count)
+        field_hip.heat_handwritten(a, b, *shape, 0.3, 1.0, 0.5, variant=variant)
+        out = b.download()
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), variant
 
 
 def test_full_size_substep_is_the_oracle_and_repeats(device):
