@@ -124,13 +124,20 @@ __global__ __launch_bounds__(TX * TY) void heat_exact(HeatArgs a)
 // Four cells a lane: the kernel the headline step runs when nx is a multiple of 4.
 //
 // A block is TY + 2 waves over a 256 x TY tile of (x, y): waves 1..TY own a row each (64 lanes x 4 cells, one 128-bit
-// load and store a plane), waves 0 and TY + 1 only load the halo rows. Every wave keeps its column's next two planes
-// in flight (loads issued two planes ahead), so the card holds enough bytes in flight to cover HBM latency. Face
-// fluxes are shared where IEEE arithmetic allows it exactly: kf * (a - b) = -(kf * (b - a)) to the bit, so the flux a
-// cell sends through its +x (or +z) face is the one its neighbour receives, negated. A face past the lattice's edge is
-// computed against the cell itself: kf * (c - c) = +0, and adding +0 to a sum that starts at +0 changes no bit, so the
-// edges need no branches. The sums keep the CPU's order: -x, +x, -y, +y, -z, +z.
-template <int TY, bool POW2>
+// load and store a plane), waves 0 and TY + 1 only load the halo rows. Every wave keeps its column's next planes in
+// flight: a ring of four plane slots, the loop unrolled four times so the slots are renamed rather than copied (a copy
+// of a register a load is still filling would wait for the load). Face fluxes are shared where IEEE arithmetic allows
+// it exactly: kf * (a - b) = -(kf * (b - a)) to the bit, so the flux a cell sends through its +x (or +z) face is the
+// one its neighbour receives, negated. A face past the lattice's edge is computed against the cell itself:
+// kf * (c - c) = +0, and adding +0 to a sum that starts at +0 changes no bit, so the edges need no branches. The sums
+// keep the CPU's order: -x, +x, -y, +y, -z, +z.
+struct Plane4 {
+    float4 v; // this lane's four cells of one plane
+    float l;  // the cell left of them (lane 0 only, inside the lattice)
+    float r;  // the cell right of them (lane 63 only, inside the lattice)
+};
+
+template <int TY, bool POW2, bool XH>
 __global__ __launch_bounds__(64 * (TY + 2)) void heat_exact_x4(HeatArgs a)
 {
     constexpr int W = 256;
@@ -150,53 +157,70 @@ __global__ __launch_bounds__(64 * (TY + 2)) void heat_exact_x4(HeatArgs a)
     const bool live = x0 < a.nx && r >= 1 && r <= TY && y < a.ny;
     const int xs = min(x0, a.nx - 4);
     const int ys = min(max(y, 0), a.ny - 1);
-    const std::size_t plane = static_cast<std::size_t>(a.nx) * static_cast<std::size_t>(a.ny);
-    const float* base = a.in + static_cast<std::size_t>(ys) * static_cast<std::size_t>(a.nx) + static_cast<std::size_t>(xs);
-    const bool want_l = lane == 0 && xs > 0;
-    const bool want_r = lane == 63 && xs + 4 < a.nx;
-    auto ld4 = [&](int z) { return *reinterpret_cast<const float4*>(base + static_cast<std::size_t>(z) * plane); };
-    auto ldl = [&](int z) { return want_l ? base[static_cast<std::size_t>(z) * plane - 1] : 0.0f; };
-    auto ldr = [&](int z) { return want_r ? base[static_cast<std::size_t>(z) * plane + 4] : 0.0f; };
-
-    float4 c = ld4(z0);
-    float cl = ldl(z0), cr = ldr(z0);
-    const bool has1 = z0 + 1 < a.nz;
-    float4 h1 = has1 ? ld4(z0 + 1) : c;
-    float h1l = has1 ? ldl(z0 + 1) : 0.0f, h1r = has1 ? ldr(z0 + 1) : 0.0f;
-    const bool has2 = z0 + 2 < a.nz && z0 + 1 < z1;
-    float4 h2 = has2 ? ld4(z0 + 2) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float h2l = has2 ? ldl(z0 + 2) : 0.0f, h2r = has2 ? ldr(z0 + 2) : 0.0f;
-    const float4 lo = z0 > 0 ? ld4(z0 - 1) : c;
-
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const std::size_t plane = static_cast<std::size_t>(nx) * static_cast<std::size_t>(ny);
+    const float* base = a.in + static_cast<std::size_t>(ys) * static_cast<std::size_t>(nx) + static_cast<std::size_t>(xs);
+    float* outp = a.out + static_cast<std::size_t>(ys) * static_cast<std::size_t>(nx) + static_cast<std::size_t>(xs);
+    // Every load is unconditional, from a clamped plane: a load under a branch makes the compiler wait for all
+    // loads in flight. Past the top of the lattice the "next" plane is the plane itself, which is the zero-flux
+    // edge (the face against itself is +0). Halo columns (only when a row spans several tiles) likewise: lanes
+    // other than 0 and 63, and lanes at the lattice's edge, load their own cell, which nothing reads.
+    const int zlim = min(z1, nz - 1);
+    const int lofs = XH && lane == 0 && xs > 0 ? -1 : 0;
+    const int rofs = XH && lane == 63 && xs + 4 < nx ? 4 : 0;
     const double kf = a.kf;
-    double c0 = static_cast<double>(c.x), c1 = static_cast<double>(c.y), c2 = static_cast<double>(c.z),
-           c3 = static_cast<double>(c.w);
-    // The flux each cell receives through its -z face at the chunk's first plane.
+
+    auto load = [&](Plane4& p, int z) {
+        const float* q = base + static_cast<std::size_t>(min(z, zlim)) * plane;
+        p.v = *reinterpret_cast<const float4*>(q);
+        if (XH) {
+            p.l = q[lofs];
+            p.r = q[rofs];
+        }
+    };
+
+    Plane4 A, B, C, D;
+    load(A, z0);
+    load(B, z0 + 1);
+    load(C, z0 + 2);
+    const float4 lo = *reinterpret_cast<const float4*>(base + static_cast<std::size_t>(max(z0 - 1, 0)) * plane);
+    double c0 = static_cast<double>(A.v.x), c1 = static_cast<double>(A.v.y), c2 = static_cast<double>(A.v.z),
+           c3 = static_cast<double>(A.v.w);
+    // The flux each cell receives through its -z face at the chunk's first plane (+0 at the bottom of the lattice).
     double fzm0 = kf * (static_cast<double>(lo.x) - c0), fzm1 = kf * (static_cast<double>(lo.y) - c1),
            fzm2 = kf * (static_cast<double>(lo.z) - c2), fzm3 = kf * (static_cast<double>(lo.w) - c3);
-    float* outp = a.out + static_cast<std::size_t>(ys) * static_cast<std::size_t>(a.nx) + static_cast<std::size_t>(xs);
     int buf = 0;
-    for (int z = z0; z < z1; ++z) {
-        const bool more = z + 3 < a.nz && z + 2 < z1;
-        const float4 h3 = more ? ld4(z + 3) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float h3l = more ? ldl(z + 3) : 0.0f, h3r = more ? ldr(z + 3) : 0.0f;
+
+    // One plane: `cur` is plane z, `nxt` plane z + 1 (loaded), `far` receives plane z + 3.
+    auto step = [&](int z, const Plane4& cur, const Plane4& nxt, Plane4& far) {
+        load(far, z + 3);
+        const float4 c = cur.v;
         *reinterpret_cast<float4*>(&tile[buf][r][4 + 4 * lane]) = c;
-        if (lane == 0) {
-            tile[buf][r][3] = cl;
-        }
-        if (lane == 63) {
-            tile[buf][r][4 + W] = cr;
+        if (XH) {
+            if (lane == 0) {
+                tile[buf][r][3] = cur.l;
+            }
+            if (lane == 63) {
+                tile[buf][r][4 + W] = cur.r;
+            }
         }
         __syncthreads();
-        const bool up = z + 1 < a.nz;
-        const float4 hi = up ? h1 : c;
+        const float4 hi = nxt.v;
         const double d0 = static_cast<double>(hi.x), d1 = static_cast<double>(hi.y), d2 = static_cast<double>(hi.z),
                      d3 = static_cast<double>(hi.w);
         if (live) {
-            const float4 ym = y > 0 ? *reinterpret_cast<const float4*>(&tile[buf][r - 1][4 + 4 * lane]) : c;
-            const float4 yp = y + 1 < a.ny ? *reinterpret_cast<const float4*>(&tile[buf][r + 1][4 + 4 * lane]) : c;
-            const float xme = x0 > 0 ? tile[buf][r][3 + 4 * lane] : c.x;
-            const float xpe = x0 + 4 < a.nx ? tile[buf][r][8 + 4 * lane] : c.w;
+            // Values first, then the edge selects: a select between an LDS lvalue and a register would make the
+            // compiler take addresses (flat loads and scratch).
+            const float4 tym = *reinterpret_cast<const float4*>(&tile[buf][r - 1][4 + 4 * lane]);
+            const float4 typ = *reinterpret_cast<const float4*>(&tile[buf][r + 1][4 + 4 * lane]);
+            const float txm = tile[buf][r][3 + 4 * lane];
+            const float txp = tile[buf][r][8 + 4 * lane];
+            // The halo waves load a clamped row, so past the lattice's y edges the row beside is the row itself:
+            // ym and yp need no select. A row spans every lane only when nx is a whole number of tiles, so x does.
+            const float4 ym = tym;
+            const float4 yp = typ;
+            const float xme = x0 > 0 ? txm : c.x;
+            const float xpe = x0 + 4 < nx ? txp : c.w;
             const double fxm = kf * (static_cast<double>(xme) - c0);
             const double f01 = kf * (c1 - c0);
             const double f12 = kf * (c2 - c1);
@@ -250,26 +274,45 @@ __global__ __launch_bounds__(64 * (TY + 2)) void heat_exact_x4(HeatArgs a)
             fzm3 = -fzp3;
         }
         buf ^= 1;
-        c = hi;
-        cl = h1l;
-        cr = h1r;
         c0 = d0;
         c1 = d1;
         c2 = d2;
         c3 = d3;
-        h1 = h2;
-        h1l = h2l;
-        h1r = h2r;
-        h2 = h3;
-        h2l = h3l;
-        h2r = h3r;
+    };
+
+    for (int z = z0; z < z1; z += 4) {
+        step(z, A, B, D);
+        if (z + 1 >= z1) {
+            break;
+        }
+        step(z + 1, B, C, A);
+        if (z + 2 >= z1) {
+            break;
+        }
+        step(z + 2, C, D, B);
+        if (z + 3 >= z1) {
+            break;
+        }
+        step(z + 3, D, A, C);
     }
 }
 
 __global__ __launch_bounds__(256) void copy_f32x4(const float4* __restrict__ a, float4* __restrict__ b, std::size_t n4)
 {
-    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n4;
-         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    // Four 128-bit loads in flight a lane before the first store.
+    const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+    std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 v0 = a[i];
+        const float4 v1 = a[i + stride];
+        const float4 v2 = a[i + 2 * stride];
+        const float4 v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
+    for (; i < n4; i += stride) {
         b[i] = a[i];
     }
 }
@@ -337,11 +380,18 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
             return -1;
         }
         const dim3 grid4(static_cast<unsigned>(total4));
+#define FE_X4_NT(TYV, P2, XHV) hipLaunchKernelGGL((heat_exact_x4<TYV, P2, XHV>), grid4, dim3(64, TYV + 2), 0, s, a);
 #define FE_X4(TYV)                                                                                                     \
-    if (a.hh_pow2) {                                                                                                   \
-        hipLaunchKernelGGL((heat_exact_x4<TYV, true>), grid4, dim3(64, TYV + 2), 0, s, a);                             \
+    if (a.tiles_x > 1) {                                                                                               \
+        if (a.hh_pow2) {                                                                                               \
+            FE_X4_NT(TYV, true, true)                                                                                  \
+        } else {                                                                                                       \
+            FE_X4_NT(TYV, false, true)                                                                                 \
+        }                                                                                                              \
+    } else if (a.hh_pow2) {                                                                                            \
+        FE_X4_NT(TYV, true, false)                                                                                     \
     } else {                                                                                                           \
-        hipLaunchKernelGGL((heat_exact_x4<TYV, false>), grid4, dim3(64, TYV + 2), 0, s, a);                            \
+        FE_X4_NT(TYV, false, false)                                                                                    \
     }
         switch (ty) {
         case 2: FE_X4(2) break;
@@ -350,6 +400,7 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
         default: return -1;
         }
 #undef FE_X4
+#undef FE_X4_NT
         return static_cast<int>(hipGetLastError());
     }
     const dim3 grid(static_cast<unsigned>(total));
@@ -366,7 +417,11 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
 int fe_heat_handwritten(const float* in, float* out, int nx, int ny, int nz, double k, double h, double sub_dt)
 {
     if (nx % 4 == 0) {
-        return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 6, 8, 1, 2);
+        // Launch shapes measured on MI355X: small rows like many short blocks, long rows fewer, taller ones.
+        if (nx <= 256) {
+            return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 2, 32, 1, 2);
+        }
+        return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 14, 32, 0, 2);
     }
     return fe_heat_handwritten_cfg(in, out, nx, ny, nz, k, h, sub_dt, 4, 32, 1, 1);
 }
@@ -380,7 +435,8 @@ int fe_copy_f32(const float* a, float* b, size_t n)
     hipStream_t s = static_cast<hipStream_t>(fe_hip_stream());
     const std::size_t n4 = n / 4;
     if (n4 > 0) {
-        const unsigned blocks = static_cast<unsigned>(n4 / 256 < 4096 ? (n4 + 255) / 256 : 4096);
+        const std::size_t want = (n4 + 1023) / 1024;
+        const unsigned blocks = static_cast<unsigned>(want < 1 ? 1 : (want > 65536 ? 65536 : want));
         hipLaunchKernelGGL(copy_f32x4, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4*>(a),
             reinterpret_cast<float4*>(b), n4);
     }
