@@ -30,6 +30,26 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
 /* b[i] = a[i], n floats: the bandwidth roof the reference measures beside its stencil (bench/stencil_bench.cpp:41-45). */
 int fe_copy_f32(const float* a, float* b, size_t n);
 
+/* ---- the law compiler ------------------------------------------------------------------------------------------- */
+
+typedef struct fe_law fe_law;
+
+/* Compiles law sources (names label errors "name:line: message"; texts are the law files' contents). NULL when the law
+ * does not compile, with every error, one a line, in `errors`. The front end the reference's compile_law is
+ * (src/field/Law.hpp:204, src/field/Law.cpp:1662), lowered at once to the IR (src/field/ir/LawIR.hpp:537). */
+fe_law* fe_law_compile(const char* const* names, const char* const* texts, int count, char* errors, size_t error_bytes);
+void fe_law_free(fe_law* law);
+int fe_law_field_count(const fe_law* law);
+int fe_law_field_id(const fe_law* law, const char* name); /* -1 when absent (Law::field_id, src/field/Law.hpp:193) */
+const char* fe_law_field_name(const fe_law* law, int field);
+int fe_law_param_count(const fe_law* law);
+int fe_law_term_count(const fe_law* law);
+/* The IR as JSON in the reference's format (ir::to_json, src/field/ir/LawIR.hpp:542) and as a listing
+ * (ir::describe, src/field/ir/LawIR.hpp:544); free with fe_string_free. */
+char* fe_law_ir_json(const fe_law* law);
+char* fe_law_describe(const fe_law* law);
+void fe_string_free(char* s);
+
 #ifdef __cplusplus
 }
 #endif
