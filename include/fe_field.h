@@ -49,6 +49,44 @@ int fe_law_term_count(const fe_law* law);
 char* fe_law_ir_json(const fe_law* law);
 char* fe_law_describe(const fe_law* law);
 void fe_string_free(char* s);
+/* The generated gfx950 translation unit of a law (CudaSolver::source, src/field/cuda/CudaSolver.hpp:30; lawc --cuda,
+ * tools/lawc/main.cpp:221-224); NULL with the reason in `why` when the law uses an op the backend does not run. */
+char* fe_law_hip_source(const fe_law* law, char* why, size_t why_bytes);
+
+/* ---- the solver on the card ----------------------------------------------------------------------------------------- */
+
+typedef struct fe_solver fe_solver;
+
+/* A solver of `law` on an nx*ny*nz lattice of cell size h, already on the card: Solver(law, shape) + use_gpu()
+ * (src/field/Solver.hpp:111,156), the backend being HipSolver::create (the drop-in for CudaSolver::create,
+ * src/field/cuda/CudaSolver.hpp:24). NULL with the reason in `why` when there is no device or the law cannot run here.
+ * Fields start at their declared init. */
+fe_solver* fe_solver_create(const fe_law* law, int nx, int ny, int nz, double h, char* why, size_t why_bytes);
+void fe_solver_free(fe_solver* s);
+/* A whole field from the host (nx*ny*nz floats, x fastest): Solver::fields() written (src/field/Solver.hpp:119-124). */
+int fe_solver_set_field(fe_solver* s, int field, const float* values);
+/* A whole field to the host, fetched from the card when the readback is lazy (Solver::values, Solver.hpp:135). */
+int fe_solver_get_field(fe_solver* s, int field, float* out);
+int fe_solver_set_param(fe_solver* s, const char* name, double value); /* -1: no such param (Solver.hpp:143) */
+/* One step of dt: every term in order (substeps from max_dt), then every derive (Solver::step, Solver.hpp:149). 0, or
+ * nonzero with the reason in fe_solver_error(). Contract breaches are not errors: see fe_solver_violations_json. */
+int fe_solver_step(fe_solver* s, double dt);
+int fe_solver_refresh_derived(fe_solver* s);               /* Solver::refresh_derived (Solver.hpp:148) */
+int fe_solver_set_lazy_readback(fe_solver* s, int on);     /* Solver::set_lazy_readback (Solver.hpp:138) */
+int fe_solver_check_contracts_every(fe_solver* s, int steps); /* Solver::check_contracts_every (Solver.hpp:171) */
+int fe_solver_set_learned(fe_solver* s, int on);           /* Solver::set_learned (Solver.hpp:165) */
+int fe_solver_set_respect_max_dt(fe_solver* s, int on);    /* Solver::set_respect_max_dt (Solver.hpp:179) */
+/* The last step's violations as a JSON array of {kind, term, field, detail} (Solver::violations, Solver.hpp:173). */
+char* fe_solver_violations_json(fe_solver* s);
+int fe_solver_last_substeps(fe_solver* s, int* out, int count);          /* Solver.hpp:175; returns the term count */
+int fe_solver_last_changed(fe_solver* s, unsigned char* out, int count); /* Solver.hpp:151; returns the field count */
+/* The last step's StepProfile (src/field/Backend.hpp:43-65) as JSON: card ms per term from events, launches. */
+char* fe_solver_profile_json(fe_solver* s);
+/* FNV-1a over every field's bits (FieldSet::digest, src/field/Grid.hpp:341-343), fetching first when lazy. */
+uint64_t fe_solver_digest(fe_solver* s);
+const char* fe_solver_error(fe_solver* s);
+/* The device array holding a field's current cells (valid until the next step swaps it). */
+void* fe_solver_field_device(fe_solver* s, int field);
 
 #ifdef __cplusplus
 }

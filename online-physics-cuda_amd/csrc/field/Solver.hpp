@@ -1,0 +1,69 @@
+// Integrates a compiled law on one lattice, on the card.
+//
+// The reference's Solver API (src/field/Solver.hpp:103-215) for the calls that drive a backend: fields are the host's
+// truth between steps (edits go up, results come back, lazily if asked), a step is every term in declaration order
+// then every derive, contracts are reported as Violations. This build has no CPU integrator: a Solver runs only once
+// use_gpu() has succeeded, and step() refuses to run otherwise (the CPU path is the oracle's, under oracle/).
+#pragma once
+
+#include "field/Backend.hpp"
+#include "field/Law.hpp"
+
+#include <memory>
+#include <optional>
+#include <string>
+#include <string_view>
+#include <vector>
+
+namespace fieldengine::field {
+
+class Solver {
+public:
+    Solver(const Law& law, GridShape shape);
+
+    [[nodiscard]] FieldSet& fields(); // every field may change: all go up before the next step
+    [[nodiscard]] const FieldSet& fields_const();
+    [[nodiscard]] const std::vector<float>& values(int field); // fetches the one field when lazy
+    float& edit_cell(int field, std::size_t cell);
+    void mark_field_edited(int field);
+
+    bool set_param(std::string_view name, double value);
+    [[nodiscard]] std::optional<double> param(std::string_view name) const;
+    [[nodiscard]] const Law& law() const { return law_; }
+    [[nodiscard]] Law& law_for_training() { return law_; }
+
+    bool use_gpu(std::string& why);
+    [[nodiscard]] bool on_gpu() const { return gpu_ != nullptr; }
+    [[nodiscard]] LawBackend* backend() const { return gpu_.get(); }
+
+    void refresh_derived();
+    void step(double dt);
+    void set_lazy_readback(bool on);
+    [[nodiscard]] bool lazy_readback() const { return lazy_; }
+    void check_contracts_every(int steps);
+    void set_learned(bool on) { learned_ = on; }
+    void set_respect_max_dt(bool respect) { respect_max_dt_ = respect; }
+
+    [[nodiscard]] const std::vector<Violation>& violations() const { return violations_; }
+    [[nodiscard]] const std::vector<int>& last_substeps() const { return substeps_; }
+    [[nodiscard]] const std::vector<char>& last_changed() const { return changed_; }
+    [[nodiscard]] const StepProfile& profile() const;
+
+private:
+    void fetch(const std::vector<char>& which);
+
+    Law law_;
+    FieldSet fields_;
+    std::vector<char> stale_;
+    FieldEdits edits_;
+    bool lazy_ = false;
+    bool learned_ = false;
+    bool respect_max_dt_ = true;
+    std::vector<Violation> violations_;
+    std::vector<int> substeps_;
+    std::vector<char> changed_;
+    std::shared_ptr<LawBackend> gpu_;
+    StepProfile empty_profile_;
+};
+
+} // namespace fieldengine::field

@@ -1,0 +1,640 @@
+// The --hip generator. See HipCodegen.hpp.
+#include "field/hip/HipCodegen.hpp"
+
+#include "kernels/exact.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <set>
+#include <sstream>
+
+namespace fieldengine::field::hip {
+
+namespace {
+
+// The generic kernels every law module has: contracts (range and finite, clamping, conserved totals), the reduction
+// passes' final fold and the change flags. All reductions are fixed-order: a block sums a fixed range of cells, each
+// lane a fixed stride of it, the lanes in a fixed tree; the partials fold in one block, in order. No float atomics.
+const char* kPrelude = R"HIP(
+typedef long long i64;
+typedef unsigned long long u64;
+#define IDX(x, y, z) ((i64)(z) * k.nxy + (i64)(y) * k.nx + (i64)(x))
+#define AT(slot, x, y, z) k.A[(i64)(slot) * k.cells + IDX(x, y, z)]
+#define CELL_HEAD const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x); const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y); \
+    const int z = (int)blockIdx.z; if (x >= k.nx || y >= k.ny) return;
+__device__ __forceinline__ int ex(const K& k, int v) { return v < 0 ? 0 : (v >= k.nx ? k.nx - 1 : v); }
+__device__ __forceinline__ int ey(const K& k, int v) { return v < 0 ? 0 : (v >= k.ny ? k.ny - 1 : v); }
+__device__ __forceinline__ int ez(const K& k, int v) { return v < 0 ? 0 : (v >= k.nz ? k.nz - 1 : v); }
+__device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
+// The cells i along an axis whose clamp(i + d) is j (one run: j - d inside, and what is clamped onto an edge cell).
+__device__ __forceinline__ void run_to(int j, int d, int n, int& lo, int& hi)
+{
+    lo = j == 0 ? 0 : j - d; if (lo < 0) lo = 0;
+    hi = j == n - 1 ? n - 1 : j - d; if (hi > n - 1) hi = n - 1;
+}
+// "Field t changed": one store a wave at most, by its first lane that saw a change.
+__device__ __forceinline__ void mark(const K& k, int t, bool changed)
+{
+    const u64 m = __ballot(changed);
+    if (m != 0ull && (int)(threadIdx.x & 63u) == __ffsll((long long)m) - 1) k.C[t] = 1u;
+}
+// Neumaier's compensated sum in double.
+struct Sum { double s; double c; };
+__device__ __forceinline__ void add(Sum& a, double v)
+{
+    const double t = a.s + v;
+    a.c += fabs(a.s) >= fabs(v) ? (a.s - t) + v : (v - t) + a.s;
+    a.s = t;
+}
+__device__ __forceinline__ void merge(Sum& a, const Sum& b) { add(a, b.s); a.c += b.c; }
+
+// A field's range contract (src/field/Solver.cpp:463-511): counts of non-finite and out-of-range cells and the least
+// index of a bad one (integer atomics: the order does not matter). A clamped field's finite strays are clamped instead.
+extern "C" __global__ void k_range(K k, int field, double lo, double hi, int clamp, unsigned* rec, u64* first)
+{
+    CELL_HEAD
+    const i64 i = IDX(x, y, z);
+    const float v = k.now[field][i];
+    const double d = (double)v;
+    const bool fin = finite_d(d);
+    bool bad = !fin;
+    if (fin && (d < lo || d > hi)) {
+        if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
+        else bad = true;
+    }
+    if (bad) {
+        atomicAdd(&rec[fin ? 0 : 1], 1u);
+        atomicMin(first, (u64)i);
+    }
+}
+
+// Conserved totals, pass 1: block b sums cells [b * chunk, (b + 1) * chunk), lane t every 256th from t; then the lanes
+// fold in a fixed tree. out[2b], out[2b+1]: the sum and the sum of magnitudes (each a compensated pair, rounded).
+extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 chunk, double* out)
+{
+    __shared__ double ss[256], sc[256], ms[256], mc[256];
+    const int t = (int)threadIdx.x;
+    const i64 begin = (i64)blockIdx.x * chunk;
+    i64 end = begin + chunk; if (end > k.cells) end = k.cells;
+    Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
+    const float* v = k.now[field];
+    for (i64 i = begin + t; i < end; i += 256) { const double d = (double)v[i]; add(s, d); add(m, fabs(d)); }
+    ss[t] = s.s; sc[t] = s.c; ms[t] = m.s; mc[t] = m.c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            Sum a = {ss[t], sc[t]}; merge(a, Sum{ss[t + w], sc[t + w]}); ss[t] = a.s; sc[t] = a.c;
+            Sum b = {ms[t], mc[t]}; merge(b, Sum{ms[t + w], mc[t + w]}); ms[t] = b.s; mc[t] = b.c;
+        }
+        __syncthreads();
+    }
+    if (t == 0) { out[2 * blockIdx.x] = ss[0] + sc[0]; out[2 * blockIdx.x + 1] = ms[0] + mc[0]; }
+}
+
+// Pass 2: one block folds `parts` pairs in order into dst[0], dst[1].
+extern "C" __global__ __launch_bounds__(256) void k_total_final(const double* parts, int count, double* dst)
+{
+    __shared__ double ss[256], sc[256], ms[256], mc[256];
+    const int t = (int)threadIdx.x;
+    Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
+    for (int i = t; i < count; i += 256) { add(s, parts[2 * i]); add(m, parts[2 * i + 1]); }
+    ss[t] = s.s; sc[t] = s.c; ms[t] = m.s; mc[t] = m.c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            Sum a = {ss[t], sc[t]}; merge(a, Sum{ss[t + w], sc[t + w]}); ss[t] = a.s; sc[t] = a.c;
+            Sum b = {ms[t], mc[t]}; merge(b, Sum{ms[t + w], mc[t + w]}); ms[t] = b.s; mc[t] = b.c;
+        }
+        __syncthreads();
+    }
+    if (t == 0) { dst[0] = ss[0] + sc[0]; dst[1] = ms[0] + mc[0]; }
+}
+
+// A reduction's partials (sum, least, most per block, from k_reduce passes) folded in order into U[uslot].
+extern "C" __global__ __launch_bounds__(256) void k_reduce_final(K k, int kind, int parts, int uslot)
+{
+    __shared__ double ss[256], sc[256], lo[256], hi[256];
+    const int t = (int)threadIdx.x;
+    Sum s = {0.0, 0.0}; double a = k.L[1]; double b = k.L[2];
+    for (int i = t; i < parts; i += 256) {
+        add(s, k.L[3 * i]);
+        a = k.L[3 * i + 1] < a ? k.L[3 * i + 1] : a;
+        b = b < k.L[3 * i + 2] ? k.L[3 * i + 2] : b;
+    }
+    ss[t] = s.s; sc[t] = s.c; lo[t] = a; hi[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            Sum p = {ss[t], sc[t]}; merge(p, Sum{ss[t + w], sc[t + w]}); ss[t] = p.s; sc[t] = p.c;
+            lo[t] = lo[t + w] < lo[t] ? lo[t + w] : lo[t];
+            hi[t] = hi[t] < hi[t + w] ? hi[t + w] : hi[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) ((double*)k.U)[uslot] = kind == 0 ? ss[0] + sc[0] : (kind == 1 ? hi[0] : lo[0]);
+}
+)HIP";
+
+std::string literal(double v)
+{
+    if (std::isnan(v)) {
+        return "__builtin_nan(\"\")";
+    }
+    if (std::isinf(v)) {
+        return v > 0 ? "__builtin_inf()" : "(-__builtin_inf())";
+    }
+    char text[64];
+    std::snprintf(text, sizeof(text), "%a", v); // exact: a hexadecimal double literal
+    return std::string("(") + text + ")";
+}
+
+class Generator {
+public:
+    Generator(const ir::Module& m, const Law& law, LawLayout& lay) : m_(m), law_(law), lay_(lay) {}
+
+    std::string run(std::string& why)
+    {
+        for (std::size_t id = 0; id < m_.insts.size(); ++id) {
+            const Op op = m_.insts[id].op;
+            if (op == Op::Distance || op == Op::Poisson || op == Op::Fall || op == Op::Spread || op == Op::Topple ||
+                op == Op::Mlp || op == Op::Flux || op == Op::LDiffuse || op == Op::Local) {
+                if (used(static_cast<int>(id))) {
+                    why = std::string("the HIP backend does not run ") + ir::op_name(op) + " yet";
+                    return "";
+                }
+            }
+        }
+        allocate();
+        lay_.fields = static_cast<int>(law_.fields.size());
+        const int nf = std::max(1, lay_.fields);
+        out_ << "// generated by the Field Engine's --hip emitter for gfx950\n";
+        out_ << FE_EXACT_HELPERS_TEXT;
+        out_ << "struct K {\n    float* now[" << nf << "];\n    float* nxt[" << nf << "];\n"
+             << "    const double* P;\n    const double* U;\n    float* A;\n    double* S;\n    double* L;\n    unsigned* C;\n"
+             << "    long long nxy;\n    long long cells;\n    double h;\n    double dt;\n    int nx;\n    int ny;\n    int nz;\n"
+             << "    int pad;\n};\n";
+        lay_.kernel_arg_bytes = (2 * nf + 6) * 8 + 4 * 8 + 4 * 4;
+        out_ << kPrelude;
+        for (std::size_t id = 0; id < m_.insts.size(); ++id) {
+            if (used(static_cast<int>(id))) {
+                emit(static_cast<int>(id));
+            }
+        }
+        for (const auto& [id, stage] : lay_.stages) {
+            stage_kernel(id);
+        }
+        for (std::size_t t = 0; t < m_.terms.size(); ++t) {
+            lay_.terms.push_back(program(m_.terms[t], "t" + std::to_string(t), false));
+            lay_.learned.push_back(program(m_.learned[t], "l" + std::to_string(t), false));
+        }
+        for (std::size_t d = 0; d < m_.derives.size(); ++d) {
+            lay_.derives.push_back(program(m_.derives[d], "d" + std::to_string(d), true));
+        }
+        return out_.str();
+    }
+
+private:
+    [[nodiscard]] const ir::Inst& at(int id) const { return m_.insts[static_cast<std::size_t>(id)]; }
+    [[nodiscard]] bool scalar(int id) const { return at(id).shape == ir::Shape::Scalar; }
+
+    // Whether any program reaches the instruction (folding leaves dead ones behind).
+    bool used(int id)
+    {
+        if (live_.empty()) {
+            live_.assign(m_.insts.size(), 0);
+            std::vector<int> todo;
+            auto root = [&](int v) {
+                if (v >= 0) {
+                    todo.push_back(v);
+                }
+            };
+            for (const auto* progs : {&m_.terms, &m_.learned, &m_.derives}) {
+                for (const ir::Program& p : *progs) {
+                    for (const ir::Equation& e : p.equations) {
+                        root(e.value);
+                    }
+                    root(p.max_dt);
+                }
+            }
+            while (!todo.empty()) {
+                const int v = todo.back();
+                todo.pop_back();
+                if (live_[static_cast<std::size_t>(v)]) {
+                    continue;
+                }
+                live_[static_cast<std::size_t>(v)] = 1;
+                for (const int a : at(v).args) {
+                    todo.push_back(a);
+                }
+            }
+        }
+        return live_[static_cast<std::size_t>(id)] != 0;
+    }
+
+    void allocate()
+    {
+        lay_.slot.assign(m_.insts.size(), -1);
+        lay_.uslot.assign(m_.insts.size(), -1);
+        std::set<int> stages;
+        for (const auto* progs : {&m_.terms, &m_.learned, &m_.derives}) {
+            for (const ir::Program& p : *progs) {
+                stages.insert(p.stages.begin(), p.stages.end());
+                stages.insert(p.limit_stages.begin(), p.limit_stages.end());
+            }
+        }
+        for (const int id : stages) {
+            const ir::Inst& in = at(id);
+            StageCode s;
+            s.inst = id;
+            s.kernel = "st" + std::to_string(id);
+            if (in.op == Op::MaxOf || in.op == Op::MinOf || in.op == Op::SumOf) {
+                s.kind = StageCode::Kind::Reduce;
+                s.reduce = in.op;
+                lay_.uslot[static_cast<std::size_t>(id)] = lay_.uniforms++;
+            } else if (in.op == Op::Tridiag) {
+                s.kind = StageCode::Kind::Tridiag;
+                s.axis = in.index;
+                lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
+            } else {
+                s.kind = StageCode::Kind::Stored;
+                lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
+            }
+            lay_.stages.emplace(id, s);
+        }
+    }
+
+    static std::string v(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z")
+    {
+        return "v" + std::to_string(id) + "(k, " + x + ", " + y + ", " + z + ")";
+    }
+
+    // An argument as a double at a point: a scalar as computed, a cell value widened from its float.
+    std::string d(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z") const
+    {
+        if (scalar(id)) {
+            return "s" + std::to_string(id) + "(k)";
+        }
+        return "(double)" + v(id, x, y, z);
+    }
+
+    void emit(int id)
+    {
+        const ir::Inst& in = at(id);
+        const std::string n = std::to_string(id);
+        std::ostringstream f;
+        if (in.shape == ir::Shape::Scalar) {
+            f << "__device__ __forceinline__ double s" << n << "(const K& k) { ";
+            switch (in.op) {
+            case Op::Const: f << "return " << literal(in.number) << ";"; break;
+            case Op::Param: f << "return k.P[" << in.index << "];"; break;
+            case Op::Cell: f << "return k.h;"; break;
+            case Op::Dt: f << "return k.dt;"; break;
+            case Op::Local: f << "return 0.0;"; break;
+            case Op::MaxOf: case Op::MinOf: case Op::SumOf: {
+                const int u = lay_.uslot[static_cast<std::size_t>(id)];
+                f << "return " << (u >= 0 ? "k.U[" + std::to_string(u) + "]" : std::string("0.0")) << ";";
+                break;
+            }
+            default: f << "return " << arith(in, id) << ";"; break;
+            }
+            f << " }\n";
+            out_ << f.str();
+            return;
+        }
+        const auto stage = lay_.stages.find(id);
+        const bool stored = stage != lay_.stages.end() && stage->second.kind == StageCode::Kind::Stored;
+        const std::string head = "__device__ __forceinline__ float " + std::string(stored ? "b" : "v") + n +
+            "(const K& k, int x, int y, int z)";
+        f << head << "\n{\n";
+        switch (in.op) {
+        case Op::Field:
+            f << "    return k.now[" << in.index << "][IDX(x, y, z)];\n";
+            break;
+        case Op::Rim:
+            f << "    return (x == 0 || y == 0 || z == 0 || x + 1 == k.nx || y + 1 == k.ny || z + 1 == k.nz) ? 1.0f : 0.0f;\n";
+            break;
+        case Op::Shift:
+            f << "    return " << v(in.args[0], "ex(k, x + (" + std::to_string(in.offset[0]) + "))",
+                "ey(k, y + (" + std::to_string(in.offset[1]) + "))", "ez(k, z + (" + std::to_string(in.offset[2]) + "))")
+              << ";\n";
+            break;
+        case Op::Mul: {
+            // Zero times anything is zero on the CPU, without looking at the anything (src/field/Law.cpp:804-843).
+            const int a = in.args[0];
+            const int b = in.args[1];
+            if (scalar(a) || scalar(b)) {
+                const int s = scalar(a) ? a : b;
+                f << "    const double s = s" << s << "(k);\n    if (s == 0.0) return 0.0f;\n";
+            }
+            f << "    return fe_rf(" << d(a) << " * " << d(b) << ");\n";
+            break;
+        }
+        case Op::Lap:
+            // (xm + xp + ym + yp + zm + zp - 6 c) / h^2 in double, left to right; not finite -> 0.
+            f << "    const double c = " << d(in.args[0]) << ";\n"
+              << "    double s = " << d(in.args[0], "ex(k, x - 1)") << " + " << d(in.args[0], "ex(k, x + 1)") << ";\n"
+              << "    s = s + " << d(in.args[0], "x", "ey(k, y - 1)") << ";\n"
+              << "    s = s + " << d(in.args[0], "x", "ey(k, y + 1)") << ";\n"
+              << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z - 1)") << ";\n"
+              << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z + 1)") << ";\n"
+              << "    const double o = (s - 6.0 * c) / (k.h * k.h);\n"
+              << "    return fe_rf(finite_d(o) ? o : 0.0);\n";
+            break;
+        case Op::Gx: case Op::Gy: case Op::Gz: {
+            const int axis = in.op == Op::Gx ? 0 : (in.op == Op::Gy ? 1 : 2);
+            const char* c = axis == 0 ? "x" : (axis == 1 ? "y" : "z");
+            const char* cnt = axis == 0 ? "k.nx" : (axis == 1 ? "k.ny" : "k.nz");
+            const std::string hi = std::string(c) + " + 1 < " + cnt + " ? " + c + " + 1 : " + c;
+            const std::string lo = std::string(c) + " > 0 ? " + c + " - 1 : " + c;
+            f << "    const int hi = " << hi << ";\n    const int lo = " << lo << ";\n";
+            const std::string ahi = axis == 0 ? d(in.args[0], "hi") : (axis == 1 ? d(in.args[0], "x", "hi") : d(in.args[0], "x", "y", "hi"));
+            const std::string alo = axis == 0 ? d(in.args[0], "lo") : (axis == 1 ? d(in.args[0], "x", "lo") : d(in.args[0], "x", "y", "lo"));
+            const std::string body = "(" + ahi + " - " + alo + ") / (k.h * (double)(hi - lo))";
+            if (axis == 0) {
+                f << "    const double o = " << body << ";\n";
+            } else {
+                f << "    const double o = " << cnt << " > 1 ? " << body << " : 0.0;\n";
+            }
+            f << "    return fe_rf(finite_d(o) ? o : 0.0);\n";
+            break;
+        }
+        case Op::Diffuse: {
+            // div(k grad F) in flux form: faces -x +x -y +y -z +z, each kf (F[nb] - F[c]) with the harmonic mean
+            // kf = 2 ka kb / (ka + kb) (src/field/Law.cpp:986-1018), a missing face skipped.
+            const int kk = in.args[0];
+            const int fa = in.args[1];
+            f << "    const double c = " << d(fa) << ";\n    double acc = 0.0;\n";
+            if (scalar(kk)) {
+                f << "    const double kv = s" << kk << "(k);\n"
+                  << "    const double kf = (kv + kv) > 0.0 ? 2.0 * kv * kv / (kv + kv) : 0.0;\n";
+            } else {
+                f << "    const double ka = " << d(kk) << ";\n";
+            }
+            const char* sides[6][4] = {{"x > 0", "x - 1", "y", "z"}, {"x + 1 < k.nx", "x + 1", "y", "z"},
+                {"y > 0", "x", "y - 1", "z"}, {"y + 1 < k.ny", "x", "y + 1", "z"}, {"z > 0", "x", "y", "z - 1"},
+                {"z + 1 < k.nz", "x", "y", "z + 1"}};
+            for (const auto& s : sides) {
+                f << "    if (" << s[0] << ") { ";
+                if (!scalar(kk)) {
+                    f << "const double kb = " << d(kk, s[1], s[2], s[3]) << "; const double kf = (ka + kb) > 0.0 ? 2.0 * ka * kb / (ka + kb) : 0.0; ";
+                }
+                f << "acc += kf * (" << d(fa, s[1], s[2], s[3]) << " - c); }\n";
+            }
+            f << "    return fe_rf(acc / (k.h * k.h));\n";
+            break;
+        }
+        case Op::Adv: {
+            // -(u . grad F), upwind (src/field/Law.cpp:1020-1040).
+            const int fa = in.args[0];
+            f << "    const double c = " << d(fa) << ";\n"
+              << "    const double u = " << d(in.args[1]) << ";\n    const double w = " << d(in.args[2]) << ";\n"
+              << "    const double s = " << d(in.args[3]) << ";\n    double o = 0.0;\n";
+            auto up = [&](const char* speed, const char* c, const char* cnt, const std::string& lo, const std::string& hi) {
+                f << "    { const double q = " << speed << " > 0.0 ? (" << c << " > 0 ? " << speed << " * (c - " << lo
+                  << ") / k.h : 0.0) : (" << c << " + 1 < " << cnt << " ? " << speed << " * (" << hi << " - c) / k.h : 0.0);\n";
+            };
+            up("u", "x", "k.nx", d(fa, "ex(k, x - 1)"), d(fa, "ex(k, x + 1)"));
+            f << "      o = q; }\n";
+            up("w", "y", "k.ny", d(fa, "x", "ey(k, y - 1)"), d(fa, "x", "ey(k, y + 1)"));
+            f << "      o = o + q; }\n";
+            up("s", "z", "k.nz", d(fa, "x", "y", "ez(k, z - 1)"), d(fa, "x", "y", "ez(k, z + 1)"));
+            f << "      o = o + q; }\n";
+            f << "    return fe_rf(-o);\n";
+            break;
+        }
+        case Op::ShiftT: {
+            f << "    int lx, hx, ly, hy, lz, hz;\n"
+              << "    run_to(x, " << in.offset[0] << ", k.nx, lx, hx); run_to(y, " << in.offset[1] << ", k.ny, ly, hy); run_to(z, "
+              << in.offset[2] << ", k.nz, lz, hz);\n"
+              << "    double s = 0.0;\n"
+              << "    for (int zz = lz; zz <= hz; ++zz) for (int yy = ly; yy <= hy; ++yy) for (int xx = lx; xx <= hx; ++xx) s += "
+              << d(in.args[0], "xx", "yy", "zz") << ";\n"
+              << "    return fe_rf(s);\n";
+            break;
+        }
+        case Op::GxT: case Op::GyT: case Op::GzT: {
+            const int axis = in.op == Op::GxT ? 0 : (in.op == Op::GyT ? 1 : 2);
+            const char* c = axis == 0 ? "x" : (axis == 1 ? "y" : "z");
+            const char* cnt = axis == 0 ? "k.nx" : (axis == 1 ? "k.ny" : "k.nz");
+            const std::string aq = axis == 0 ? d(in.args[0], "q") : (axis == 1 ? d(in.args[0], "x", "q") : d(in.args[0], "x", "y", "q"));
+            f << "    double s = 0.0;\n"
+              << "    for (int dd = 1; dd >= -1; dd -= 2) { int lo, hi; run_to(" << c << ", dd, " << cnt << ", lo, hi);\n"
+              << "        for (int q = lo; q <= hi; ++q) {\n"
+              << "            const int span = (q + 1 < " << cnt << " - 1 ? q + 1 : " << cnt << " - 1) - (q - 1 > 0 ? q - 1 : 0);\n"
+              << "            const double w = span > 0 ? " << aq << " / (k.h * span) : 0.0;\n"
+              << "            s += dd > 0 ? w : -w; } }\n"
+              << "    return fe_rf(s);\n";
+            break;
+        }
+        case Op::Tridiag:
+            f << "    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n";
+            break;
+        default:
+            f << "    return fe_rf(" << arith(in, id) << ");\n";
+            break;
+        }
+        f << "}\n";
+        out_ << f.str();
+        if (stored) {
+            out_ << "__device__ __forceinline__ float v" << n << "(const K& k, int x, int y, int z) { return AT("
+                 << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z); }\n";
+        }
+    }
+
+    // An element-wise op in double over its arguments at (x, y, z), as the CPU's lambdas compute it
+    // (src/field/Law.cpp:782-857): std::min/std::max/std::clamp comparisons, not fmin/fmax.
+    std::string arith(const ir::Inst& in, int id) const
+    {
+        auto a = [&](std::size_t i) { return d(in.args[i]); };
+        switch (in.op) {
+        case Op::Neg: return "(-" + a(0) + ")";
+        case Op::Abs: return "fabs(" + a(0) + ")";
+        case Op::Sqrt: return "([](double q) { return sqrt(0.0 < q ? q : 0.0); })(" + a(0) + ")";
+        case Op::Exp: return "([](double q) { return exp(80.0 < q ? 80.0 : q); })(" + a(0) + ")";
+        case Op::Log: return "([](double q) { return log(q < 1e-30 ? 1e-30 : q); })(" + a(0) + ")";
+        case Op::Step: return "(" + a(0) + " > 0.0 ? 1.0 : 0.0)";
+        case Op::Pos: return "([](double q) { return q < 0.0 ? 0.0 : q; })(" + a(0) + ")";
+        case Op::Add: return "(" + a(0) + " + " + a(1) + ")";
+        case Op::Sub: return "(" + a(0) + " - " + a(1) + ")";
+        case Op::Mul: return "([](double p, double q) { return p == 0.0 || q == 0.0 ? 0.0 : p * q; })(" + a(0) + ", " + a(1) + ")";
+        case Op::Div: return "([](double p, double q) { return q == 0.0 ? 0.0 : p / q; })(" + a(0) + ", " + a(1) + ")";
+        case Op::Pow: return "pow(" + a(0) + ", " + a(1) + ")";
+        case Op::Min: return "fe_min(" + a(0) + ", " + a(1) + ")";
+        case Op::Max: return "fe_max(" + a(0) + ", " + a(1) + ")";
+        case Op::Clamp: return "([](double q, double lo, double hi) { return fe_clamp(q, lo, fe_max(lo, hi)); })(" + a(0) + ", " +
+            a(1) + ", " + a(2) + ")";
+        default:
+            (void)id;
+            return "0.0";
+        }
+    }
+
+    void stage_kernel(int id)
+    {
+        const StageCode& s = lay_.stages.at(id);
+        const ir::Inst& in = at(id);
+        const int slot = lay_.slot[static_cast<std::size_t>(id)];
+        switch (s.kind) {
+        case StageCode::Kind::Stored:
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD AT(" << slot << ", x, y, z) = b" << id
+                 << "(k, x, y, z); }\n";
+            break;
+        case StageCode::Kind::Reduce: {
+            // Pass 1: block b sums (or bounds) cells [b * chunk, ...), lane t every 256th; a fixed tree; L[3b..3b+2].
+            const int arg = in.args[0];
+            const int kind = in.op == Op::SumOf ? 0 : (in.op == Op::MaxOf ? 1 : 2);
+            out_ << "extern \"C\" __global__ __launch_bounds__(256) void " << s.kernel << "(K k, i64 chunk) {\n"
+                 << "    __shared__ double ss[256], sc[256], lo[256], hi[256];\n"
+                 << "    const int t = (int)threadIdx.x; const i64 begin = (i64)blockIdx.x * chunk; i64 end = begin + chunk;\n"
+                 << "    if (end > k.cells) end = k.cells;\n";
+            if (scalar(arg)) {
+                out_ << "    const double first = s" << arg << "(k);\n";
+            } else {
+                out_ << "    const double first = (double)v" << arg << "(k, 0, 0, 0);\n";
+            }
+            out_ << "    Sum s = {0.0, 0.0}; double a = first; double b = first;\n"
+                 << "    for (i64 i = begin + t; i < end; i += 256) {\n"
+                 << "        const int x = (int)(i % k.nx); const int y = (int)((i / k.nx) % k.ny); const int z = (int)(i / k.nxy);\n";
+            if (scalar(arg)) {
+                out_ << "        const double q = s" << arg << "(k);\n";
+            } else {
+                out_ << "        const double q = (double)v" << arg << "(k, x, y, z);\n";
+            }
+            out_ << "        add(s, q); a = q < a ? q : a; b = b < q ? q : b; }\n"
+                 << "    ss[t] = s.s; sc[t] = s.c; lo[t] = a; hi[t] = b; __syncthreads();\n"
+                 << "    for (int w = 128; w > 0; w >>= 1) {\n"
+                 << "        if (t < w) { Sum p = {ss[t], sc[t]}; merge(p, Sum{ss[t + w], sc[t + w]}); ss[t] = p.s; sc[t] = p.c;\n"
+                 << "            lo[t] = lo[t + w] < lo[t] ? lo[t + w] : lo[t]; hi[t] = hi[t] < hi[t + w] ? hi[t + w] : hi[t]; }\n"
+                 << "        __syncthreads(); }\n"
+                 << "    if (t == 0) { k.L[3 * blockIdx.x] = ss[0] + sc[0]; k.L[3 * blockIdx.x + 1] = lo[0]; k.L[3 * blockIdx.x + 2] = hi[0]; }\n"
+                 << "}\n";
+            (void)kind;
+            break;
+        }
+        case StageCode::Kind::Tridiag: {
+            // A thread a line: Thomas in double with the edge cell repeating past the ends (its neighbour's coefficient
+            // on the diagonal), as the CPU solves it (src/field/Law.cpp:1098-1138); c' and d' in the scratch S.
+            static const char* lines[3] = {"k.ny * k.nz", "k.nx * k.nz", "k.nx * k.ny"};
+            static const char* coords[3] = {"int x = 0; int y = (int)(line % k.ny); int z = (int)(line / k.ny); const int n = k.nx;",
+                "int x = (int)(line % k.nx); int y = 0; int z = (int)(line / k.nx); const int n = k.ny;",
+                "int x = (int)(line % k.nx); int y = (int)(line / k.nx); int z = 0; const int n = k.nz;"};
+            const char* c = s.axis == 0 ? "x" : (s.axis == 1 ? "y" : "z");
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n"
+                 << "    const i64 line = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (line >= (i64)" << lines[s.axis] << ") return;\n"
+                 << "    " << coords[s.axis] << "\n"
+                 << "    double cp = 0.0; double dp = 0.0;\n"
+                 << "    for (int i = 0; i < n; ++i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
+                 << "        const double a = " << d(in.args[0]) << "; const double u = " << d(in.args[2]) << ";\n"
+                 << "        const double b = " << d(in.args[1]) << " + (i == 0 ? a : 0.0) + (i + 1 == n ? u : 0.0);\n"
+                 << "        const double m = i == 0 ? b : b - a * cp;\n"
+                 << "        cp = i + 1 == n ? 0.0 : u / m;\n"
+                 << "        dp = (" << d(in.args[3]) << " - (i == 0 ? 0.0 : a * dp)) / m;\n"
+                 << "        k.S[q] = cp; k.S[k.cells + q] = dp; }\n"
+                 << "    double next = 0.0;\n"
+                 << "    for (int i = n - 1; i >= 0; --i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
+                 << "        next = k.S[k.cells + q] - k.S[q] * next; AT(" << slot << ", x, y, z) = fe_rf(next); }\n"
+                 << "}\n";
+            break;
+        }
+        }
+    }
+
+    ProgramCode program(const ir::Program& p, const std::string& tag, bool derive)
+    {
+        ProgramCode code;
+        code.stages = p.stages;
+        code.limit_stages = p.limit_stages;
+        code.max_dt = p.max_dt;
+        if (p.equations.empty()) {
+            return code;
+        }
+        code.gather = "g_" + tag;
+        std::ostringstream g;
+        if (derive) {
+            // out = float((accumulate ? double(out) : 0.0) + value) (src/field/Solver.cpp:252-268).
+            const ir::Equation& eq = p.equations[0];
+            const bool reads_itself = std::find(p.reads.begin(), p.reads.end(), eq.target) != p.reads.end();
+            code.in_place = !reads_itself;
+            const std::string t = std::to_string(eq.target);
+            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const i64 i = IDX(x, y, z);\n"
+              << "    const double val = " << d(eq.value) << ";\n    const float old = k.now[" << t << "][i];\n"
+              << "    const float nv = fe_rf(" << (p.accumulate ? "(double)old" : "0.0") << " + val);\n"
+              << "    const bool ch = __float_as_uint(old) != __float_as_uint(nv);\n";
+            if (code.in_place) {
+                g << "    if (ch) k.now[" << t << "][i] = nv;\n";
+            } else {
+                g << "    k.nxt[" << t << "][i] = nv;\n";
+                code.swaps.push_back(eq.target);
+            }
+            g << "    mark(k, " << t << ", ch);\n}\n";
+            out_ << g.str();
+            return code;
+        }
+        code.in_place = p.in_place;
+        g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const i64 i = IDX(x, y, z);\n";
+        // Every rate first, from the pre-state; then each target's deltas in equation order.
+        for (std::size_t e = 0; e < p.equations.size(); ++e) {
+            g << "    const double r" << e << " = " << d(p.equations[e].value) << ";\n";
+        }
+        std::vector<int> targets;
+        for (const ir::Equation& eq : p.equations) {
+            if (std::find(targets.begin(), targets.end(), eq.target) == targets.end()) {
+                targets.push_back(eq.target);
+            }
+        }
+        for (const int target : targets) {
+            const std::string t = std::to_string(target);
+            g << "    { const float o = k.now[" << t << "][i]; float n = o;\n";
+            for (std::size_t e = 0; e < p.equations.size(); ++e) {
+                if (p.equations[e].target != target) {
+                    continue;
+                }
+                if (scalar(p.equations[e].value)) {
+                    // A rate that is one number and 0 is idle on the CPU: nothing is added.
+                    g << "      if (r" << e << " != 0.0) n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                } else {
+                    g << "      n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                }
+            }
+            g << "      const bool ch = __float_as_uint(o) != __float_as_uint(n);\n";
+            if (code.in_place) {
+                g << "      if (ch) k.now[" << t << "][i] = n;\n";
+            } else {
+                g << "      k.nxt[" << t << "][i] = n;\n";
+            }
+            g << "      mark(k, " << t << ", ch); }\n";
+        }
+        g << "}\n";
+        if (!code.in_place) {
+            code.swaps = targets;
+        }
+        out_ << g.str();
+        return code;
+    }
+
+    const ir::Module& m_;
+    const Law& law_;
+    LawLayout& lay_;
+    std::ostringstream out_;
+    std::vector<char> live_;
+};
+
+} // namespace
+
+std::string generate(const ir::Module& m, const Law& law, LawLayout& layout, std::string& why)
+{
+    layout = LawLayout{};
+    return Generator(m, law, layout).run(why);
+}
+
+std::vector<std::string> compile_options()
+{
+    // -O3; no fused multiply-adds (the CPU reference has none, and its roundings are what the kernels reproduce);
+    // f32 denormals flushed by the hardware (the CPU's FTZ/DAZ threads); packed FP32 off (some packed forms have
+    // given wrong results on this hardware).
+    return {"-O3", "-std=c++17", "-ffp-contract=off", "-fgpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang",
+        "-packed-fp32-ops"};
+}
+
+} // namespace fieldengine::field::hip

@@ -1,0 +1,63 @@
+// From the law's IR to one HIP translation unit for gfx950, compiled at run time by hipRTC.
+//
+// The reference's generator (src/field/cuda/CudaCodegen.hpp:12-55) emits float arithmetic; this one emits the reference
+// CPU's: every cell-shaped instruction is a device function that computes in double from its arguments (floats widened,
+// scalars kept in double) and rounds once to float, the way the CPU evaluator stores each node (src/field/Law.cpp:757-
+// 1466); scalar instructions stay double. The gather kernel of a term applies each equation as the CPU integrator does:
+// d = float(double(rate) * sub_dt), T = T + d (src/field/Solver.cpp:391-427). So a generated step is the CPU's step to
+// the bit, on every law whose values are not identically zero over the whole lattice (the CPU turns such an array into
+// the scalar 0, src/field/Law.cpp:554-563, which changes nothing but signed zeros) and whose ops are exact in IEEE
+// arithmetic or correctly rounded in both libraries (exp, log and pow are within one double ulp on both sides, so their
+// float results agree unless a value falls within that of a float rounding boundary).
+//
+// Kernel arguments are one struct by value (the field table of now/next buffers, the lattice, the device buffers),
+// so a step has no constant-memory writes between launches and can be captured in a graph.
+#pragma once
+
+#include "field/Law.hpp"
+#include "field/ir/LawIR.hpp"
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace fieldengine::field::hip {
+
+struct StageCode {
+    enum class Kind { Stored, Reduce, Tridiag };
+    Kind kind = Kind::Stored;
+    int inst = -1;
+    std::string kernel;     // the stage's kernel (Reduce: its per-block pass; the final pass is k_reduce_final)
+    int axis = 0;           // Tridiag: the axis its lines run along
+    Op reduce = Op::SumOf;
+};
+
+struct ProgramCode {
+    std::string gather;     // empty: no equations
+    bool in_place = false;
+    std::vector<int> swaps; // ping-pong: the fields whose two buffers swap after the kernel
+    std::vector<int> stages;
+    std::vector<int> limit_stages;
+    int max_dt = -1;
+};
+
+struct LawLayout {
+    int fields = 0;
+    int aux_slots = 0;      // float slots of A, one lattice each
+    int uniforms = 0;       // doubles of U (reductions)
+    std::vector<int> slot;  // per instruction: its slot in A, or -1
+    std::vector<int> uslot; // per instruction: its value in U, or -1
+    std::map<int, StageCode> stages;
+    std::vector<ProgramCode> terms;
+    std::vector<ProgramCode> learned;
+    std::vector<ProgramCode> derives;
+    int kernel_arg_bytes = 0; // sizeof the generated argument struct
+};
+
+// The translation unit of a law, or "" with `why` when it uses an op this backend does not run yet.
+std::string generate(const ir::Module& m, const Law& law, LawLayout& layout, std::string& why);
+
+// hipRTC options every generated source is compiled with (see the prelude for why each is there).
+std::vector<std::string> compile_options();
+
+} // namespace fieldengine::field::hip

@@ -1,0 +1,813 @@
+// The HIP backend. See HipSolver.hpp.
+#include "field/hip/HipSolver.hpp"
+
+#include "fe_hip.h"
+#include "field/hip/HipCodegen.hpp"
+#include "field/ir/LawIR.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+
+namespace fieldengine::field {
+
+namespace {
+
+// One process-wide lock around every backend's create/step/derive/fetch, as the reference's backends take
+// (src/field/cuda/CudaSolver.cpp:40-44): they share one device and one stream.
+std::mutex& device_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+void check(int status, const char* what)
+{
+    if (status != 0) {
+        throw std::runtime_error(std::string(what) + ": " + (status > 0 ? fe_hip_error_string(status) : "invalid arguments"));
+    }
+}
+
+struct Buffer {
+    void* p = nullptr;
+    std::size_t bytes = 0;
+    Buffer() = default;
+    Buffer(const Buffer&) = delete;
+    Buffer& operator=(const Buffer&) = delete;
+    ~Buffer()
+    {
+        if (p != nullptr) {
+            fe_hip_mem_free(p);
+        }
+    }
+    void allocate(std::size_t n)
+    {
+        if (p != nullptr) {
+            fe_hip_mem_free(p);
+            p = nullptr;
+        }
+        bytes = n;
+        check(fe_hip_mem_alloc(&p, n == 0 ? 8 : n), "device allocation");
+    }
+    template <typename T>
+    [[nodiscard]] T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+constexpr unsigned kBlockX = 64;
+constexpr unsigned kBlockY = 4;
+constexpr int kReduceBlocks = 1024; // blocks of a reduction's first pass (its partials fold in one block)
+constexpr int kTotals = 1024;      // doubles of conserved totals judged at once
+
+struct Impl final : LawBackend {
+    Law* law = nullptr;
+    GridShape shape;
+    ir::Module module;
+    hip::LawLayout layout;
+    std::string name = "HIP";
+    std::string adapter_name;
+    fe_hip_module mod = nullptr;
+    std::map<std::string, fe_hip_function> fns;
+    StepProfile prof;
+    int check_every = 1;
+    long long step_count = 0;
+    bool lazy = false;
+    std::vector<char> stale; // per field: the card holds newer cells than the host
+
+    // Device state.
+    std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
+    std::vector<void*> now, nxt;
+    Buffer P, U, A, S, L, C, records, firsts, totals;
+    std::vector<double> uploaded_params;
+    bool params_valid = false;
+    std::vector<fe_hip_event> events;
+
+    ~Impl() override
+    {
+        for (fe_hip_event e : events) {
+            fe_hip_event_destroy(e);
+        }
+        if (mod != nullptr) {
+            fe_hip_module_unload(mod);
+        }
+    }
+
+    [[nodiscard]] const std::string& adapter() const override { return adapter_name; }
+    [[nodiscard]] const char* api() const override { return "HIP"; }
+    [[nodiscard]] const StepProfile& profile() const override { return prof; }
+    void check_contracts_every(int steps) override { check_every = std::max(1, steps); }
+    void set_lazy(bool on) override { lazy = on; }
+
+    fe_hip_function fn(const std::string& n)
+    {
+        const auto it = fns.find(n);
+        if (it != fns.end()) {
+            return it->second;
+        }
+        fe_hip_function f = nullptr;
+        check(fe_hip_module_function(mod, n.c_str(), &f), ("kernel " + n).c_str());
+        fns.emplace(n, f);
+        return f;
+    }
+
+    // The generated module's argument struct, by value (HipCodegen.cpp: struct K).
+    std::vector<std::uint64_t> args(double dt) const
+    {
+        const int nf = std::max(1, static_cast<int>(law->fields.size()));
+        std::vector<std::uint64_t> k(static_cast<std::size_t>(2 * nf + 6 + 4 + 2), 0);
+        std::size_t w = 0;
+        for (int f = 0; f < nf; ++f) {
+            k[w++] = reinterpret_cast<std::uint64_t>(f < static_cast<int>(now.size()) ? now[static_cast<std::size_t>(f)] : nullptr);
+        }
+        for (int f = 0; f < nf; ++f) {
+            k[w++] = reinterpret_cast<std::uint64_t>(f < static_cast<int>(nxt.size()) ? nxt[static_cast<std::size_t>(f)] : nullptr);
+        }
+        k[w++] = reinterpret_cast<std::uint64_t>(P.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(U.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(A.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(S.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(L.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(C.p);
+        const long long nxy = static_cast<long long>(shape.nx) * shape.ny;
+        const long long cells = nxy * shape.nz;
+        std::memcpy(&k[w++], &nxy, 8);
+        std::memcpy(&k[w++], &cells, 8);
+        std::memcpy(&k[w++], &shape.cell, 8);
+        std::memcpy(&k[w++], &dt, 8);
+        const int ints[4] = {shape.nx, shape.ny, shape.nz, 0};
+        std::memcpy(&k[w], ints, 16);
+        return k;
+    }
+
+    void launch_cells(const std::string& kernel, double dt, std::vector<void*> extra = {})
+    {
+        std::vector<std::uint64_t> k = args(dt);
+        std::vector<void*> argv;
+        argv.push_back(k.data());
+        argv.insert(argv.end(), extra.begin(), extra.end());
+        const unsigned gx = (static_cast<unsigned>(shape.nx) + kBlockX - 1) / kBlockX;
+        const unsigned gy = (static_cast<unsigned>(shape.ny) + kBlockY - 1) / kBlockY;
+        check(fe_hip_launch(fn(kernel), gx, gy, static_cast<unsigned>(shape.nz), kBlockX, kBlockY, 1, 0, argv.data()),
+            ("launch " + kernel).c_str());
+        ++prof.launches;
+    }
+
+    void build(std::string& why)
+    {
+        module = ir::lower(*law);
+        const std::string src = hip::generate(module, *law, layout, why);
+        if (src.empty()) {
+            throw std::runtime_error(why);
+        }
+        std::vector<std::string> opts = hip::compile_options();
+        std::vector<const char*> argv;
+        for (const std::string& o : opts) {
+            argv.push_back(o.c_str());
+        }
+        void* code = nullptr;
+        std::size_t bytes = 0;
+        char* log = nullptr;
+        int cached = 0;
+        const int r = fe_hip_compile(src.c_str(), "law.hip", argv.data(), static_cast<int>(argv.size()), &code, &bytes, &log, &cached);
+        const std::string log_text = log != nullptr ? log : "";
+        fe_hip_free(log);
+        if (r != 0) {
+            throw std::runtime_error("the generated source does not compile: " + log_text);
+        }
+        const int lr = fe_hip_module_load(code, bytes, &mod);
+        fe_hip_free(code);
+        check(lr, "module load");
+        allocate();
+    }
+
+    void allocate()
+    {
+        const std::size_t cells = shape.cells();
+        const std::size_t nf = law->fields.size();
+        std::vector<char> swapped(nf, 0);
+        for (const auto* progs : {&layout.terms, &layout.learned, &layout.derives}) {
+            for (const hip::ProgramCode& p : *progs) {
+                for (const int f : p.swaps) {
+                    swapped[static_cast<std::size_t>(f)] = 1;
+                }
+            }
+        }
+        now_buf.clear();
+        next_buf.clear();
+        now.assign(nf, nullptr);
+        nxt.assign(nf, nullptr);
+        for (std::size_t f = 0; f < nf; ++f) {
+            now_buf.push_back(std::make_unique<Buffer>());
+            now_buf.back()->allocate(cells * 4);
+            now[f] = now_buf.back()->p;
+            next_buf.push_back(std::make_unique<Buffer>());
+            if (swapped[f]) {
+                next_buf.back()->allocate(cells * 4);
+                nxt[f] = next_buf.back()->p;
+            }
+        }
+        P.allocate(std::max<std::size_t>(1, law->param_values.size()) * 8);
+        U.allocate(static_cast<std::size_t>(std::max(1, layout.uniforms)) * 8);
+        A.allocate(static_cast<std::size_t>(std::max(1, layout.aux_slots)) * cells * 4);
+        bool tridiag = false;
+        for (const auto& [id, s] : layout.stages) {
+            tridiag = tridiag || s.kind == hip::StageCode::Kind::Tridiag;
+        }
+        S.allocate(tridiag ? 2 * cells * 8 : 8);
+        L.allocate(static_cast<std::size_t>(3 * kReduceBlocks) * 8);
+        C.allocate(std::max<std::size_t>(1, nf) * 4);
+        records.allocate(std::max<std::size_t>(1, nf) * 2 * 4);
+        firsts.allocate(std::max<std::size_t>(1, nf) * 8);
+        totals.allocate(static_cast<std::size_t>(kTotals) * 8);
+        stale.assign(nf, 0);
+    }
+
+    void upload_params()
+    {
+        if (params_valid && uploaded_params == law->param_values) {
+            return;
+        }
+        if (!law->param_values.empty()) {
+            check(fe_hip_memcpy_htod_async(P.p, law->param_values.data(), law->param_values.size() * 8), "params upload");
+        }
+        uploaded_params = law->param_values;
+        params_valid = true;
+    }
+
+    void upload(FieldSet& fields, FieldEdits& edits)
+    {
+        const std::size_t cells = shape.cells();
+        for (std::size_t f = 0; f < law->fields.size(); ++f) {
+            bool dirty = edits.all;
+            if (!dirty && f < edits.planes.size()) {
+                dirty = std::any_of(edits.planes[f].begin(), edits.planes[f].end(), [](char c) { return c != 0; });
+            }
+            if (!dirty) {
+                continue;
+            }
+            // The card only ever holds flushed floats: what the CPU reads of a denormal is its zero (DAZ).
+            std::vector<float>& v = fields.values(static_cast<int>(f));
+            for (float& x : v) {
+                if (x != 0.0f && std::fabs(x) < 1.17549435e-38f) {
+                    x = std::copysign(0.0f, x);
+                }
+            }
+            check(fe_hip_memcpy_htod_async(now[f], v.data(), cells * 4), "field upload");
+            prof.upload_bytes += cells * 4;
+            stale[f] = 0;
+        }
+        edits.all = false;
+        edits.planes.assign(law->fields.size(), std::vector<char>(static_cast<std::size_t>(shape.nz), 0));
+        edits.boxes.clear();
+        edits.shift[0] = edits.shift[1] = edits.shift[2] = 0;
+    }
+
+    // A scalar instruction on the host, in double, with reductions read from U.
+    double scalar(int id, const std::vector<double>& u) const
+    {
+        const ir::Inst& in = module.insts[static_cast<std::size_t>(id)];
+        auto a = [&](std::size_t i) { return scalar(in.args[i], u); };
+        switch (in.op) {
+        case Op::Const: return in.number;
+        case Op::Param: return law->param_values[static_cast<std::size_t>(in.index)];
+        case Op::Cell: return shape.cell;
+        case Op::Dt: return 0.0;
+        case Op::Local: return 0.0;
+        case Op::MaxOf: case Op::MinOf: case Op::SumOf: {
+            const int s = layout.uslot[static_cast<std::size_t>(id)];
+            return s >= 0 ? u[static_cast<std::size_t>(s)] : 0.0;
+        }
+        case Op::Neg: return -a(0);
+        case Op::Abs: return std::abs(a(0));
+        case Op::Sqrt: { const double q = a(0); return std::sqrt(0.0 < q ? q : 0.0); }
+        case Op::Exp: { const double q = a(0); return std::exp(80.0 < q ? 80.0 : q); }
+        case Op::Log: { const double q = a(0); return std::log(q < 1e-30 ? 1e-30 : q); }
+        case Op::Step: return a(0) > 0.0 ? 1.0 : 0.0;
+        case Op::Pos: { const double q = a(0); return q < 0.0 ? 0.0 : q; }
+        case Op::Add: return a(0) + a(1);
+        case Op::Sub: return a(0) - a(1);
+        case Op::Mul: { const double p = a(0); if (p == 0.0) return 0.0; const double q = a(1); return q == 0.0 ? 0.0 : p * q; }
+        case Op::Div: { const double q = a(1); return q == 0.0 ? 0.0 : a(0) / q; }
+        case Op::Pow: return std::pow(a(0), a(1));
+        case Op::Min: { const double p = a(0), q = a(1); return q < p ? q : p; }
+        case Op::Max: { const double p = a(0), q = a(1); return p < q ? q : p; }
+        case Op::Clamp: {
+            const double v = a(0), lo = a(1), h = a(2);
+            const double hi = lo < h ? h : lo;
+            return v < lo ? lo : (hi < v ? hi : v);
+        }
+        default: return 0.0;
+        }
+    }
+
+    void run_stage(int id, double dt)
+    {
+        const hip::StageCode& s = layout.stages.at(id);
+        switch (s.kind) {
+        case hip::StageCode::Kind::Stored:
+            launch_cells(s.kernel, dt);
+            break;
+        case hip::StageCode::Kind::Tridiag: {
+            const long long lines = s.axis == 0 ? 1LL * shape.ny * shape.nz : (s.axis == 1 ? 1LL * shape.nx * shape.nz : 1LL * shape.nx * shape.ny);
+            std::vector<std::uint64_t> k = args(dt);
+            void* argv[] = {k.data()};
+            check(fe_hip_launch(fn(s.kernel), static_cast<unsigned>((lines + 63) / 64), 1, 1, 64, 1, 1, 0, argv), "tridiag launch");
+            ++prof.launches;
+            break;
+        }
+        case hip::StageCode::Kind::Reduce: {
+            const long long cells = static_cast<long long>(shape.cells());
+            long long chunk = (cells + kReduceBlocks - 1) / kReduceBlocks;
+            chunk = std::max(1LL, chunk);
+            const int blocks = static_cast<int>((cells + chunk - 1) / chunk);
+            std::vector<std::uint64_t> k = args(dt);
+            void* argv1[] = {k.data(), &chunk};
+            check(fe_hip_launch(fn(s.kernel), static_cast<unsigned>(blocks), 1, 1, 256, 1, 1, 0, argv1), "reduce launch");
+            int kind = s.reduce == Op::SumOf ? 0 : (s.reduce == Op::MaxOf ? 1 : 2);
+            int parts = blocks;
+            int uslot = layout.uslot[static_cast<std::size_t>(id)];
+            void* argv2[] = {k.data(), &kind, &parts, &uslot};
+            check(fe_hip_launch(fn("k_reduce_final"), 1, 1, 1, 256, 1, 1, 0, argv2), "reduce final launch");
+            prof.launches += 2;
+            break;
+        }
+        }
+    }
+
+    void swap(const std::vector<int>& fields)
+    {
+        for (const int f : fields) {
+            std::swap(now[static_cast<std::size_t>(f)], nxt[static_cast<std::size_t>(f)]);
+        }
+    }
+
+    // The conserved total (sum, sum of magnitudes) of a field into totals[slot], [slot + 1].
+    void total(int field, int slot)
+    {
+        const long long cells = static_cast<long long>(shape.cells());
+        long long chunk = std::max(1LL, (cells + kReduceBlocks - 1) / kReduceBlocks);
+        const int blocks = static_cast<int>((cells + chunk - 1) / chunk);
+        std::vector<std::uint64_t> k = args(0.0);
+        double* parts = L.as<double>();
+        void* argv1[] = {k.data(), &field, &chunk, &parts};
+        check(fe_hip_launch(fn("k_total"), static_cast<unsigned>(blocks), 1, 1, 256, 1, 1, 0, argv1), "total launch");
+        double* dst = totals.as<double>() + slot;
+        int count = blocks;
+        void* argv2[] = {&parts, &count, &dst};
+        check(fe_hip_launch(fn("k_total_final"), 1, 1, 1, 256, 1, 1, 0, argv2), "total final launch");
+        prof.launches += 2;
+    }
+
+    void range(int field)
+    {
+        const FieldDecl& d = law->fields[static_cast<std::size_t>(field)];
+        double lo = d.lo, hi = d.hi;
+        int clamp = d.clamp ? 1 : 0;
+        unsigned* rec = records.as<unsigned>() + 2 * field;
+        unsigned long long* first = firsts.as<unsigned long long>() + field;
+        launch_cells("k_range", 0.0, {&field, &lo, &hi, &clamp, &rec, &first});
+    }
+
+    void reset_records()
+    {
+        const std::size_t nf = std::max<std::size_t>(1, law->fields.size());
+        check(fe_hip_memset_d32_async(records.p, 0u, nf * 2), "records reset");
+        check(fe_hip_memset_d32_async(firsts.p, 0xffffffffu, nf * 2), "records reset");
+    }
+
+    struct Judge {
+        std::string term;
+        std::vector<int> writes;
+        std::vector<std::vector<int>> groups;
+        int slot = 0; // totals[slot + 4g ...]: before sum, before mag, after sum, after mag
+        int substeps = 1;
+    };
+
+    // Runs one program for a step of dt: its substeps, stages and gather; queues contract kernels when checking.
+    // Returns the substeps taken.
+    int run_program(const ir::Program& p, const hip::ProgramCode& code, const std::string& term, double dt, bool respect,
+        bool contracts, std::vector<Violation>& violations, std::vector<Judge>& judges, int& total_slot)
+    {
+        if (code.gather.empty()) {
+            return 1;
+        }
+        int substeps = 1;
+        if (respect && code.max_dt >= 0) {
+            std::vector<double> u(static_cast<std::size_t>(std::max(1, layout.uniforms)), 0.0);
+            if (!code.limit_stages.empty()) {
+                for (const int s : code.limit_stages) {
+                    run_stage(s, 0.0);
+                }
+                check(fe_hip_memcpy_dtoh_async(u.data(), U.p, u.size() * 8), "uniforms readback");
+                check(fe_hip_stream_sync(), "sync");
+            }
+            const double limit = scalar(code.max_dt, u);
+            if (limit > 0.0 && std::isfinite(limit) && dt > limit) {
+                const double wanted = std::ceil(dt / limit);
+                if (wanted > 4096.0) {
+                    Violation v;
+                    v.kind = Violation::Kind::SubstepCap;
+                    v.term = term;
+                    std::ostringstream d;
+                    d << "max_dt " << limit << " asks for " << wanted << " substeps; capped at 4096";
+                    v.detail = d.str();
+                    violations.push_back(v);
+                    substeps = 4096;
+                } else {
+                    substeps = static_cast<int>(wanted);
+                }
+            }
+        }
+        const double sub_dt = dt / substeps;
+        Judge j;
+        j.term = term;
+        j.writes = p.writes;
+        j.groups = p.conserves;
+        j.substeps = substeps;
+        j.slot = total_slot;
+        // Slots: per group of m fields, m (sum, magnitude) pairs before the substeps, then m after.
+        if (contracts) {
+            for (const std::vector<int>& group : p.conserves) {
+                for (std::size_t q = 0; q < group.size(); ++q) {
+                    total(group[q], total_slot + 2 * static_cast<int>(q));
+                }
+                total_slot += 4 * static_cast<int>(group.size());
+            }
+        }
+        for (int s = 0; s < substeps; ++s) {
+            for (const int st : code.stages) {
+                run_stage(st, sub_dt);
+            }
+            launch_cells(code.gather, sub_dt);
+            swap(code.swaps);
+        }
+        if (contracts) {
+            int slot = j.slot;
+            for (const std::vector<int>& group : p.conserves) {
+                const int m = static_cast<int>(group.size());
+                for (int q = 0; q < m; ++q) {
+                    total(group[static_cast<std::size_t>(q)], slot + 2 * m + 2 * q);
+                }
+                slot += 4 * m;
+            }
+            for (const int f : p.writes) {
+                range(f);
+            }
+            judges.push_back(j);
+        }
+        return substeps;
+    }
+
+    // Totals slots one program needs (kTotals doubles in all).
+    static int total_slots(const ir::Program& p)
+    {
+        int n = 0;
+        for (const std::vector<int>& g : p.conserves) {
+            n += 4 * static_cast<int>(g.size());
+        }
+        return n;
+    }
+
+    void judge(const std::vector<Judge>& judges, std::vector<Violation>& violations)
+    {
+        if (judges.empty()) {
+            return;
+        }
+        const std::size_t nf = law->fields.size();
+        std::vector<unsigned> rec(std::max<std::size_t>(1, nf) * 2);
+        std::vector<unsigned long long> first(std::max<std::size_t>(1, nf));
+        std::vector<double> tot(kTotals);
+        check(fe_hip_memcpy_dtoh_async(rec.data(), records.p, rec.size() * 4), "records readback");
+        check(fe_hip_memcpy_dtoh_async(first.data(), firsts.p, first.size() * 8), "records readback");
+        check(fe_hip_memcpy_dtoh_async(tot.data(), totals.p, tot.size() * 8), "totals readback");
+        check(fe_hip_stream_sync(), "sync");
+        for (const Judge& j : judges) {
+            int slot = j.slot;
+            for (std::size_t g = 0; g < j.groups.size(); ++g) {
+                const int m = static_cast<int>(j.groups[g].size());
+                double before = 0.0, scale = 0.0, after = 0.0;
+                std::string names;
+                for (int q = 0; q < m; ++q) {
+                    before += tot[static_cast<std::size_t>(slot + 2 * q)];
+                    scale += tot[static_cast<std::size_t>(slot + 2 * q + 1)];
+                    after += tot[static_cast<std::size_t>(slot + 2 * m + 2 * q)];
+                    names += (names.empty() ? "" : " + ") + law->fields[static_cast<std::size_t>(j.groups[g][static_cast<std::size_t>(q)])].name;
+                }
+                slot += 4 * m;
+                // Float storage rounds every cell every substep: the budget scales with the amount and the work
+                // (src/field/Solver.cpp:433-461).
+                const double tolerance = 1e-5 * (scale + 1.0) * std::sqrt(static_cast<double>(j.substeps));
+                if (std::abs(after - before) > tolerance) {
+                    Violation v;
+                    v.kind = Violation::Kind::Conservation;
+                    v.term = j.term;
+                    v.field = names;
+                    std::ostringstream d;
+                    d << "total went " << before << " -> " << after << " (tolerance " << tolerance << ")";
+                    v.detail = d.str();
+                    violations.push_back(v);
+                }
+            }
+            for (const int f : j.writes) {
+                const unsigned out = rec[2 * static_cast<std::size_t>(f)];
+                const unsigned nan = rec[2 * static_cast<std::size_t>(f) + 1];
+                if (out + nan == 0) {
+                    continue;
+                }
+                const FieldDecl& d = law->fields[static_cast<std::size_t>(f)];
+                Violation v;
+                v.kind = nan > 0 ? Violation::Kind::NotFinite : Violation::Kind::Range;
+                v.term = j.term;
+                v.field = d.name;
+                const unsigned long long i = first[static_cast<std::size_t>(f)];
+                const std::size_t nx = static_cast<std::size_t>(shape.nx), ny = static_cast<std::size_t>(shape.ny);
+                std::ostringstream det;
+                det << out + nan << " cell(s); first at (" << i % nx << "," << (i / nx) % ny << "," << i / (nx * ny)
+                    << "), declared range [" << d.lo << ", " << d.hi << "]";
+                v.detail = det.str();
+                violations.push_back(v);
+            }
+        }
+    }
+
+    void snapshot(const std::vector<int>& fields, std::vector<std::unique_ptr<Buffer>>& saved)
+    {
+        const std::size_t bytes = shape.cells() * 4;
+        saved.clear();
+        for (const int f : fields) {
+            saved.push_back(std::make_unique<Buffer>());
+            saved.back()->allocate(bytes);
+            check(fe_hip_memcpy_dtod_async(saved.back()->p, now[static_cast<std::size_t>(f)], bytes), "snapshot");
+        }
+    }
+
+    void restore(const std::vector<int>& fields, std::vector<std::unique_ptr<Buffer>>& saved)
+    {
+        const std::size_t bytes = shape.cells() * 4;
+        for (std::size_t i = 0; i < fields.size(); ++i) {
+            check(fe_hip_memcpy_dtod_async(now[static_cast<std::size_t>(fields[i])], saved[i]->p, bytes), "restore");
+        }
+    }
+
+    void run_derives(bool contracts, std::vector<Judge>& judges)
+    {
+        for (std::size_t d = 0; d < module.derives.size(); ++d) {
+            const hip::ProgramCode& code = layout.derives[d];
+            if (code.gather.empty()) {
+                continue;
+            }
+            for (const int st : code.stages) {
+                run_stage(st, 0.0);
+            }
+            launch_cells(code.gather, 0.0);
+            swap(code.swaps);
+        }
+        if (contracts && !module.derives.empty()) {
+            Judge j;
+            std::vector<int> targets;
+            for (const ir::Program& p : module.derives) {
+                for (const int t : p.writes) {
+                    if (std::find(targets.begin(), targets.end(), t) == targets.end()) {
+                        targets.push_back(t);
+                    }
+                }
+            }
+            for (const int f : targets) {
+                range(f);
+            }
+            j.writes = targets;
+            judges.push_back(j);
+        }
+    }
+
+    void download_changed(FieldSet& fields, const std::vector<char>& which)
+    {
+        const std::size_t cells = shape.cells();
+        for (std::size_t f = 0; f < which.size(); ++f) {
+            if (!which[f]) {
+                continue;
+            }
+            check(fe_hip_memcpy_dtoh_async(fields.values(static_cast<int>(f)).data(), now[f], cells * 4), "field download");
+            prof.download_bytes += cells * 4;
+            stale[f] = 0;
+        }
+        check(fe_hip_stream_sync(), "sync");
+    }
+
+    void read_changed(std::vector<char>& changed)
+    {
+        const std::size_t nf = law->fields.size();
+        std::vector<unsigned> c(std::max<std::size_t>(1, nf));
+        check(fe_hip_memcpy_dtoh_async(c.data(), C.p, c.size() * 4), "change flags readback");
+        check(fe_hip_stream_sync(), "sync");
+        changed.assign(nf, 0);
+        for (std::size_t f = 0; f < nf; ++f) {
+            changed[f] = c[f] != 0 ? 1 : 0;
+        }
+    }
+
+    fe_hip_event event(std::size_t i)
+    {
+        while (events.size() <= i) {
+            fe_hip_event e = nullptr;
+            check(fe_hip_event_create(&e), "event");
+            events.push_back(e);
+        }
+        return events[i];
+    }
+
+    void step(double dt, bool respect_max_dt, bool learned, Law& l, FieldSet& fields, FieldEdits& edits,
+        std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps) override
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        const auto t0 = std::chrono::steady_clock::now();
+        law = &l;
+        prof = StepProfile{};
+        prof.backend = "HIP";
+        prof.terms.resize(module.terms.size());
+        upload_params();
+        upload(fields, edits);
+        const std::size_t nf = law->fields.size();
+        check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, nf)), "change flags reset");
+        const bool contracts = (step_count % check_every) == 0;
+        ++step_count;
+        std::vector<Judge> judges;
+        int total_slot = 0;
+        if (contracts) {
+            reset_records();
+        }
+        substeps.assign(module.terms.size(), 0);
+        std::size_t ev = 0;
+        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+            TermProfile& tp = prof.terms[t];
+            tp.name = module.terms[t].name;
+            const int launches0 = prof.launches;
+            check(fe_hip_event_record(event(ev)), "event");
+            const bool guarded = learned && !module.learned[t].equations.empty();
+            // Judge what is pending before the records are reused: a guarded term judges its own at once, and the
+            // totals buffer holds kTotals doubles.
+            const int need = std::max(total_slots(module.terms[t]), total_slots(module.learned[t]));
+            if (contracts && !judges.empty() && (guarded || total_slot + need > kTotals)) {
+                judge(judges, violations);
+                judges.clear();
+                total_slot = 0;
+                reset_records();
+            }
+            bool done = false;
+            if (guarded) {
+                // Snapshot what the term writes, run the learned scheme and judge it now; a breach is undone and the
+                // classical scheme runs (src/field/cuda/CudaSolver.cpp:673-705 is the reference's guard).
+                std::vector<std::unique_ptr<Buffer>> saved;
+                snapshot(module.terms[t].writes, saved);
+                std::vector<Judge> own;
+                int slot = 0;
+                reset_records();
+                std::vector<Violation> caught;
+                const int n = run_program(module.learned[t], layout.learned[t], module.terms[t].name, dt, respect_max_dt, true,
+                    caught, own, slot);
+                judge(own, caught);
+                reset_records();
+                if (caught.empty()) {
+                    tp.learned = true;
+                    substeps[t] = n;
+                    done = true;
+                } else {
+                    restore(module.terms[t].writes, saved);
+                    tp.fell_back = true;
+                    ++prof.fallbacks;
+                    check(fe_hip_stream_sync(), "sync");
+                }
+            }
+            if (!done) {
+                substeps[t] = run_program(module.terms[t], layout.terms[t], module.terms[t].name, dt, respect_max_dt, contracts,
+                    violations, judges, total_slot);
+            }
+            check(fe_hip_event_record(event(ev + 1)), "event");
+            tp.substeps = substeps[t];
+            tp.launches = prof.launches - launches0;
+            ev += 2;
+        }
+        const std::size_t derive_ev = ev;
+        check(fe_hip_event_record(event(ev++)), "event");
+        run_derives(contracts, judges);
+        check(fe_hip_event_record(event(ev++)), "event");
+        if (contracts) {
+            judge(judges, violations);
+        }
+        read_changed(changed);
+        for (std::size_t f = 0; f < nf; ++f) {
+            stale[f] = stale[f] || changed[f];
+        }
+        if (!lazy) {
+            download_changed(fields, changed);
+        }
+        check(fe_hip_event_sync(event(ev - 1)), "event sync");
+        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+            float ms = 0.0f;
+            fe_hip_event_elapsed(event(2 * t), event(2 * t + 1), &ms);
+            prof.terms[t].ms = ms;
+            prof.gpu_ms += ms;
+        }
+        float dms = 0.0f;
+        fe_hip_event_elapsed(event(derive_ev), event(derive_ev + 1), &dms);
+        prof.derive_ms = dms;
+        prof.gpu_ms += dms;
+        prof.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void derive(Law& l, FieldSet& fields, FieldEdits& edits, std::vector<char>& changed) override
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        law = &l;
+        upload_params();
+        upload(fields, edits);
+        check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, law->fields.size())), "change flags reset");
+        std::vector<Judge> none;
+        run_derives(false, none);
+        read_changed(changed);
+        for (std::size_t f = 0; f < law->fields.size(); ++f) {
+            stale[f] = stale[f] || changed[f];
+        }
+        if (!lazy) {
+            download_changed(fields, changed);
+        }
+    }
+
+    void fetch(FieldSet& fields, const std::vector<char>& which) override
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        std::vector<char> need(law->fields.size(), 0);
+        for (std::size_t f = 0; f < need.size() && f < which.size(); ++f) {
+            need[f] = which[f] && stale[f] ? 1 : 0;
+        }
+        download_changed(fields, need);
+    }
+
+    // Device pointers of a field's current cells (for callers that keep data on the card).
+    [[nodiscard]] void* field_pointer(int f) const { return now[static_cast<std::size_t>(f)]; }
+};
+
+} // namespace
+
+const char* to_string(Violation::Kind kind)
+{
+    switch (kind) {
+    case Violation::Kind::Range: return "range";
+    case Violation::Kind::NotFinite: return "not-finite";
+    case Violation::Kind::Conservation: return "conservation";
+    case Violation::Kind::SubstepCap: return "substep-cap";
+    }
+    return "?";
+}
+
+std::shared_ptr<LawBackend> HipSolver::create(Law& law, const GridShape& shape, std::string& why)
+{
+    const std::string no = unavailable();
+    if (!no.empty()) {
+        why = no;
+        return nullptr;
+    }
+    if (shape.nx < 1 || shape.ny < 1 || shape.nz < 1 || !(shape.cell > 0.0)) {
+        why = "the lattice must have at least one cell and a positive cell size";
+        return nullptr;
+    }
+    const std::lock_guard<std::mutex> lock(device_mutex());
+    auto impl = std::make_shared<Impl>();
+    impl->law = &law;
+    impl->shape = shape;
+    fe_hip_device_info info{};
+    fe_hip_device_open(&info);
+    impl->adapter_name = std::string(info.name) + " (" + info.arch + ")";
+    try {
+        impl->build(why);
+    } catch (const std::exception& e) {
+        why = e.what();
+        return nullptr;
+    }
+    return impl;
+}
+
+std::string HipSolver::unavailable()
+{
+    return fe_hip_unavailable();
+}
+
+std::string HipSolver::source(const Law& law, std::string& why)
+{
+    hip::LawLayout layout;
+    return hip::generate(ir::lower(law), law, layout, why);
+}
+
+void* hip_backend_field_pointer(LawBackend& backend, int field)
+{
+    return static_cast<Impl&>(backend).field_pointer(field);
+}
+
+} // namespace fieldengine::field

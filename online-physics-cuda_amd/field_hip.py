@@ -59,6 +59,40 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_heat_handwritten_cfg": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double,
                                             c_int, c_int, c_int, c_int]),
         "fe_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t]),
+        "fe_hip_compile": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), c_int,
+                                   ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p),
+                                   ctypes.POINTER(c_int)]),
+        "fe_hip_free": (None, [c_void_p]),
+        "fe_law_compile": (c_void_p, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), c_int,
+                                      ctypes.c_char_p, c_size_t]),
+        "fe_law_free": (None, [c_void_p]),
+        "fe_law_field_count": (c_int, [c_void_p]),
+        "fe_law_field_id": (c_int, [c_void_p, ctypes.c_char_p]),
+        "fe_law_field_name": (ctypes.c_char_p, [c_void_p, c_int]),
+        "fe_law_param_count": (c_int, [c_void_p]),
+        "fe_law_term_count": (c_int, [c_void_p]),
+        "fe_law_ir_json": (c_void_p, [c_void_p]),
+        "fe_law_describe": (c_void_p, [c_void_p]),
+        "fe_law_hip_source": (c_void_p, [c_void_p, ctypes.c_char_p, c_size_t]),
+        "fe_string_free": (None, [c_void_p]),
+        "fe_solver_create": (c_void_p, [c_void_p, c_int, c_int, c_int, c_double, ctypes.c_char_p, c_size_t]),
+        "fe_solver_free": (None, [c_void_p]),
+        "fe_solver_set_field": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_solver_get_field": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_solver_set_param": (c_int, [c_void_p, ctypes.c_char_p, c_double]),
+        "fe_solver_step": (c_int, [c_void_p, c_double]),
+        "fe_solver_refresh_derived": (c_int, [c_void_p]),
+        "fe_solver_set_lazy_readback": (c_int, [c_void_p, c_int]),
+        "fe_solver_check_contracts_every": (c_int, [c_void_p, c_int]),
+        "fe_solver_set_learned": (c_int, [c_void_p, c_int]),
+        "fe_solver_set_respect_max_dt": (c_int, [c_void_p, c_int]),
+        "fe_solver_violations_json": (c_void_p, [c_void_p]),
+        "fe_solver_last_substeps": (c_int, [c_void_p, ctypes.POINTER(c_int), c_int]),
+        "fe_solver_last_changed": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_ubyte), c_int]),
+        "fe_solver_profile_json": (c_void_p, [c_void_p]),
+        "fe_solver_digest": (ctypes.c_uint64, [c_void_p]),
+        "fe_solver_error": (ctypes.c_char_p, [c_void_p]),
+        "fe_solver_field_device": (c_void_p, [c_void_p, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -192,3 +226,169 @@ def copy_f32(src: DeviceArray, dst: DeviceArray) -> None:
     if src.count != dst.count:
         raise FieldHipError("copy_f32: sizes differ")
     check(lib().fe_copy_f32(src.ptr, dst.ptr, src.count), "copy_f32")
+
+
+def _take_string(ptr) -> Optional[str]:
+    if not ptr:
+        return None
+    try:
+        return ctypes.string_at(ptr).decode()
+    finally:
+        lib().fe_string_free(ptr)
+
+
+def compile_hip(source: str, name: str = "probe.hip", options=()) -> bytes:
+    """hipRTC-compiles HIP source for the device's arch (gfx950 without a device) through the runtime's cache."""
+    opts = (ctypes.c_char_p * max(1, len(options)))(*[o.encode() for o in options])
+    code, size, log, cached = ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_void_p(), ctypes.c_int()
+    rc = lib().fe_hip_compile(source.encode(), name.encode(), opts, len(options), ctypes.byref(code), ctypes.byref(size),
+                              ctypes.byref(log), ctypes.byref(cached))
+    text = ctypes.string_at(log.value).decode(errors="replace") if log.value else ""
+    if log.value:
+        lib().fe_hip_free(log)
+    if rc != 0:
+        raise FieldHipError(f"hipRTC: {text}")
+    blob = ctypes.string_at(code.value, size.value)
+    lib().fe_hip_free(code)
+    return blob
+
+
+class Law:
+    """A compiled law: the front end and its IR (no device needed)."""
+
+    def __init__(self, *texts: str, names=None):
+        names = names or [f"law{i}" for i in range(len(texts))]
+        n = len(texts)
+        c_names = (ctypes.c_char_p * n)(*[x.encode() for x in names])
+        c_texts = (ctypes.c_char_p * n)(*[t.encode() for t in texts])
+        err = ctypes.create_string_buffer(1 << 16)
+        self.ptr = lib().fe_law_compile(c_names, c_texts, n, err, len(err))
+        if not self.ptr:
+            raise FieldHipError("the law does not compile:\n" + err.value.decode())
+        self.warnings = err.value.decode()
+
+    @classmethod
+    def from_files(cls, *paths: str) -> "Law":
+        return cls(*[open(p).read() for p in paths], names=list(paths))
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().fe_law_free(self.ptr)
+        except Exception:
+            pass
+
+    @property
+    def field_count(self) -> int:
+        return lib().fe_law_field_count(self.ptr)
+
+    def field_id(self, name: str) -> int:
+        return lib().fe_law_field_id(self.ptr, name.encode())
+
+    def field_names(self):
+        return [lib().fe_law_field_name(self.ptr, i).decode() for i in range(self.field_count)]
+
+    def ir_json(self) -> str:
+        return _take_string(lib().fe_law_ir_json(self.ptr))
+
+    def describe(self) -> str:
+        return _take_string(lib().fe_law_describe(self.ptr))
+
+    def hip_source(self) -> str:
+        why = ctypes.create_string_buffer(4096)
+        src = _take_string(lib().fe_law_hip_source(self.ptr, why, len(why)))
+        if src is None:
+            raise FieldHipError("no HIP source: " + why.value.decode())
+        return src
+
+
+class Solver:
+    """A law integrating on the card: the reference's Solver calls over the HIP backend (no CPU path)."""
+
+    def __init__(self, law: Law, shape, h: float):
+        require_device()
+        self.law = law  # keep the law alive
+        self.shape = tuple(int(v) for v in shape)
+        self.h = float(h)
+        why = ctypes.create_string_buffer(1 << 16)
+        nx, ny, nz = self.shape
+        self.ptr = lib().fe_solver_create(law.ptr, nx, ny, nz, self.h, why, len(why))
+        if not self.ptr:
+            raise FieldHipError("the law does not run on the card: " + why.value.decode())
+        self.cells = nx * ny * nz
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().fe_solver_free(self.ptr)
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise FieldHipError(f"{what}: " + (lib().fe_solver_error(self.ptr) or b"invalid arguments").decode())
+
+    def _field(self, field) -> int:
+        f = self.law.field_id(field) if isinstance(field, str) else int(field)
+        if f < 0:
+            raise FieldHipError(f"no field {field!r}")
+        return f
+
+    def set_field(self, field, values: np.ndarray) -> None:
+        values = np.ascontiguousarray(values, dtype=np.float32).ravel()
+        if values.size != self.cells:
+            raise FieldHipError("set_field: wrong number of cells")
+        self._check(lib().fe_solver_set_field(self.ptr, self._field(field), values.ctypes.data), "set_field")
+
+    def get_field(self, field) -> np.ndarray:
+        out = np.empty(self.cells, dtype=np.float32)
+        self._check(lib().fe_solver_get_field(self.ptr, self._field(field), out.ctypes.data), "get_field")
+        return out
+
+    def set_param(self, name: str, value: float) -> None:
+        if lib().fe_solver_set_param(self.ptr, name.encode(), float(value)) != 0:
+            raise FieldHipError(f"no param {name!r}")
+
+    def step(self, dt: float) -> None:
+        self._check(lib().fe_solver_step(self.ptr, float(dt)), "step")
+
+    def refresh_derived(self) -> None:
+        self._check(lib().fe_solver_refresh_derived(self.ptr), "refresh_derived")
+
+    def set_lazy_readback(self, on: bool) -> None:
+        self._check(lib().fe_solver_set_lazy_readback(self.ptr, int(on)), "set_lazy_readback")
+
+    def check_contracts_every(self, steps: int) -> None:
+        self._check(lib().fe_solver_check_contracts_every(self.ptr, int(steps)), "check_contracts_every")
+
+    def set_learned(self, on: bool) -> None:
+        self._check(lib().fe_solver_set_learned(self.ptr, int(on)), "set_learned")
+
+    def set_respect_max_dt(self, on: bool) -> None:
+        self._check(lib().fe_solver_set_respect_max_dt(self.ptr, int(on)), "set_respect_max_dt")
+
+    def violations(self):
+        import json
+        return json.loads(_take_string(lib().fe_solver_violations_json(self.ptr)))
+
+    def last_substeps(self):
+        n = lib().fe_solver_last_substeps(self.ptr, (ctypes.c_int * 1)(), 0)
+        buf = (ctypes.c_int * max(1, n))()
+        lib().fe_solver_last_substeps(self.ptr, buf, n)
+        return list(buf[:n])
+
+    def last_changed(self):
+        n = lib().fe_solver_last_changed(self.ptr, (ctypes.c_ubyte * 1)(), 0)
+        buf = (ctypes.c_ubyte * max(1, n))()
+        lib().fe_solver_last_changed(self.ptr, buf, n)
+        return [bool(v) for v in buf[:n]]
+
+    def profile(self):
+        import json
+        return json.loads(_take_string(lib().fe_solver_profile_json(self.ptr)))
+
+    def digest(self) -> int:
+        return int(lib().fe_solver_digest(self.ptr))
+
+    def field_device(self, field) -> int:
+        return lib().fe_solver_field_device(self.ptr, self._field(field))

@@ -1,7 +1,7 @@
 """The golden cases: law texts written for this project, grids, steps and seeded initial states.
 
 make_golden.py runs each through the reference's CPU engine and stores inputs and outputs as tests/golden/NAME.npz.
-The tests import this module for nothing but the list of names; everything they check comes from the .npz files.
+The tests read only the .npz files (the law text, grid, dt and steps travel inside them).
 """
 from __future__ import annotations
 
@@ -18,14 +18,132 @@ term conduction
   max_dt = 0.15 * h * h / k
 """
 
+# Zero-Dirichlet conduction: each boundary face holds a ghost of -T (the wall at 0), so a cell with nb such faces loses
+# 2 k nb T / h^2 on top of the zero-flux diffuse().
+HEAT_DIRICHLET_LAW = """field T init 0 range -1000000 1000000
+field nb init 0 range 0 6 static
 
-def _uniform(shape, seed, lo=0.0, hi=1.0):
+param k = 0.1
+
+term conduction
+  d T = diffuse(k, T) - 2 * k * nb * T / (h * h)
+  max_dt = 0.15 * h * h / k
+"""
+
+BLACK_SCHOLES_LAW = """# Black-Scholes in x = ln S along x; every (y, z) line is its own option
+field V init 0 range -1 1000000
+
+param sig = 0.2
+param r = 0.05
+
+term black_scholes
+  d V = 0.5 * sig * sig * (shift(V, 1, 0, 0) - 2 * V + shift(V, -1, 0, 0)) / (h * h) + (r - 0.5 * sig * sig) * gx(V) - r * V
+  max_dt = 0.4 * h * h / (sig * sig)
+"""
+
+DIFF_REACT_RK4_LAW = """# two-field diffusion-reaction (FitzHugh-Nagumo), five-point Laplacian per plane, classical RK4 through lets
+field u init 0 range -1000 1000
+field v init 0 range -1000 1000
+
+param du = 0.001
+param dv = 0.005
+param kr = 0.005
+
+term reaction_diffusion
+  let lu0 = (shift(u, 1, 0, 0) + shift(u, -1, 0, 0) + shift(u, 0, 1, 0) + shift(u, 0, -1, 0) - 4 * u) / (h * h)
+  let lv0 = (shift(v, 1, 0, 0) + shift(v, -1, 0, 0) + shift(v, 0, 1, 0) + shift(v, 0, -1, 0) - 4 * v) / (h * h)
+  let fu0 = du * lu0 + u - u * u * u - kr - v
+  let fv0 = dv * lv0 + u - v
+  let u1 = u + 0.5 * dt * fu0
+  let v1 = v + 0.5 * dt * fv0
+  let fu1 = du * (shift(u1, 1, 0, 0) + shift(u1, -1, 0, 0) + shift(u1, 0, 1, 0) + shift(u1, 0, -1, 0) - 4 * u1) / (h * h) + u1 - u1 * u1 * u1 - kr - v1
+  let fv1 = dv * (shift(v1, 1, 0, 0) + shift(v1, -1, 0, 0) + shift(v1, 0, 1, 0) + shift(v1, 0, -1, 0) - 4 * v1) / (h * h) + u1 - v1
+  let u2 = u + 0.5 * dt * fu1
+  let v2 = v + 0.5 * dt * fv1
+  let fu2 = du * (shift(u2, 1, 0, 0) + shift(u2, -1, 0, 0) + shift(u2, 0, 1, 0) + shift(u2, 0, -1, 0) - 4 * u2) / (h * h) + u2 - u2 * u2 * u2 - kr - v2
+  let fv2 = dv * (shift(v2, 1, 0, 0) + shift(v2, -1, 0, 0) + shift(v2, 0, 1, 0) + shift(v2, 0, -1, 0) - 4 * v2) / (h * h) + u2 - v2
+  let u3 = u + dt * fu2
+  let v3 = v + dt * fv2
+  let fu3 = du * (shift(u3, 1, 0, 0) + shift(u3, -1, 0, 0) + shift(u3, 0, 1, 0) + shift(u3, 0, -1, 0) - 4 * u3) / (h * h) + u3 - u3 * u3 * u3 - kr - v3
+  let fv3 = dv * (shift(v3, 1, 0, 0) + shift(v3, -1, 0, 0) + shift(v3, 0, 1, 0) + shift(v3, 0, -1, 0) - 4 * v3) / (h * h) + u3 - v3
+  d u = (fu0 + 2 * fu1 + 2 * fu2 + fu3) / 6
+  d v = (fv0 + 2 * fv1 + 2 * fv2 + fv3) / 6
+  max_dt = 0.25 * h * h / max(du, dv)
+"""
+
+CRANK_NICOLSON_LAW = """# Crank-Nicolson along x through tridiag_x; the rate is (T' - T) / dt
+field T init 0 range -1000000 1000000
+
+param k = 0.01
+
+term crank_nicolson
+  let a = 0.5 * k * dt / (h * h)
+  d T = (tridiag_x(0 - a, 1 + 2 * a, 0 - a, T + a * (shift(T, 1, 0, 0) - 2 * T + shift(T, -1, 0, 0))) - T) / dt
+"""
+
+ADI_YZ_LAW = """# implicit steps along y then z (tridiag_y, tridiag_z) of a field-dependent operator
+field T init 0 range -1000000 1000000
+field w init 1 range 0 10 static
+
+param k = 0.05
+
+term implicit_yz
+  let a = k * w * dt / (h * h)
+  let y1 = tridiag_y(0 - a, 1 + 2 * a, 0 - a, T)
+  d T = (tridiag_z(0 - 0.5 * a, 1 + a, 0 - 0.5 * a, y1) - T) / dt
+"""
+
+# Every op alone on a lattice uneven on every axis (each is its own law: A moved by the op, B static input).
+OPS = {
+    "lap": "d A = lap(A)",
+    "grad": "d A = gx(A) + gy(A) + gz(A)",
+    "diffuse_field_k": "d A = diffuse(0.5 + B, A)",
+    "adv": "d A = adv(A, B - 0.5, 0.3, -0.2)",
+    "shift": "d A = shift(A, 1, -1, 0) - A + 0.5 * shift(A, -2, 0, 1)",
+    "elementwise": "d A = step(B - 0.5) * sqrt(abs(A)) - clamp(A, 0.2, 0.4) + min(A, B) * max(A, 0.1)",
+    "transcendental": "d A = abs(B) ^ 1.5 - exp(-A) + log(1 + B) + pos(A - 0.6) / (B + 1)",
+    "reductions": "d A = 0.01 * (maxof(B) - minof(A)) + 0.0001 * sumof(B)",
+    "transposes": "d A = 0.1 * (shift_t(A, 1, 0, -1) + gx_t(B) - gy_t(A) + gz_t(A))",
+    "rim_mul_zero": "d A = rim * (A - B) + 0 * lap(A) - 0.25 * A * B",
+}
+
+
+def op_law(expr: str) -> str:
+    return ("field A init 0 range -1000000000 1000000000\nfield B init 0 range -1000000000 1000000000\n"
+            "term t\n  " + expr + "\n")
+
+
+MULTI_LAW = """# several equations, several targets, a conserved group, a clamped field and derives
+field a init 0 range -1000 1000
+field b init 0 range 0 2 clamp
+field c init 0 range -1000 1000
+derive s = a + b
+derive s += 0.5 * c
+derive g = gx(a) * gy(b)
+
+param ka = 0.3
+param kb = 0.2
+
+term exchange
+  d a = diffuse(ka, a) - 0.1 * (a - c)
+  d c = 0.1 * (a - c)
+  d b = diffuse(kb, b) + 0.05 * a
+  d b = 0.02 * (1 - b)
+  conserves a + c
+  max_dt = 0.2 * h * h / max(ka, kb)
+
+term decay
+  d c = -0.01 * c * abs(s)
+"""
+
+
+def _uniform(n, seed, lo=0.0, hi=1.0):
     rng = np.random.default_rng(seed)
-    return (lo + (hi - lo) * rng.random(int(np.prod(shape)))).astype(np.float32)
+    return (lo + (hi - lo) * rng.random(int(n))).astype(np.float32)
 
 
 def _start_value(n):
-    # A deterministic spread of values 290..1290, as the reference's stencil bench starts its lattice.
+    # A deterministic spread of values 290..1290 (a splitmix-style hash of the cell index).
     i = np.arange(n, dtype=np.uint64)
     with np.errstate(over="ignore"):
         h = i * np.uint64(0x9E3779B97F4A7C15)
@@ -35,17 +153,64 @@ def _start_value(n):
     return (290.0 + 1000.0 * (h >> np.uint64(40)).astype(np.float32) / np.float32(16777216.0)).astype(np.float32)
 
 
+def _boundary_faces(nx, ny, nz):
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    nb = np.zeros((nz, ny, nx), dtype=np.float32)
+    for c, n in ((x, nx), (y, ny), (z, nz)):
+        nb += (c == 0).astype(np.float32) + (c == n - 1).astype(np.float32)
+    return nb.ravel()
+
+
+def _sine_mode(nx, ny, nz, h, noise_seed=None, amp=1e-3):
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) * h, (np.arange(ny) + 0.5) * h, (np.arange(nx) + 0.5) * h, indexing="ij")
+    t = (np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)).ravel()
+    if noise_seed is not None:
+        t = t + amp * np.random.default_rng(noise_seed).random(t.size)
+    return t.astype(np.float32)
+
+
+def _bs_payoff(nx, planes, h, strike=100.0):
+    # cell-averaged put payoff max(K - e^x, 0) on x in ln 100 +- 4
+    x0 = np.log(100.0) - 4.0
+    edges = x0 + h * np.arange(nx + 1)
+    lo, hi = edges[:-1], edges[1:]
+    k = np.log(strike)
+    a = np.minimum(hi, k)
+    avg = np.where(a > lo, (strike * (a - lo) - (np.exp(a) - np.exp(lo))) / h, 0.0)
+    return np.tile(avg, planes).astype(np.float32)
+
+
+def _ops_init(n, seed):
+    return {"A": _uniform(n, seed, 0.1, 0.9), "B": (np.random.default_rng(seed + 1).random(n) < 0.4).astype(np.float32)}
+
+
+G_OPS = (20, 14, 9, 0.5)
+
 CASES = [
-    # Uneven sizes on every axis, three substeps a step (dt = 0.9 over max_dt = 0.375).
     dict(name="heat_20x14x9", law=HEAT_LAW, grid=(20, 14, 9, 0.5), dt=0.9, steps=4,
-         params={"k": 0.1}, init={"T": _uniform((20, 14, 9), 1)}),
-    # k = h = 1, one substep: the configuration the reference's own bit-identity check uses.
+         params={"k": 0.1}, init={"T": _uniform(20 * 14 * 9, 1)}),
     dict(name="heat_16_k1", law=HEAT_LAW, grid=(16, 16, 16, 1.0), dt=0.1, steps=5,
          params={"k": 1.0}, init={"T": _start_value(16 * 16 * 16)}),
-    # The conduction configuration at a small size: h = 1/32, k = 0.1, dt = 0.4 h^2 / k (three substeps).
     dict(name="heat_32_cfg", law=HEAT_LAW, grid=(32, 32, 32, 1.0 / 32), dt=0.4 * (1.0 / 32) ** 2 / 0.1, steps=6,
-         params={"k": 0.1}, init={"T": _uniform((32, 32, 32), 7)}),
-    # A flat lattice, one cell thick: the z faces never exist.
+         params={"k": 0.1}, init={"T": _uniform(32 ** 3, 7)}),
     dict(name="heat_64x48x1", law=HEAT_LAW, grid=(64, 48, 1, 1.0), dt=0.1, steps=7,
-         params={"k": 0.7}, init={"T": _uniform((64, 48, 1), 3, -2.0, 5.0)}),
+         params={"k": 0.7}, init={"T": _uniform(64 * 48, 3, -2.0, 5.0)}),
+    dict(name="dirichlet_16", law=HEAT_DIRICHLET_LAW, grid=(16, 16, 16, 1.0 / 16), dt=0.4 * (1.0 / 16) ** 2 / 0.1, steps=20,
+         params={}, init={"T": _sine_mode(16, 16, 16, 1.0 / 16, noise_seed=5), "nb": _boundary_faces(16, 16, 16)}),
+    dict(name="bs_256x4", law=BLACK_SCHOLES_LAW, grid=(256, 1, 4, 8.0 / 256), dt=0.4 * (8.0 / 256) ** 2 / 0.04, steps=30,
+         params={}, init={"V": _bs_payoff(256, 4, 8.0 / 256)}),
+    dict(name="diffreact_rk4", law=DIFF_REACT_RK4_LAW, grid=(32, 24, 3, 2.0 / 32), dt=0.05, steps=5,
+         params={}, init={"u": np.random.default_rng(8).standard_normal(32 * 24 * 3).astype(np.float32),
+                          "v": np.random.default_rng(9).standard_normal(32 * 24 * 3).astype(np.float32)}),
+    dict(name="crank_nicolson", law=CRANK_NICOLSON_LAW, grid=(48, 5, 3, 1.0 / 48), dt=0.02, steps=6,
+         params={}, init={"T": _uniform(48 * 5 * 3, 10)}),
+    dict(name="adi_yz", law=ADI_YZ_LAW, grid=(6, 17, 11, 0.25), dt=0.3, steps=4,
+         params={}, init={"T": _uniform(6 * 17 * 11, 12), "w": _uniform(6 * 17 * 11, 13, 0.5, 2.0)}),
+    dict(name="multi", law=MULTI_LAW, grid=(18, 11, 7, 0.4), dt=0.25, steps=5, params={},
+         init={"a": _uniform(18 * 11 * 7, 14, -1.0, 1.0), "b": _uniform(18 * 11 * 7, 15, 0.0, 1.9),
+               "c": _uniform(18 * 11 * 7, 16, 0.0, 2.0)}),
+] + [
+    dict(name="op_" + key, law=op_law(expr), grid=G_OPS, dt=0.02, steps=4, params={},
+         init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))
+    for i, (key, expr) in enumerate(OPS.items())
 ]
