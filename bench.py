@@ -3,11 +3,13 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 
-One "step" is one fused substep of `d T = diffuse(k, T)` over the whole lattice (ping-pong buffers, contracts off),
-on a seeded uniform [0,1) field already resident in HBM; the unit is cell-updates/s. Beside it: the copy kernel (the
-bandwidth roof), the handwritten kernel it must be bit-identical to, the same step at 512^3 (a working set larger than
-the 256 MB Infinity Cache), and the reference's own OpenMP CPU engine on this host's cores on a bounded sample. Prints
-ONE JSON line.
+One "step" is one step of the law `d T = diffuse(k, T)` (conserves T, max_dt = 0.15 h^2/k) on an N^3 fp32 lattice,
+taken by the engine (law text -> IR -> generated gfx950 kernel -> HIP backend) at dt = 0.4 h^2/k / 3, which is one
+fused substep: one read and one write of every cell. The field is seeded uniform [0,1) and resident in HBM; readback
+is lazy and contracts are off in the timed steps (the same step with contracts on is reported beside it). The unit is
+cell-updates/s. Also measured: the handwritten kernel the engine must equal to the bit (and whether it does), a copy
+kernel (the bandwidth roof), the same step at 512^3 (larger than the 256 MB Infinity Cache), and the reference's own
+OpenMP CPU engine on this host's cores on a bounded sample. Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -31,13 +33,21 @@ import field_hip  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 BYTES_PER_CELL = 8.0   # one float read + one float written per cell-update: the algorithmic minimum
 
+HEAT_LAW = """field T init 0 range -1000000 1000000
+param k = 0.1
+term conduction
+  d T = diffuse(k, T)
+  conserves T
+  max_dt = 0.15 * h * h / k
+"""
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
 def time_launches(launch, steps: int, warmup: int):
-    """Whole-run time of `steps` back-to-back launches (ms per launch) and the mean of per-launch event times."""
+    """Whole-run card time of `steps` back-to-back calls (ms per call) and the median of per-call event times."""
     for _ in range(warmup):
         launch()
     field_hip.sync()
@@ -51,44 +61,74 @@ def time_launches(launch, steps: int, warmup: int):
         ev.begin()
         launch()
         per.append(ev.end())
-    return whole, float(np.mean(per)), float(np.median(per))
+    return whole, float(np.median(per))
 
 
-def conduction(n: int, steps: int, warmup: int, seed: int = 2026):
+def conduction(n: int, steps: int, warmup: int, seed: int = 2026, verify_steps: int = 3):
     cells = n ** 3
     k, h = 0.1, 1.0 / n
-    sub_dt = 0.4 * h * h / k / 3  # one of the three substeps a step of dt = 0.4 h^2 / k takes under max_dt 0.15 h^2 / k
-    rng = np.random.default_rng(seed)
-    t0 = rng.random(cells, dtype=np.float32)
+    sub_dt = 0.4 * h * h / k / 3
+    t0 = np.random.default_rng(seed).random(cells, dtype=np.float32)
+    out = {"n": n, "cells": cells}
+
+    # The roof: a copy of the same bytes.
     a = field_hip.DeviceArray.from_host(t0)
     b = field_hip.DeviceArray(cells)
+    _, copy_ms = time_launches(lambda: field_hip.copy_f32(a, b), steps, warmup)
+    out["copy_ms"] = copy_ms
+    out["copy_gbs"] = BYTES_PER_CELL * cells / (copy_ms * 1e6)
+
+    # The handwritten kernel.
     bufs = [a, b]
 
     def hand():
         field_hip.heat_handwritten(bufs[0], bufs[1], n, n, n, k, h, sub_dt)
         bufs.reverse()
 
-    def copy():
-        field_hip.copy_f32(bufs[0], bufs[1])
+    hand_whole, hand_ms = time_launches(hand, steps, warmup)
+    out.update(hand_ms_per_step=hand_whole, hand_kernel_ms=hand_ms, hand_gbs=BYTES_PER_CELL * cells / (hand_ms * 1e6))
 
-    copy_whole, copy_ev, _ = time_launches(copy, steps, warmup)
-    hand_whole, hand_ev, hand_med = time_launches(hand, steps, warmup)
-    return {
-        "n": n, "cells": cells,
-        "copy_ms": copy_ev, "copy_gbs": BYTES_PER_CELL * cells / (copy_ev * 1e6),
-        "hand_ms_per_step": hand_whole, "hand_kernel_ms": hand_ev, "hand_kernel_median_ms": hand_med,
-        "hand_gbs": BYTES_PER_CELL * cells / (hand_ev * 1e6),
-        "hand_cell_updates_per_s": cells / (hand_whole * 1e-3),
-    }
+    # The engine: the law compiled, generated and run by the HIP backend.
+    law = field_hip.Law(HEAT_LAW)
+    solver = field_hip.Solver(law, (n, n, n), h)
+    solver.set_field("T", t0)
+    solver.set_lazy_readback(True)
+    solver.check_contracts_every(1 << 30)
+    solver.step(sub_dt)  # the first step uploads the field and takes the contracts' step
+    eng_whole, _ = time_launches(lambda: solver.step(sub_dt), steps, warmup)
+    per_kernel = []
+    for _ in range(min(steps, 50)):
+        solver.step(sub_dt)
+        per_kernel.append(solver.profile()["terms"][0]["ms"])
+    eng_kernel = float(np.median(per_kernel))
+    out.update(engine_ms_per_step=eng_whole, engine_kernel_ms=eng_kernel,
+               engine_gbs=BYTES_PER_CELL * cells / (eng_kernel * 1e6),
+               engine_cell_updates_per_s=cells / (eng_whole * 1e-3))
+    solver.check_contracts_every(1)
+    with_contracts = []
+    for _ in range(min(steps, 20)):
+        t = time.perf_counter()
+        solver.step(sub_dt)
+        with_contracts.append((time.perf_counter() - t) * 1e3)
+    out["engine_ms_per_step_contracts_on"] = float(np.median(with_contracts))
+    out["engine_contracts_violations"] = solver.violations()
 
-
-HEAT_LAW = """field T init 0 range -1000000 1000000
-param k = 0.1
-term conduction
-  d T = diffuse(k, T)
-  conserves T
-  max_dt = 0.15 * h * h / k
-"""
+    # Bit identity: the same steps from the same start, engine and handwritten.
+    s2 = field_hip.Solver(law, (n, n, n), h)
+    s2.set_field("T", t0)
+    s2.set_lazy_readback(True)
+    s2.check_contracts_every(1 << 30)
+    for _ in range(verify_steps):
+        s2.step(sub_dt)
+    engine_t = s2.get_field("T")
+    a.upload(t0)
+    bufs = [a, b]
+    for _ in range(verify_steps):
+        hand()
+    hand_t = bufs[0].download()
+    out["engine_equals_handwritten_bits"] = bool(np.array_equal(engine_t.view(np.uint32), hand_t.view(np.uint32)))
+    out["verify_steps"] = verify_steps
+    return out
 
 
 def cpu_baseline(n: int, budget_s: float):
@@ -106,7 +146,7 @@ def cpu_baseline(n: int, budget_s: float):
             scene = os.path.join(tmp, "s.scene")
             open(scene, "w").write(f"grid {n} {n} {n} {h!r}\nnoise T 1.0 7\n")
             out = os.path.join(tmp, "t.json")
-            steps = 3
+            steps = 5
             env = dict(os.environ, OMP_NUM_THREADS=str(cores))
             t = time.time()
             subprocess.run([lawc, "--scene", scene, "--steps", str(steps + 1), "--warmup", "1", "--dt", repr(dt), "--json",
@@ -115,17 +155,14 @@ def cpu_baseline(n: int, budget_s: float):
             res = json.load(open(out))
             ms = res["median_ms"]
             return {"value": n ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "reference",
-                    "sample": f"reference lawc (CPU, OpenMP {cores} threads), d T = diffuse(k,T) with its contracts, "
-                              f"{n}^3, {steps} timed steps of one substep after 1 warm-up, median {ms:.1f} ms/step "
-                              f"({wall:.1f} s wall)"}
+                    "sample": f"reference lawc (its CPU engine, OpenMP {cores} threads), d T = diffuse(k,T) with its "
+                              f"contracts, {n}^3, {steps} timed steps of one substep after 1 warm-up, median {ms:.1f} "
+                              f"ms/step ({wall:.1f} s wall)"}
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     from oracle import heat
-    rng = np.random.default_rng(7)
-    t0 = rng.random(n ** 3, dtype=np.float32)
-    times = []
-    t_start = time.time()
-    cur = t0
+    t0 = np.random.default_rng(7).random(n ** 3, dtype=np.float32)
+    times, cur, t_start = [], t0, time.time()
     while time.time() - t_start < budget_s or len(times) < 2:
         t = time.time()
         cur = heat.substep(cur, (n, n, n), 0.1, h, dt)
@@ -153,31 +190,34 @@ def main() -> int:
 
     main_case = conduction(args.size, args.steps, args.warmup)
     log(f"{args.size}^3: {json.dumps(main_case)}")
-    large = None if args.no_large else conduction(512, max(20, args.steps // 8), max(3, args.warmup // 4))
+    large = None if args.no_large else conduction(512, max(20, args.steps // 8), max(3, args.warmup // 4), verify_steps=1)
     if large:
         log(f"512^3: {json.dumps(large)}")
     cpu = None if args.no_cpu else cpu_baseline(args.size, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {json.dumps(cpu)}")
 
-    value = main_case["hand_cell_updates_per_s"]
-    achieved = BYTES_PER_CELL * main_case["cells"] / (main_case["hand_kernel_ms"] * 1e6)
+    achieved = BYTES_PER_CELL * main_case["cells"] / (main_case["engine_kernel_ms"] * 1e6)
     line = {
-        "metric": "cell-updates/s (fused 7-point conduction step)",
-        "value": value,
+        "metric": "cell-updates/s (fused 7-point conduction step through the engine)",
+        "value": main_case["engine_cell_updates_per_s"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": main_case["hand_ms_per_step"],
+        "ms_per_step": main_case["engine_ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 storage, f64 per-cell arithmetic (the reference CPU's)",
         "data": "synthetic (seeded uniform [0,1))",
-        "config": {"workload": f"d T = diffuse(k, T), {args.size}^3 fp32, h=1/{args.size}, k=0.1, one substep of "
-                               f"dt=0.4h^2/k/3, ping-pong, contracts off", "kernel": "handwritten heat_exact",
-                   "large_512": large, "copy_roof_gbs": main_case["copy_gbs"]},
+        "config": {"workload": f"law d T = diffuse(k, T) on {args.size}^3 fp32, h=1/{args.size}, k=0.1, one substep of "
+                               f"dt=0.4h^2/k/3 a step, ping-pong, lazy readback, contracts off",
+                   "engine_equals_handwritten_bits": main_case["engine_equals_handwritten_bits"],
+                   "handwritten": {k: main_case[k] for k in ("hand_kernel_ms", "hand_gbs", "hand_ms_per_step")},
+                   "engine_kernel_ms": main_case["engine_kernel_ms"],
+                   "engine_ms_per_step_contracts_on": main_case["engine_ms_per_step_contracts_on"],
+                   "copy_roof_gbs": main_case["copy_gbs"], "large_512": large},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,

@@ -27,6 +27,18 @@ __device__ __forceinline__ int ex(const K& k, int v) { return v < 0 ? 0 : (v >= 
 __device__ __forceinline__ int ey(const K& k, int v) { return v < 0 ? 0 : (v >= k.ny ? k.ny - 1 : v); }
 __device__ __forceinline__ int ez(const K& k, int v) { return v < 0 ? 0 : (v >= k.nz ? k.nz - 1 : v); }
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// Where a value is read from. Direct: the field's array in HBM. A window kernel has its own (an LDS ring of planes).
+struct Direct {
+    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }
+};
+// Workgroups are dealt round-robin over the 8 XCDs (each with its own L2): consecutive tiles to one XCD keep the halo
+// a tile shares with its neighbour in that XCD's L2 more often.
+__device__ __forceinline__ int xcd_tile(int b, int total)
+{
+    const int per = total / 8, rem = total % 8, xcd = b % 8, local = b / 8;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
 // The cells i along an axis whose clamp(i + d) is j (one run: j - d inside, and what is clamped onto an edge cell).
 __device__ __forceinline__ void run_to(int j, int d, int n, int& lo, int& hi)
 {
@@ -266,14 +278,14 @@ private:
 
     static std::string v(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z")
     {
-        return "v" + std::to_string(id) + "(k, " + x + ", " + y + ", " + z + ")";
+        return "v" + std::to_string(id) + "(k, m, " + x + ", " + y + ", " + z + ")";
     }
 
     // An argument as a double at a point: a scalar as computed, a cell value widened from its float.
     std::string d(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z") const
     {
         if (scalar(id)) {
-            return "s" + std::to_string(id) + "(k)";
+            return "s" + std::to_string(id) + "(k, m)";
         }
         return "(double)" + v(id, x, y, z);
     }
@@ -284,7 +296,7 @@ private:
         const std::string n = std::to_string(id);
         std::ostringstream f;
         if (in.shape == ir::Shape::Scalar) {
-            f << "__device__ __forceinline__ double s" << n << "(const K& k) { ";
+            f << "template <class M> __device__ __forceinline__ double s" << n << "(const K& k, const M& m) { (void)m; ";
             switch (in.op) {
             case Op::Const: f << "return " << literal(in.number) << ";"; break;
             case Op::Param: f << "return k.P[" << in.index << "];"; break;
@@ -304,12 +316,12 @@ private:
         }
         const auto stage = lay_.stages.find(id);
         const bool stored = stage != lay_.stages.end() && stage->second.kind == StageCode::Kind::Stored;
-        const std::string head = "__device__ __forceinline__ float " + std::string(stored ? "b" : "v") + n +
-            "(const K& k, int x, int y, int z)";
+        const std::string head = "template <class M> __device__ __forceinline__ float " + std::string(stored ? "b" : "v") + n +
+            "(const K& k, const M& m, int x, int y, int z)";
         f << head << "\n{\n";
         switch (in.op) {
         case Op::Field:
-            f << "    return k.now[" << in.index << "][IDX(x, y, z)];\n";
+            f << "    return m.template fld<" << in.index << ">(k, x, y, z);\n";
             break;
         case Op::Rim:
             f << "    return (x == 0 || y == 0 || z == 0 || x + 1 == k.nx || y + 1 == k.ny || z + 1 == k.nz) ? 1.0f : 0.0f;\n";
@@ -325,7 +337,7 @@ private:
             const int b = in.args[1];
             if (scalar(a) || scalar(b)) {
                 const int s = scalar(a) ? a : b;
-                f << "    const double s = s" << s << "(k);\n    if (s == 0.0) return 0.0f;\n";
+                f << "    const double s = s" << s << "(k, m);\n    if (s == 0.0) return 0.0f;\n";
             }
             f << "    return fe_rf(" << d(a) << " * " << d(b) << ");\n";
             break;
@@ -366,7 +378,7 @@ private:
             const int fa = in.args[1];
             f << "    const double c = " << d(fa) << ";\n    double acc = 0.0;\n";
             if (scalar(kk)) {
-                f << "    const double kv = s" << kk << "(k);\n"
+                f << "    const double kv = s" << kk << "(k, m);\n"
                   << "    const double kf = (kv + kv) > 0.0 ? 2.0 * kv * kv / (kv + kv) : 0.0;\n";
             } else {
                 f << "    const double ka = " << d(kk) << ";\n";
@@ -437,8 +449,9 @@ private:
         f << "}\n";
         out_ << f.str();
         if (stored) {
-            out_ << "__device__ __forceinline__ float v" << n << "(const K& k, int x, int y, int z) { return AT("
-                 << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z); }\n";
+            out_ << "template <class M> __device__ __forceinline__ float v" << n
+                 << "(const K& k, const M& m, int x, int y, int z) { (void)m; return AT(" << lay_.slot[static_cast<std::size_t>(id)]
+                 << ", x, y, z); }\n";
         }
     }
 
@@ -477,8 +490,8 @@ private:
         const int slot = lay_.slot[static_cast<std::size_t>(id)];
         switch (s.kind) {
         case StageCode::Kind::Stored:
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD AT(" << slot << ", x, y, z) = b" << id
-                 << "(k, x, y, z); }\n";
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD const Direct m; AT(" << slot
+                 << ", x, y, z) = b" << id << "(k, m, x, y, z); }\n";
             break;
         case StageCode::Kind::Reduce: {
             // Pass 1: block b sums (or bounds) cells [b * chunk, ...), lane t every 256th; a fixed tree; L[3b..3b+2].
@@ -488,18 +501,19 @@ private:
                  << "    __shared__ double ss[256], sc[256], lo[256], hi[256];\n"
                  << "    const int t = (int)threadIdx.x; const i64 begin = (i64)blockIdx.x * chunk; i64 end = begin + chunk;\n"
                  << "    if (end > k.cells) end = k.cells;\n";
+            out_ << "    const Direct m;\n";
             if (scalar(arg)) {
-                out_ << "    const double first = s" << arg << "(k);\n";
+                out_ << "    const double first = s" << arg << "(k, m);\n";
             } else {
-                out_ << "    const double first = (double)v" << arg << "(k, 0, 0, 0);\n";
+                out_ << "    const double first = (double)v" << arg << "(k, m, 0, 0, 0);\n";
             }
             out_ << "    Sum s = {0.0, 0.0}; double a = first; double b = first;\n"
                  << "    for (i64 i = begin + t; i < end; i += 256) {\n"
                  << "        const int x = (int)(i % k.nx); const int y = (int)((i / k.nx) % k.ny); const int z = (int)(i / k.nxy);\n";
             if (scalar(arg)) {
-                out_ << "        const double q = s" << arg << "(k);\n";
+                out_ << "        const double q = s" << arg << "(k, m);\n";
             } else {
-                out_ << "        const double q = (double)v" << arg << "(k, x, y, z);\n";
+                out_ << "        const double q = (double)v" << arg << "(k, m, x, y, z);\n";
             }
             out_ << "        add(s, q); a = q < a ? q : a; b = b < q ? q : b; }\n"
                  << "    ss[t] = s.s; sc[t] = s.c; lo[t] = a; hi[t] = b; __syncthreads();\n"
@@ -520,16 +534,16 @@ private:
                 "int x = (int)(line % k.nx); int y = 0; int z = (int)(line / k.nx); const int n = k.ny;",
                 "int x = (int)(line % k.nx); int y = (int)(line / k.nx); int z = 0; const int n = k.nz;"};
             const char* c = s.axis == 0 ? "x" : (s.axis == 1 ? "y" : "z");
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n"
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n    const Direct m;\n"
                  << "    const i64 line = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (line >= (i64)" << lines[s.axis] << ") return;\n"
                  << "    " << coords[s.axis] << "\n"
                  << "    double cp = 0.0; double dp = 0.0;\n"
                  << "    for (int i = 0; i < n; ++i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
                  << "        const double a = " << d(in.args[0]) << "; const double u = " << d(in.args[2]) << ";\n"
                  << "        const double b = " << d(in.args[1]) << " + (i == 0 ? a : 0.0) + (i + 1 == n ? u : 0.0);\n"
-                 << "        const double m = i == 0 ? b : b - a * cp;\n"
-                 << "        cp = i + 1 == n ? 0.0 : u / m;\n"
-                 << "        dp = (" << d(in.args[3]) << " - (i == 0 ? 0.0 : a * dp)) / m;\n"
+                 << "        const double piv = i == 0 ? b : b - a * cp;\n"
+                 << "        cp = i + 1 == n ? 0.0 : u / piv;\n"
+                 << "        dp = (" << d(in.args[3]) << " - (i == 0 ? 0.0 : a * dp)) / piv;\n"
                  << "        k.S[q] = cp; k.S[k.cells + q] = dp; }\n"
                  << "    double next = 0.0;\n"
                  << "    for (int i = n - 1; i >= 0; --i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
@@ -557,7 +571,8 @@ private:
             const bool reads_itself = std::find(p.reads.begin(), p.reads.end(), eq.target) != p.reads.end();
             code.in_place = !reads_itself;
             const std::string t = std::to_string(eq.target);
-            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const i64 i = IDX(x, y, z);\n"
+            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m;\n"
+              << "    const i64 i = IDX(x, y, z);\n"
               << "    const double val = " << d(eq.value) << ";\n    const float old = k.now[" << t << "][i];\n"
               << "    const float nv = fe_rf(" << (p.accumulate ? "(double)old" : "0.0") << " + val);\n"
               << "    const bool ch = __float_as_uint(old) != __float_as_uint(nv);\n";
@@ -572,10 +587,11 @@ private:
             return code;
         }
         code.in_place = p.in_place;
-        g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const i64 i = IDX(x, y, z);\n";
-        // Every rate first, from the pre-state; then each target's deltas in equation order.
+        // The body: every rate first, from the pre-state; then each target's deltas in equation order.
+        std::ostringstream body;
+        body << "    const i64 i = IDX(x, y, z);\n";
         for (std::size_t e = 0; e < p.equations.size(); ++e) {
-            g << "    const double r" << e << " = " << d(p.equations[e].value) << ";\n";
+            body << "    const double r" << e << " = " << d(p.equations[e].value) << ";\n";
         }
         std::vector<int> targets;
         for (const ir::Equation& eq : p.equations) {
@@ -585,32 +601,162 @@ private:
         }
         for (const int target : targets) {
             const std::string t = std::to_string(target);
-            g << "    { const float o = k.now[" << t << "][i]; float n = o;\n";
+            body << "    { const float o = m.template fld<" << t << ">(k, x, y, z); float n = o;\n";
             for (std::size_t e = 0; e < p.equations.size(); ++e) {
                 if (p.equations[e].target != target) {
                     continue;
                 }
                 if (scalar(p.equations[e].value)) {
                     // A rate that is one number and 0 is idle on the CPU: nothing is added.
-                    g << "      if (r" << e << " != 0.0) n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                    body << "      if (r" << e << " != 0.0) n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
                 } else {
-                    g << "      n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                    body << "      n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
                 }
             }
-            g << "      const bool ch = __float_as_uint(o) != __float_as_uint(n);\n";
+            body << "      const bool ch = __float_as_uint(o) != __float_as_uint(n);\n";
             if (code.in_place) {
-                g << "      if (ch) k.now[" << t << "][i] = n;\n";
+                body << "      if (ch) k.now[" << t << "][i] = n;\n";
             } else {
-                g << "      k.nxt[" << t << "][i] = n;\n";
+                body << "      k.nxt[" << t << "][i] = n;\n";
             }
-            g << "      mark(k, " << t << ", ch); }\n";
+            body << "      mark(k, " << t << ", ch); }\n";
         }
-        g << "}\n";
         if (!code.in_place) {
             code.swaps = targets;
         }
-        out_ << g.str();
+        std::map<int, int> reach;
+        int cost = 0;
+        const bool bounded = field_reach(p, reach, cost);
+        int radius = 0;
+        std::vector<int> windowed;
+        for (const auto& [f, r] : reach) {
+            if (r > 0) {
+                windowed.push_back(f);
+                radius = std::max(radius, r);
+            }
+        }
+        if (bounded && radius >= 1 && radius <= 2 && windowed.size() <= (radius == 1 ? 4u : 3u) && cost <= 4000) {
+            window_kernel(code, body.str(), windowed, radius);
+        } else {
+            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m;\n" << body.str() << "}\n";
+            out_ << g.str();
+        }
         return code;
+    }
+
+    // How far from its cell the gather reads each field through what it inlines (stages are read where they lie).
+    // False when a read's reach is not bounded by the instructions' radii (the transposes walk runs of cells).
+    bool field_reach(const ir::Program& p, std::map<int, int>& out, int& cost) const
+    {
+        bool bounded = true;
+        std::set<std::pair<int, int>> seen;
+        std::function<void(int, int, bool)> walk = [&](int id, int off, bool root) {
+            if (!seen.insert({id, off}).second) {
+                return;
+            }
+            const ir::Inst& in = at(id);
+            if (!root && lay_.stages.count(id) != 0) {
+                return;
+            }
+            if (root && lay_.stages.count(id) != 0) {
+                return; // the whole value was computed before: read at its own cell
+            }
+            if (in.op == Op::Field) {
+                int& r = out[in.index];
+                r = std::max(r, off);
+                return;
+            }
+            if (in.op == Op::ShiftT || in.op == Op::GxT || in.op == Op::GyT || in.op == Op::GzT) {
+                bounded = false;
+            }
+            for (const int a : in.args) {
+                walk(a, off + in.radius, false);
+            }
+        };
+        cost = 0;
+        for (const ir::Equation& eq : p.equations) {
+            walk(eq.value, 0, true);
+            cost += at(eq.value).cost;
+        }
+        return bounded;
+    }
+
+    // A gather over tiles of 64 x TY cells marching in z: the fields it reads off-cell come from an LDS ring of
+    // planes (radius R: the ring holds planes z-R..z+R and the one being filled), each plane of each field loaded
+    // once by its tile (rows and halo columns at clamped coordinates), two planes ahead of use. The arithmetic is the
+    // same as the direct kernel's, read through the window: the same bits.
+    void window_kernel(ProgramCode& code, const std::string& body, const std::vector<int>& wf, int radius)
+    {
+        const int R = radius;
+        const int TY = R == 1 ? 6 : 4;
+        const int RS = R == 1 ? 4 : 8; // ring slots: a power of two >= 2R + 2
+        const int TW = 64 + 2 * R;
+        const int TP = (TY + 2 * R) * TW;
+        const int RF = RS * TP;
+        const int NW = static_cast<int>(wf.size());
+        code.window = true;
+        code.radius = R;
+        code.ty = TY;
+        code.zchunk = 32;
+        code.block_y = TY + 2 * R;
+        const std::string W = "W_" + code.gather;
+        std::ostringstream s;
+        s << "struct " << W << " {\n    float* lds; int x0; int y0;\n"
+          << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const {\n";
+        for (int w = 0; w < NW; ++w) {
+            s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return lds["
+              << w * RF << " + (z & " << RS - 1 << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + " << R
+              << ")];\n";
+        }
+        s << "        " << (NW ? "else " : "") << "return k.now[F][IDX(x, y, z)];\n    }\n};\n";
+        s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << code.gather
+          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
+          << "    __shared__ float lds[" << NW * RF << "];\n"
+          << "    const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z);\n"
+          << "    const int x0 = (t % tiles_x) * 64; const int y0 = ((t / tiles_x) % tiles_y) * " << TY << ";\n"
+          << "    const int z0 = (t / (tiles_x * tiles_y)) * zchunk; const int z1 = z0 + zchunk < k.nz ? z0 + zchunk : k.nz;\n"
+          << "    const int lane = (int)threadIdx.x; const int r = (int)threadIdx.y;\n"
+          << "    const int x = x0 + lane; const int y = y0 + r - " << R << ";\n"
+          << "    const bool live = r >= " << R << " && r < " << R + TY << " && x < k.nx && y < k.ny;\n"
+          << "    const int xs = clampi(x, 0, k.nx - 1); const int ys = clampi(y, 0, k.ny - 1);\n"
+          // The halo column each lane also loads (lanes 0..R-1 the left, 64-R..63 the right; the others their own).
+          << "    const int hslot = lane < " << R << " ? lane : (lane >= " << 64 - R << " ? lane + " << 2 * R << " : -1);\n"
+          << "    const int hx = clampi(lane < " << R << " ? x0 - " << R << " + lane : (lane >= " << 64 - R << " ? x0 + lane + " << R
+          << " : x), 0, k.nx - 1);\n"
+          << "    const i64 rowo = (i64)ys * k.nx;\n"
+          << "    const int zlim = z1 - 1 + " << R << " < k.nz - 1 ? z1 - 1 + " << R << " : k.nz - 1;\n"
+          << "    " << W << " m; m.lds = lds; m.x0 = x0; m.y0 = y0;\n"
+          << "    float* mine = lds + r * " << TW << " + lane + " << R << ";\n"
+          << "    float* halo = lds + r * " << TW << " + (hslot < 0 ? 0 : hslot);\n"
+          << "    struct Pl { float v[" << NW << "]; float h[" << NW << "]; };\n"
+          // Unconditional loads at clamped planes: a load under a branch makes the compiler wait for every load.
+          << "    auto load = [&](Pl& p, int zp) { const i64 o = (i64)clampi(zp, 0, zlim) * k.nxy + rowo;\n";
+        for (int w = 0; w < NW; ++w) {
+            s << "        p.v[" << w << "] = k.now[" << wf[static_cast<std::size_t>(w)] << "][o + xs]; p.h[" << w << "] = k.now["
+              << wf[static_cast<std::size_t>(w)] << "][o + hx];\n";
+        }
+        s << "    };\n"
+          << "    auto put = [&](const Pl& p, int zp) { const int sl = (zp & " << RS - 1 << ") * " << TP << ";\n";
+        for (int w = 0; w < NW; ++w) {
+            s << "        mine[" << w * RF << " + sl] = p.v[" << w << "]; if (hslot >= 0) halo[" << w * RF << " + sl] = p.h[" << w << "];\n";
+        }
+        s << "    };\n"
+          << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); }\n"
+          << "    Pl A, B, C, D;\n"
+          << "    load(A, z0 + " << R << "); load(B, z0 + " << R + 1 << "); load(C, z0 + " << R + 2 << ");\n"
+          << "    auto step = [&](const int z, const Pl& cur, Pl& far) {\n"
+          << "        load(far, z + " << R + 3 << ");\n"
+          << "        put(cur, z + " << R << ");\n"
+          << "        __syncthreads();\n"
+          << "        if (live) {\n" << body << "        }\n"
+          << "    };\n"
+          << "    for (int z = z0; z < z1; z += 4) {\n"
+          << "        step(z, A, D); if (z + 1 >= z1) break;\n"
+          << "        step(z + 1, B, A); if (z + 2 >= z1) break;\n"
+          << "        step(z + 2, C, B); if (z + 3 >= z1) break;\n"
+          << "        step(z + 3, D, C);\n"
+          << "    }\n}\n";
+        out_ << s.str();
     }
 
     const ir::Module& m_;

@@ -75,7 +75,9 @@ struct Impl final : LawBackend {
     std::string adapter_name;
     fe_hip_module mod = nullptr;
     std::map<std::string, fe_hip_function> fns;
-    StepProfile prof;
+    mutable StepProfile prof;
+    mutable bool timings_pending = false;
+    std::size_t derive_event = 0;
     int check_every = 1;
     long long step_count = 0;
     bool lazy = false;
@@ -101,7 +103,11 @@ struct Impl final : LawBackend {
 
     [[nodiscard]] const std::string& adapter() const override { return adapter_name; }
     [[nodiscard]] const char* api() const override { return "HIP"; }
-    [[nodiscard]] const StepProfile& profile() const override { return prof; }
+    [[nodiscard]] const StepProfile& profile() const override
+    {
+        resolve_timings();
+        return prof;
+    }
     void check_contracts_every(int steps) override { check_every = std::max(1, steps); }
     void set_lazy(bool on) override { lazy = on; }
 
@@ -156,6 +162,28 @@ struct Impl final : LawBackend {
         const unsigned gy = (static_cast<unsigned>(shape.ny) + kBlockY - 1) / kBlockY;
         check(fe_hip_launch(fn(kernel), gx, gy, static_cast<unsigned>(shape.nz), kBlockX, kBlockY, 1, 0, argv.data()),
             ("launch " + kernel).c_str());
+        ++prof.launches;
+    }
+
+    // A term's gather: a thread a cell, or (window kernels) z-marching tiles of 64 x ty cells.
+    void launch_gather(const hip::ProgramCode& code, double dt)
+    {
+        if (!code.window) {
+            launch_cells(code.gather, dt);
+            return;
+        }
+        int zchunk = code.zchunk;
+        int tx = (shape.nx + 63) / 64;
+        int ty = (shape.ny + code.ty - 1) / code.ty;
+        int tz = (shape.nz + zchunk - 1) / zchunk;
+        const long long total = 1LL * tx * ty * tz;
+        if (total > 0x7fffffffLL) {
+            throw std::runtime_error("lattice too large for one launch");
+        }
+        std::vector<std::uint64_t> k = args(dt);
+        void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz};
+        check(fe_hip_launch(fn(code.gather), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(code.block_y), 1, 0, argv),
+            ("launch " + code.gather).c_str());
         ++prof.launches;
     }
 
@@ -445,7 +473,7 @@ struct Impl final : LawBackend {
             for (const int st : code.stages) {
                 run_stage(st, sub_dt);
             }
-            launch_cells(code.gather, sub_dt);
+            launch_gather(code, sub_dt);
             swap(code.swaps);
         }
         if (contracts) {
@@ -694,32 +722,59 @@ struct Impl final : LawBackend {
             tp.launches = prof.launches - launches0;
             ev += 2;
         }
-        const std::size_t derive_ev = ev;
+        derive_event = ev;
         check(fe_hip_event_record(event(ev++)), "event");
         run_derives(contracts, judges);
         check(fe_hip_event_record(event(ev++)), "event");
         if (contracts) {
             judge(judges, violations);
         }
-        read_changed(changed);
+        if (lazy && !contracts) {
+            // Nothing comes back and nothing is waited for: every field a program writes is taken as changed, and the
+            // card's time is read from the events when the profile is asked for.
+            changed.assign(nf, 0);
+            for (const auto* progs : {&module.terms, &module.learned, &module.derives}) {
+                for (const ir::Program& p : *progs) {
+                    for (const int f : p.writes) {
+                        changed[static_cast<std::size_t>(f)] = 1;
+                    }
+                }
+            }
+        } else {
+            read_changed(changed);
+        }
         for (std::size_t f = 0; f < nf; ++f) {
             stale[f] = stale[f] || changed[f];
         }
         if (!lazy) {
             download_changed(fields, changed);
         }
-        check(fe_hip_event_sync(event(ev - 1)), "event sync");
-        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+        timings_pending = true;
+        if (!lazy) {
+            resolve_timings();
+        }
+        prof.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // Card time per term and for the derives, from the last step's events (waits for them).
+    void resolve_timings() const
+    {
+        if (!timings_pending) {
+            return;
+        }
+        timings_pending = false;
+        fe_hip_event_sync(events[derive_event + 1]);
+        double gpu = 0.0;
+        for (std::size_t t = 0; t < prof.terms.size(); ++t) {
             float ms = 0.0f;
-            fe_hip_event_elapsed(event(2 * t), event(2 * t + 1), &ms);
+            fe_hip_event_elapsed(events[2 * t], events[2 * t + 1], &ms);
             prof.terms[t].ms = ms;
-            prof.gpu_ms += ms;
+            gpu += ms;
         }
         float dms = 0.0f;
-        fe_hip_event_elapsed(event(derive_ev), event(derive_ev + 1), &dms);
+        fe_hip_event_elapsed(events[derive_event], events[derive_event + 1], &dms);
         prof.derive_ms = dms;
-        prof.gpu_ms += dms;
-        prof.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        prof.gpu_ms = gpu + dms;
     }
 
     void derive(Law& l, FieldSet& fields, FieldEdits& edits, std::vector<char>& changed) override
