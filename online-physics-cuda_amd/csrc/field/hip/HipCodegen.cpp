@@ -23,15 +23,22 @@ typedef unsigned long long u64;
 #define AT(slot, x, y, z) k.A[(i64)(slot) * k.cells + IDX(x, y, z)]
 #define CELL_HEAD const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x); const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y); \
     const int z = (int)blockIdx.z; if (x >= k.nx || y >= k.ny) return;
-__device__ __forceinline__ int ex(const K& k, int v) { return v < 0 ? 0 : (v >= k.nx ? k.nx - 1 : v); }
-__device__ __forceinline__ int ey(const K& k, int v) { return v < 0 ? 0 : (v >= k.ny ? k.ny - 1 : v); }
-__device__ __forceinline__ int ez(const K& k, int v) { return v < 0 ? 0 : (v >= k.nz ? k.nz - 1 : v); }
+// Clamps written as max(0, min(v, n - 1)): one v_med3_i32.
+__device__ __forceinline__ int ex(const K& k, int v) { return max(0, min(v, k.nx - 1)); }
+__device__ __forceinline__ int ey(const K& k, int v) { return max(0, min(v, k.ny - 1)); }
+__device__ __forceinline__ int ez(const K& k, int v) { return max(0, min(v, k.nz - 1)); }
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-// Where a value is read from. Direct: the field's array in HBM. A window kernel has its own (an LDS ring of planes).
-struct Direct {
-    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }
-};
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(lo, min(v, hi)); }
+// a / h, a / h^2, a / (h * span): when the divisor is a power of two (flags, set by the host) the quotient is the exact
+// product with its reciprocal, so the multiply gives the division's bits without the division's cost.
+__device__ __forceinline__ double div_h(const K& k, double a) { return (k.flags & 1) ? a * k.inv_h : a / k.h; }
+__device__ __forceinline__ double div_hh(const K& k, double a) { return (k.flags & 2) ? a * k.inv_hh : a / (k.h * k.h); }
+__device__ __forceinline__ double div_hs(const K& k, double a, int span)
+{
+    return (k.flags & 1) && span > 0 ? a * (span == 1 ? k.inv_h : 0.5 * k.inv_h) : a / (k.h * (double)span);
+}
+// Where a value is read from (struct Direct, generated after this prelude: the field's array in HBM; a window kernel
+// has its own, an LDS ring of planes).
 // Workgroups are dealt round-robin over the 8 XCDs (each with its own L2): consecutive tiles to one XCD keep the halo
 // a tile shares with its neighbour in that XCD's L2 more often.
 __device__ __forceinline__ int xcd_tile(int b, int total)
@@ -123,6 +130,14 @@ extern "C" __global__ __launch_bounds__(256) void k_total_final(const double* pa
     if (t == 0) { dst[0] = ss[0] + sc[0]; dst[1] = ms[0] + mc[0]; }
 }
 
+// Clears n words (the change flags) with a kernel: a graph replays kernels, and fills recorded as memset nodes have
+// given wrong results on replay on this hardware.
+extern "C" __global__ void k_zero(unsigned* p, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0u;
+}
+
 // A reduction's partials (sum, least, most per block, from k_reduce passes) folded in order into U[uslot].
 extern "C" __global__ __launch_bounds__(256) void k_reduce_final(K k, int kind, int parts, int uslot)
 {
@@ -182,12 +197,23 @@ public:
         const int nf = std::max(1, lay_.fields);
         out_ << "// generated by the Field Engine's --hip emitter for gfx950\n";
         out_ << FE_EXACT_HELPERS_TEXT;
+        // The params travel by value in the arguments (the kernarg segment is read-only: the compiler hoists every read
+        // of them out of loops), so a step's launches carry the values they were queued with.
+        const int np = std::max(1, static_cast<int>(law_.param_values.size()));
+        const int nu = std::max(1, lay_.uniforms);
         out_ << "struct K {\n    float* now[" << nf << "];\n    float* nxt[" << nf << "];\n"
-             << "    const double* P;\n    const double* U;\n    float* A;\n    double* S;\n    double* L;\n    unsigned* C;\n"
-             << "    long long nxy;\n    long long cells;\n    double h;\n    double dt;\n    int nx;\n    int ny;\n    int nz;\n"
-             << "    int pad;\n};\n";
-        lay_.kernel_arg_bytes = (2 * nf + 6) * 8 + 4 * 8 + 4 * 4;
+             << "    const double* U;\n    float* A;\n    double* S;\n    double* L;\n    unsigned* C;\n"
+             << "    long long nxy;\n    long long cells;\n    double h;\n    double dt;\n    double inv_h;\n    double inv_hh;\n"
+             << "    int nx;\n    int ny;\n    int nz;\n    int flags;\n    double P[" << np << "];\n};\n";
+        lay_.kernel_arg_bytes = (2 * nf + 5) * 8 + 6 * 8 + 4 * 4 + 8 * np;
         out_ << kPrelude;
+        // Reductions' results are read once a kernel into registers (a load inside the z loop would be repeated and,
+        // waited for, would drain the loads in flight).
+        out_ << "struct Direct {\n    double u[" << nu << "];\n"
+             << "    __device__ explicit Direct(const K& k) { for (int q = 0; q < " << nu << "; ++q) u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
+             << "; }\n"
+             << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
+             << "};\n";
         for (std::size_t id = 0; id < m_.insts.size(); ++id) {
             if (used(static_cast<int>(id))) {
                 emit(static_cast<int>(id));
@@ -305,7 +331,7 @@ private:
             case Op::Local: f << "return 0.0;"; break;
             case Op::MaxOf: case Op::MinOf: case Op::SumOf: {
                 const int u = lay_.uslot[static_cast<std::size_t>(id)];
-                f << "return " << (u >= 0 ? "k.U[" + std::to_string(u) + "]" : std::string("0.0")) << ";";
+                f << "return " << (u >= 0 ? "m.u[" + std::to_string(u) + "]" : std::string("0.0")) << ";";
                 break;
             }
             default: f << "return " << arith(in, id) << ";"; break;
@@ -350,7 +376,7 @@ private:
               << "    s = s + " << d(in.args[0], "x", "ey(k, y + 1)") << ";\n"
               << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z - 1)") << ";\n"
               << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z + 1)") << ";\n"
-              << "    const double o = (s - 6.0 * c) / (k.h * k.h);\n"
+              << "    const double o = div_hh(k, s - 6.0 * c);\n"
               << "    return fe_rf(finite_d(o) ? o : 0.0);\n";
             break;
         case Op::Gx: case Op::Gy: case Op::Gz: {
@@ -362,7 +388,7 @@ private:
             f << "    const int hi = " << hi << ";\n    const int lo = " << lo << ";\n";
             const std::string ahi = axis == 0 ? d(in.args[0], "hi") : (axis == 1 ? d(in.args[0], "x", "hi") : d(in.args[0], "x", "y", "hi"));
             const std::string alo = axis == 0 ? d(in.args[0], "lo") : (axis == 1 ? d(in.args[0], "x", "lo") : d(in.args[0], "x", "y", "lo"));
-            const std::string body = "(" + ahi + " - " + alo + ") / (k.h * (double)(hi - lo))";
+            const std::string body = "div_hs(k, " + ahi + " - " + alo + ", hi - lo)";
             if (axis == 0) {
                 f << "    const double o = " << body << ";\n";
             } else {
@@ -376,6 +402,8 @@ private:
             // kf = 2 ka kb / (ka + kb) (src/field/Law.cpp:986-1018), a missing face skipped.
             const int kk = in.args[0];
             const int fa = in.args[1];
+            // Every neighbour is read at its clamped coordinate and a missing face adds an exact +0: the sum starts at +0
+            // and can never become -0, so adding +0 changes no bit, and the reads issue together (no branches).
             f << "    const double c = " << d(fa) << ";\n    double acc = 0.0;\n";
             if (scalar(kk)) {
                 f << "    const double kv = s" << kk << "(k, m);\n"
@@ -383,36 +411,46 @@ private:
             } else {
                 f << "    const double ka = " << d(kk) << ";\n";
             }
-            const char* sides[6][4] = {{"x > 0", "x - 1", "y", "z"}, {"x + 1 < k.nx", "x + 1", "y", "z"},
-                {"y > 0", "x", "y - 1", "z"}, {"y + 1 < k.ny", "x", "y + 1", "z"}, {"z > 0", "x", "y", "z - 1"},
-                {"z + 1 < k.nz", "x", "y", "z + 1"}};
-            for (const auto& s : sides) {
-                f << "    if (" << s[0] << ") { ";
+            const char* sides[6][4] = {{"x > 0", "ex(k, x - 1)", "y", "z"}, {"x + 1 < k.nx", "ex(k, x + 1)", "y", "z"},
+                {"y > 0", "x", "ey(k, y - 1)", "z"}, {"y + 1 < k.ny", "x", "ey(k, y + 1)", "z"}, {"z > 0", "x", "y", "ez(k, z - 1)"},
+                {"z + 1 < k.nz", "x", "y", "ez(k, z + 1)"}};
+            for (int q = 0; q < 6; ++q) {
+                const auto& s = sides[q];
+                f << "    const double n" << q << " = " << d(fa, s[1], s[2], s[3]) << ";\n";
                 if (!scalar(kk)) {
-                    f << "const double kb = " << d(kk, s[1], s[2], s[3]) << "; const double kf = (ka + kb) > 0.0 ? 2.0 * ka * kb / (ka + kb) : 0.0; ";
+                    f << "    const double kb" << q << " = " << d(kk, s[1], s[2], s[3]) << ";\n";
                 }
-                f << "acc += kf * (" << d(fa, s[1], s[2], s[3]) << " - c); }\n";
             }
-            f << "    return fe_rf(acc / (k.h * k.h));\n";
+            for (int q = 0; q < 6; ++q) {
+                const auto& s = sides[q];
+                if (scalar(kk)) {
+                    f << "    acc += (" << s[0] << ") ? kf * (n" << q << " - c) : 0.0;\n";
+                } else {
+                    f << "    { const double kf = (ka + kb" << q << ") > 0.0 ? 2.0 * ka * kb" << q << " / (ka + kb" << q
+                      << ") : 0.0; acc += (" << s[0] << ") ? kf * (n" << q << " - c) : 0.0; }\n";
+                }
+            }
+            f << "    return fe_rf(div_hh(k, acc));\n";
             break;
         }
         case Op::Adv: {
-            // -(u . grad F), upwind (src/field/Law.cpp:1020-1040).
+            // -(u . grad F), upwind (src/field/Law.cpp:1020-1040); neighbours read first, the upwind choice a select.
             const int fa = in.args[0];
             f << "    const double c = " << d(fa) << ";\n"
               << "    const double u = " << d(in.args[1]) << ";\n    const double w = " << d(in.args[2]) << ";\n"
-              << "    const double s = " << d(in.args[3]) << ";\n    double o = 0.0;\n";
-            auto up = [&](const char* speed, const char* c, const char* cnt, const std::string& lo, const std::string& hi) {
-                f << "    { const double q = " << speed << " > 0.0 ? (" << c << " > 0 ? " << speed << " * (c - " << lo
-                  << ") / k.h : 0.0) : (" << c << " + 1 < " << cnt << " ? " << speed << " * (" << hi << " - c) / k.h : 0.0);\n";
+              << "    const double s = " << d(in.args[3]) << ";\n"
+              << "    const double xl = " << d(fa, "ex(k, x - 1)") << "; const double xh = " << d(fa, "ex(k, x + 1)") << ";\n"
+              << "    const double yl = " << d(fa, "x", "ey(k, y - 1)") << "; const double yh = " << d(fa, "x", "ey(k, y + 1)") << ";\n"
+              << "    const double zl = " << d(fa, "x", "y", "ez(k, z - 1)") << "; const double zh = " << d(fa, "x", "y", "ez(k, z + 1)")
+              << ";\n";
+            auto up = [&](const char* speed, const char* c, const char* cnt, const char* lo, const char* hi) {
+                return std::string("(") + speed + " > 0.0 ? (" + c + " > 0 ? div_h(k, " + speed + " * (c - " + lo + ")) : 0.0) : (" + c +
+                    " + 1 < " + cnt + " ? div_h(k, " + speed + " * (" + hi + " - c)) : 0.0))";
             };
-            up("u", "x", "k.nx", d(fa, "ex(k, x - 1)"), d(fa, "ex(k, x + 1)"));
-            f << "      o = q; }\n";
-            up("w", "y", "k.ny", d(fa, "x", "ey(k, y - 1)"), d(fa, "x", "ey(k, y + 1)"));
-            f << "      o = o + q; }\n";
-            up("s", "z", "k.nz", d(fa, "x", "y", "ez(k, z - 1)"), d(fa, "x", "y", "ez(k, z + 1)"));
-            f << "      o = o + q; }\n";
-            f << "    return fe_rf(-o);\n";
+            f << "    double o = " << up("u", "x", "k.nx", "xl", "xh") << ";\n"
+              << "    o = o + " << up("w", "y", "k.ny", "yl", "yh") << ";\n"
+              << "    o = o + " << up("s", "z", "k.nz", "zl", "zh") << ";\n"
+              << "    return fe_rf(-o);\n";
             break;
         }
         case Op::ShiftT: {
@@ -490,7 +528,7 @@ private:
         const int slot = lay_.slot[static_cast<std::size_t>(id)];
         switch (s.kind) {
         case StageCode::Kind::Stored:
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD const Direct m; AT(" << slot
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD const Direct m(k); AT(" << slot
                  << ", x, y, z) = b" << id << "(k, m, x, y, z); }\n";
             break;
         case StageCode::Kind::Reduce: {
@@ -501,7 +539,7 @@ private:
                  << "    __shared__ double ss[256], sc[256], lo[256], hi[256];\n"
                  << "    const int t = (int)threadIdx.x; const i64 begin = (i64)blockIdx.x * chunk; i64 end = begin + chunk;\n"
                  << "    if (end > k.cells) end = k.cells;\n";
-            out_ << "    const Direct m;\n";
+            out_ << "    const Direct m(k);\n";
             if (scalar(arg)) {
                 out_ << "    const double first = s" << arg << "(k, m);\n";
             } else {
@@ -534,7 +572,7 @@ private:
                 "int x = (int)(line % k.nx); int y = 0; int z = (int)(line / k.nx); const int n = k.ny;",
                 "int x = (int)(line % k.nx); int y = (int)(line / k.nx); int z = 0; const int n = k.nz;"};
             const char* c = s.axis == 0 ? "x" : (s.axis == 1 ? "y" : "z");
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n    const Direct m;\n"
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n    const Direct m(k);\n"
                  << "    const i64 line = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (line >= (i64)" << lines[s.axis] << ") return;\n"
                  << "    " << coords[s.axis] << "\n"
                  << "    double cp = 0.0; double dp = 0.0;\n"
@@ -571,7 +609,7 @@ private:
             const bool reads_itself = std::find(p.reads.begin(), p.reads.end(), eq.target) != p.reads.end();
             code.in_place = !reads_itself;
             const std::string t = std::to_string(eq.target);
-            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m;\n"
+            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m(k);\n"
               << "    const i64 i = IDX(x, y, z);\n"
               << "    const double val = " << d(eq.value) << ";\n    const float old = k.now[" << t << "][i];\n"
               << "    const float nv = fe_rf(" << (p.accumulate ? "(double)old" : "0.0") << " + val);\n"
@@ -638,7 +676,7 @@ private:
         if (bounded && radius >= 1 && radius <= 2 && windowed.size() <= (radius == 1 ? 4u : 3u) && cost <= 4000) {
             window_kernel(code, body.str(), windowed, radius);
         } else {
-            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m;\n" << body.str() << "}\n";
+            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m(k);\n" << body.str() << "}\n";
             out_ << g.str();
         }
         return code;
@@ -701,7 +739,7 @@ private:
         code.block_y = TY + 2 * R;
         const std::string W = "W_" + code.gather;
         std::ostringstream s;
-        s << "struct " << W << " {\n    float* lds; int x0; int y0;\n"
+        s << "struct " << W << " {\n    float* lds; int x0; int y0; double u[" << std::max(1, lay_.uniforms) << "];\n"
           << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const {\n";
         for (int w = 0; w < NW; ++w) {
             s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return lds["
@@ -726,6 +764,8 @@ private:
           << "    const i64 rowo = (i64)ys * k.nx;\n"
           << "    const int zlim = z1 - 1 + " << R << " < k.nz - 1 ? z1 - 1 + " << R << " : k.nz - 1;\n"
           << "    " << W << " m; m.lds = lds; m.x0 = x0; m.y0 = y0;\n"
+          << "    for (int q = 0; q < " << std::max(1, lay_.uniforms) << "; ++q) m.u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
+          << ";\n"
           << "    float* mine = lds + r * " << TW << " + lane + " << R << ";\n"
           << "    float* halo = lds + r * " << TW << " + (hslot < 0 ? 0 : hslot);\n"
           << "    struct Pl { float v[" << NW << "]; float h[" << NW << "]; };\n"

@@ -127,7 +127,8 @@ struct Impl final : LawBackend {
     std::vector<std::uint64_t> args(double dt) const
     {
         const int nf = std::max(1, static_cast<int>(law->fields.size()));
-        std::vector<std::uint64_t> k(static_cast<std::size_t>(2 * nf + 6 + 4 + 2), 0);
+        const int np = std::max(1, static_cast<int>(law->param_values.size()));
+        std::vector<std::uint64_t> k(static_cast<std::size_t>(2 * nf + 5 + 6 + 2 + np), 0);
         std::size_t w = 0;
         for (int f = 0; f < nf; ++f) {
             k[w++] = reinterpret_cast<std::uint64_t>(f < static_cast<int>(now.size()) ? now[static_cast<std::size_t>(f)] : nullptr);
@@ -135,7 +136,6 @@ struct Impl final : LawBackend {
         for (int f = 0; f < nf; ++f) {
             k[w++] = reinterpret_cast<std::uint64_t>(f < static_cast<int>(nxt.size()) ? nxt[static_cast<std::size_t>(f)] : nullptr);
         }
-        k[w++] = reinterpret_cast<std::uint64_t>(P.p);
         k[w++] = reinterpret_cast<std::uint64_t>(U.p);
         k[w++] = reinterpret_cast<std::uint64_t>(A.p);
         k[w++] = reinterpret_cast<std::uint64_t>(S.p);
@@ -147,8 +147,21 @@ struct Impl final : LawBackend {
         std::memcpy(&k[w++], &cells, 8);
         std::memcpy(&k[w++], &shape.cell, 8);
         std::memcpy(&k[w++], &dt, 8);
-        const int ints[4] = {shape.nx, shape.ny, shape.nz, 0};
+        // 1 / h and 1 / h^2, used only where h (h^2) is a power of two: then a product with them is the quotient's bits.
+        int e = 0;
+        const bool h_pow2 = std::frexp(shape.cell, &e) == 0.5;
+        const double hh = shape.cell * shape.cell;
+        const bool hh_pow2 = std::frexp(hh, &e) == 0.5;
+        const double inv_h = h_pow2 ? 1.0 / shape.cell : 0.0;
+        const double inv_hh = hh_pow2 ? 1.0 / hh : 0.0;
+        std::memcpy(&k[w++], &inv_h, 8);
+        std::memcpy(&k[w++], &inv_hh, 8);
+        const int ints[4] = {shape.nx, shape.ny, shape.nz, (h_pow2 ? 1 : 0) | (hh_pow2 ? 2 : 0)};
         std::memcpy(&k[w], ints, 16);
+        w += 2;
+        for (std::size_t p = 0; p < law->param_values.size(); ++p) {
+            std::memcpy(&k[w++], &law->param_values[p], 8); // the params by value (struct K's last member)
+        }
         return k;
     }
 
