@@ -27,6 +27,12 @@ typedef unsigned long long u64;
 __device__ __forceinline__ int ex(const K& k, int v) { return max(0, min(v, k.nx - 1)); }
 __device__ __forceinline__ int ey(const K& k, int v) { return max(0, min(v, k.ny - 1)); }
 __device__ __forceinline__ int ez(const K& k, int v) { return max(0, min(v, k.nz - 1)); }
+// A field read at a neighbour's coordinate (in a function templated on its memory M): clamped into the lattice where M
+// reads HBM; a window's LDS holds every coordinate within its reach at the clamped cell's value, so there the
+// coordinate goes unclamped and the neighbours' addresses stay constant offsets of the cell's.
+#define LX(v) (M::kClamps ? ex(k, (v)) : (v))
+#define LY(v) (M::kClamps ? ey(k, (v)) : (v))
+#define LZ(v) (M::kClamps ? ez(k, (v)) : (v))
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(lo, min(v, hi)); }
 // a / h, a / h^2, a / (h * span): when the divisor is a power of two (flags, set by the host) the quotient is the exact
@@ -209,7 +215,7 @@ public:
         out_ << kPrelude;
         // Reductions' results are read once a kernel into registers (a load inside the z loop would be repeated and,
         // waited for, would drain the loads in flight).
-        out_ << "struct Direct {\n    double u[" << nu << "];\n"
+        out_ << "struct Direct {\n    static constexpr bool kClamps = true;\n    double u[" << nu << "];\n"
              << "    __device__ explicit Direct(const K& k) { for (int q = 0; q < " << nu << "; ++q) u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
              << "; }\n"
              << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
@@ -302,6 +308,16 @@ private:
         }
     }
 
+    // A neighbour's coordinate along axis a (0 x, 1 y, 2 z) for reading argument `arg` at `v`: clamped into the lattice,
+    // or through LX/LY/LZ when the argument is a field itself (the memory it is read from decides).
+    std::string cl(int arg, int a, const std::string& v) const
+    {
+        static const char* direct[3] = {"ex(k, ", "ey(k, ", "ez(k, "};
+        static const char* leaf[3] = {"LX(", "LY(", "LZ("};
+        const bool is_leaf = at(arg).op == Op::Field && lay_.stages.count(arg) == 0;
+        return std::string(is_leaf ? leaf[a] : direct[a]) + v + ")";
+    }
+
     static std::string v(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z")
     {
         return "v" + std::to_string(id) + "(k, m, " + x + ", " + y + ", " + z + ")";
@@ -352,11 +368,13 @@ private:
         case Op::Rim:
             f << "    return (x == 0 || y == 0 || z == 0 || x + 1 == k.nx || y + 1 == k.ny || z + 1 == k.nz) ? 1.0f : 0.0f;\n";
             break;
-        case Op::Shift:
-            f << "    return " << v(in.args[0], "ex(k, x + (" + std::to_string(in.offset[0]) + "))",
-                "ey(k, y + (" + std::to_string(in.offset[1]) + "))", "ez(k, z + (" + std::to_string(in.offset[2]) + "))")
+        case Op::Shift: {
+            const int a = in.args[0];
+            f << "    return " << v(a, cl(a, 0, "x + (" + std::to_string(in.offset[0]) + ")"), cl(a, 1, "y + (" + std::to_string(in.offset[1]) + ")"),
+                cl(a, 2, "z + (" + std::to_string(in.offset[2]) + ")"))
               << ";\n";
             break;
+        }
         case Op::Mul: {
             // Zero times anything is zero on the CPU, without looking at the anything (src/field/Law.cpp:804-843).
             const int a = in.args[0];
@@ -368,17 +386,19 @@ private:
             f << "    return fe_rf(" << d(a) << " * " << d(b) << ");\n";
             break;
         }
-        case Op::Lap:
+        case Op::Lap: {
             // (xm + xp + ym + yp + zm + zp - 6 c) / h^2 in double, left to right; not finite -> 0.
-            f << "    const double c = " << d(in.args[0]) << ";\n"
-              << "    double s = " << d(in.args[0], "ex(k, x - 1)") << " + " << d(in.args[0], "ex(k, x + 1)") << ";\n"
-              << "    s = s + " << d(in.args[0], "x", "ey(k, y - 1)") << ";\n"
-              << "    s = s + " << d(in.args[0], "x", "ey(k, y + 1)") << ";\n"
-              << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z - 1)") << ";\n"
-              << "    s = s + " << d(in.args[0], "x", "y", "ez(k, z + 1)") << ";\n"
+            const int a = in.args[0];
+            f << "    const double c = " << d(a) << ";\n"
+              << "    double s = " << d(a, cl(a, 0, "x - 1")) << " + " << d(a, cl(a, 0, "x + 1")) << ";\n"
+              << "    s = s + " << d(a, "x", cl(a, 1, "y - 1")) << ";\n"
+              << "    s = s + " << d(a, "x", cl(a, 1, "y + 1")) << ";\n"
+              << "    s = s + " << d(a, "x", "y", cl(a, 2, "z - 1")) << ";\n"
+              << "    s = s + " << d(a, "x", "y", cl(a, 2, "z + 1")) << ";\n"
               << "    const double o = div_hh(k, s - 6.0 * c);\n"
               << "    return fe_rf(finite_d(o) ? o : 0.0);\n";
             break;
+        }
         case Op::Gx: case Op::Gy: case Op::Gz: {
             const int axis = in.op == Op::Gx ? 0 : (in.op == Op::Gy ? 1 : 2);
             const char* c = axis == 0 ? "x" : (axis == 1 ? "y" : "z");
@@ -402,35 +422,44 @@ private:
             // kf = 2 ka kb / (ka + kb) (src/field/Law.cpp:986-1018), a missing face skipped.
             const int kk = in.args[0];
             const int fa = in.args[1];
-            // Every neighbour is read at its clamped coordinate and a missing face adds an exact +0: the sum starts at +0
-            // and can never become -0, so adding +0 changes no bit, and the reads issue together (no branches).
-            f << "    const double c = " << d(fa) << ";\n    double acc = 0.0;\n";
+            // Every neighbour is read at its clamped coordinate, so across a missing face the neighbour is the cell and
+            // the face's term is kf * (c - c): +0 or -0 when c and kf are finite, NaN otherwise. The sum starts at +0
+            // and can never become -0, so adding a zero changes no bit: when the six-term sum is not NaN, every missing
+            // face added a zero and it is the CPU's sum bit for bit. Only a NaN sum is taken again with the missing
+            // faces skipped (then it is whatever the CPU's is). No selects in the common case, and the reads issue
+            // together.
+            f << "    const double c = " << d(fa) << ";\n";
             if (scalar(kk)) {
                 f << "    const double kv = s" << kk << "(k, m);\n"
                   << "    const double kf = (kv + kv) > 0.0 ? 2.0 * kv * kv / (kv + kv) : 0.0;\n";
             } else {
                 f << "    const double ka = " << d(kk) << ";\n";
             }
-            const char* sides[6][4] = {{"x > 0", "ex(k, x - 1)", "y", "z"}, {"x + 1 < k.nx", "ex(k, x + 1)", "y", "z"},
-                {"y > 0", "x", "ey(k, y - 1)", "z"}, {"y + 1 < k.ny", "x", "ey(k, y + 1)", "z"}, {"z > 0", "x", "y", "ez(k, z - 1)"},
-                {"z + 1 < k.nz", "x", "y", "ez(k, z + 1)"}};
+            const char* cond[6] = {"x > 0", "x + 1 < k.nx", "y > 0", "y + 1 < k.ny", "z > 0", "z + 1 < k.nz"};
+            const int axis[6] = {0, 0, 1, 1, 2, 2};
+            const char* step[6] = {"x - 1", "x + 1", "y - 1", "y + 1", "z - 1", "z + 1"};
+            auto at_face = [&](int arg, int q) {
+                const std::string c = cl(arg, axis[q], step[q]);
+                return axis[q] == 0 ? d(arg, c) : (axis[q] == 1 ? d(arg, "x", c) : d(arg, "x", "y", c));
+            };
             for (int q = 0; q < 6; ++q) {
-                const auto& s = sides[q];
-                f << "    const double n" << q << " = " << d(fa, s[1], s[2], s[3]) << ";\n";
+                f << "    const double n" << q << " = " << at_face(fa, q) << ";\n";
                 if (!scalar(kk)) {
-                    f << "    const double kb" << q << " = " << d(kk, s[1], s[2], s[3]) << ";\n";
+                    f << "    const double kb" << q << " = " << at_face(kk, q) << ";\n"
+                      << "    const double kf" << q << " = (ka + kb" << q << ") > 0.0 ? 2.0 * ka * kb" << q << " / (ka + kb" << q
+                      << ") : 0.0;\n";
                 }
             }
+            auto kfq = [&](int q) { return scalar(kk) ? std::string("kf") : "kf" + std::to_string(q); };
+            f << "    double acc = 0.0;\n";
             for (int q = 0; q < 6; ++q) {
-                const auto& s = sides[q];
-                if (scalar(kk)) {
-                    f << "    acc += (" << s[0] << ") ? kf * (n" << q << " - c) : 0.0;\n";
-                } else {
-                    f << "    { const double kf = (ka + kb" << q << ") > 0.0 ? 2.0 * ka * kb" << q << " / (ka + kb" << q
-                      << ") : 0.0; acc += (" << s[0] << ") ? kf * (n" << q << " - c) : 0.0; }\n";
-                }
+                f << "    acc += " << kfq(q) << " * (n" << q << " - c);\n";
             }
-            f << "    return fe_rf(div_hh(k, acc));\n";
+            f << "    if (acc != acc) {\n        acc = 0.0;\n";
+            for (int q = 0; q < 6; ++q) {
+                f << "        if (" << cond[q] << ") acc += " << kfq(q) << " * (n" << q << " - c);\n";
+            }
+            f << "    }\n    return fe_rf(div_hh(k, acc));\n";
             break;
         }
         case Op::Adv: {
@@ -439,10 +468,11 @@ private:
             f << "    const double c = " << d(fa) << ";\n"
               << "    const double u = " << d(in.args[1]) << ";\n    const double w = " << d(in.args[2]) << ";\n"
               << "    const double s = " << d(in.args[3]) << ";\n"
-              << "    const double xl = " << d(fa, "ex(k, x - 1)") << "; const double xh = " << d(fa, "ex(k, x + 1)") << ";\n"
-              << "    const double yl = " << d(fa, "x", "ey(k, y - 1)") << "; const double yh = " << d(fa, "x", "ey(k, y + 1)") << ";\n"
-              << "    const double zl = " << d(fa, "x", "y", "ez(k, z - 1)") << "; const double zh = " << d(fa, "x", "y", "ez(k, z + 1)")
-              << ";\n";
+              << "    const double xl = " << d(fa, cl(fa, 0, "x - 1")) << "; const double xh = " << d(fa, cl(fa, 0, "x + 1")) << ";\n"
+              << "    const double yl = " << d(fa, "x", cl(fa, 1, "y - 1")) << "; const double yh = " << d(fa, "x", cl(fa, 1, "y + 1"))
+              << ";\n"
+              << "    const double zl = " << d(fa, "x", "y", cl(fa, 2, "z - 1")) << "; const double zh = "
+              << d(fa, "x", "y", cl(fa, 2, "z + 1")) << ";\n";
             auto up = [&](const char* speed, const char* c, const char* cnt, const char* lo, const char* hi) {
                 return std::string("(") + speed + " > 0.0 ? (" + c + " > 0 ? div_h(k, " + speed + " * (c - " + lo + ")) : 0.0) : (" + c +
                     " + 1 < " + cnt + " ? div_h(k, " + speed + " * (" + hi + " - c)) : 0.0))";
@@ -625,43 +655,61 @@ private:
             return code;
         }
         code.in_place = p.in_place;
-        // The body: every rate first, from the pre-state; then each target's deltas in equation order.
-        std::ostringstream body;
-        body << "    const i64 i = IDX(x, y, z);\n";
-        for (std::size_t e = 0; e < p.equations.size(); ++e) {
-            body << "    const double r" << e << " = " << d(p.equations[e].value) << ";\n";
-        }
+        // The cell function: every rate first, from the pre-state; then each target's deltas in equation order. o[q]
+        // and n[q] are target q's old and new value.
         std::vector<int> targets;
         for (const ir::Equation& eq : p.equations) {
             if (std::find(targets.begin(), targets.end(), eq.target) == targets.end()) {
                 targets.push_back(eq.target);
             }
         }
-        for (const int target : targets) {
-            const std::string t = std::to_string(target);
-            body << "    { const float o = m.template fld<" << t << ">(k, x, y, z); float n = o;\n";
+        const std::string cellfn = "c_" + code.gather;
+        std::ostringstream c;
+        c << "template <class M> __device__ __forceinline__ void " << cellfn
+          << "(const K& k, const M& m, int x, int y, int z, float* o, float* n)\n{\n";
+        for (std::size_t e = 0; e < p.equations.size(); ++e) {
+            c << "    const double r" << e << " = " << d(p.equations[e].value) << ";\n";
+        }
+        for (std::size_t q = 0; q < targets.size(); ++q) {
+            const int target = targets[q];
+            c << "    o[" << q << "] = m.template fld<" << target << ">(k, x, y, z); n[" << q << "] = o[" << q << "];\n";
             for (std::size_t e = 0; e < p.equations.size(); ++e) {
                 if (p.equations[e].target != target) {
                     continue;
                 }
+                const std::string upd = "n[" + std::to_string(q) + "] = fe_ftz(n[" + std::to_string(q) + "] + fe_rf(r" + std::to_string(e) +
+                    " * k.dt));\n";
                 if (scalar(p.equations[e].value)) {
                     // A rate that is one number and 0 is idle on the CPU: nothing is added.
-                    body << "      if (r" << e << " != 0.0) n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                    c << "    if (r" << e << " != 0.0) " << upd;
                 } else {
-                    body << "      n = fe_ftz(n + fe_rf(r" << e << " * k.dt));\n";
+                    c << "    " << upd;
                 }
             }
-            body << "      const bool ch = __float_as_uint(o) != __float_as_uint(n);\n";
-            if (code.in_place) {
-                body << "      if (ch) k.now[" << t << "][i] = n;\n";
-            } else {
-                body << "      k.nxt[" << t << "][i] = n;\n";
-            }
-            body << "      mark(k, " << t << ", ch); }\n";
         }
+        c << "}\n";
+        out_ << c.str();
         if (!code.in_place) {
             code.swaps = targets;
         }
+        const int nt = static_cast<int>(targets.size());
+        // The direct kernel: a thread a cell.
+        g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m(k);\n"
+          << "    float o[" << nt << "], n[" << nt << "];\n    " << cellfn << "(k, m, x, y, z, o, n);\n"
+          << "    const i64 i = IDX(x, y, z);\n";
+        for (int q = 0; q < nt; ++q) {
+            const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
+            const std::string qs = std::to_string(q);
+            g << "    { const bool ch = __float_as_uint(o[" << qs << "]) != __float_as_uint(n[" << qs << "]);\n";
+            if (code.in_place) {
+                g << "      if (ch) k.now[" << t << "][i] = n[" << qs << "];\n";
+            } else {
+                g << "      k.nxt[" << t << "][i] = n[" << qs << "];\n";
+            }
+            g << "      mark(k, " << t << ", ch); }\n";
+        }
+        g << "}\n";
+        out_ << g.str();
         std::map<int, int> reach;
         int cost = 0;
         const bool bounded = field_reach(p, reach, cost);
@@ -673,11 +721,8 @@ private:
                 radius = std::max(radius, r);
             }
         }
-        if (bounded && radius >= 1 && radius <= 2 && windowed.size() <= (radius == 1 ? 4u : 3u) && cost <= 4000) {
-            window_kernel(code, body.str(), windowed, radius);
-        } else {
-            g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m(k);\n" << body.str() << "}\n";
-            out_ << g.str();
+        if (bounded && radius >= 1 && radius <= 2 && cost <= 4000) {
+            window_kernels(code, cellfn, targets, windowed, radius);
         }
         return code;
     }
@@ -719,83 +764,188 @@ private:
         return bounded;
     }
 
-    // A gather over tiles of 64 x TY cells marching in z: the fields it reads off-cell come from an LDS ring of
-    // planes (radius R: the ring holds planes z-R..z+R and the one being filled), each plane of each field loaded
-    // once by its tile (rows and halo columns at clamped coordinates), two planes ahead of use. The arithmetic is the
-    // same as the direct kernel's, read through the window: the same bits.
-    void window_kernel(ProgramCode& code, const std::string& body, const std::vector<int>& wf, int radius)
+    // Window gathers: tiles of cells marching in z, the fields read off-cell (`wf`, reach <= R) from an LDS ring of
+    // planes. The ring holds planes z-R..z+R and the one being filled (RS >= 2R + 2 slots, a slot a plane, slot of
+    // plane p = (p + RS) % RS); each plane of each field is loaded once by its tile, rows and halo columns at clamped
+    // coordinates, so every coordinate within R of a cell holds the clamped cell's value (what a clamped read gets),
+    // three planes ahead of use through a four-slot register ring (unrolled: register copies of loads in flight would
+    // make the compiler wait for them). One barrier a plane: the slot being filled is one no lane reads that plane.
+    //   x1: 64 x TY cells, a lane a cell; rows of 64 + 2R.
+    //   x4: 256 x TY cells, a lane four consecutive cells (float4 loads and stores; nx a multiple of 4); rows of
+    //       256 + 8 with the cells at [4, 260), so each lane's four are 16-byte aligned, the left halo before them
+    //       and the right after. Lanes past the end of a row write the row's last value into their slots, which is
+    //       the clamped value of every coordinate past the end.
+    // The arithmetic is the cell function's, read through the window: the direct kernel's bits.
+    void window_kernels(ProgramCode& code, const std::string& cellfn, const std::vector<int>& targets, const std::vector<int>& wf,
+        int radius)
     {
         const int R = radius;
-        const int TY = R == 1 ? 6 : 4;
-        const int RS = R == 1 ? 4 : 8; // ring slots: a power of two >= 2R + 2
-        const int TW = 64 + 2 * R;
-        const int TP = (TY + 2 * R) * TW;
-        const int RF = RS * TP;
         const int NW = static_cast<int>(wf.size());
+        const int RS = 2 * R + 2;
+        constexpr int kLdsBudget = 64 * 1024;
+        struct Shape {
+            int vx;
+            int ty;
+        };
+        std::vector<Shape> shapes;
+        auto lds_bytes = [&](int vx, int ty) {
+            const int tw = vx == 4 ? 264 : 64 + 2 * R;
+            return NW * RS * (ty + 2 * R) * tw * 4;
+        };
+        for (const Shape sh : {Shape{1, R == 1 ? 6 : 4}, Shape{4, 2}, Shape{4, 6}}) {
+            if (NW > 0 && lds_bytes(sh.vx, sh.ty) <= kLdsBudget) {
+                shapes.push_back(sh);
+            }
+        }
+        if (shapes.empty()) {
+            return;
+        }
         code.window = true;
         code.radius = R;
-        code.ty = TY;
         code.zchunk = 32;
-        code.block_y = TY + 2 * R;
-        const std::string W = "W_" + code.gather;
+        for (const Shape sh : shapes) {
+            const std::string name = code.gather + "_w" + std::to_string(sh.vx) + "_" + std::to_string(sh.ty);
+            window_kernel(cellfn, name, targets, wf, R, sh.vx, sh.ty, code.in_place);
+            code.windows.push_back({name, sh.vx, sh.ty, sh.ty + 2 * R});
+        }
+    }
+
+    void window_kernel(const std::string& cellfn, const std::string& name, const std::vector<int>& targets, const std::vector<int>& wf,
+        int R, int VX, int TY, bool in_place)
+    {
+        const int NW = static_cast<int>(wf.size());
+        const int RS = 2 * R + 2;
+        const int TW = VX == 4 ? 264 : 64 + 2 * R; // row stride
+        const int X0 = VX == 4 ? 4 : R;            // where the tile's first cell sits in its row
+        const int TP = (TY + 2 * R) * TW;
+        const int RF = RS * TP;
+        const int NT = static_cast<int>(targets.size());
+        const int SPAN = 64 * VX;
+        const std::string W = "W_" + name;
         std::ostringstream s;
-        s << "struct " << W << " {\n    float* lds; int x0; int y0; double u[" << std::max(1, lay_.uniforms) << "];\n"
+        s << "struct " << W << " {\n    static constexpr bool kClamps = false;\n    const float* lds; int x0; int y0; double u["
+          << std::max(1, lay_.uniforms) << "];\n"
           << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const {\n";
         for (int w = 0; w < NW; ++w) {
             s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return lds["
-              << w * RF << " + (z & " << RS - 1 << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + " << R
-              << ")];\n";
+              << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + "
+              << X0 << ")];\n";
         }
-        s << "        " << (NW ? "else " : "") << "return k.now[F][IDX(x, y, z)];\n    }\n};\n";
-        s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << code.gather
+        s << "        else return k.now[F][IDX(x, y, z)];\n    }\n};\n";
+        s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
           << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
-          << "    __shared__ float lds[" << NW * RF << "];\n"
+          << "    __shared__ __attribute__((aligned(16))) float lds[" << NW * RF << "];\n"
           << "    const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z);\n"
-          << "    const int x0 = (t % tiles_x) * 64; const int y0 = ((t / tiles_x) % tiles_y) * " << TY << ";\n"
+          << "    const int x0 = (t % tiles_x) * " << SPAN << "; const int y0 = ((t / tiles_x) % tiles_y) * " << TY << ";\n"
           << "    const int z0 = (t / (tiles_x * tiles_y)) * zchunk; const int z1 = z0 + zchunk < k.nz ? z0 + zchunk : k.nz;\n"
           << "    const int lane = (int)threadIdx.x; const int r = (int)threadIdx.y;\n"
-          << "    const int x = x0 + lane; const int y = y0 + r - " << R << ";\n"
+          << "    const int x = x0 + " << VX << " * lane; const int y = y0 + r - " << R << ";\n"
           << "    const bool live = r >= " << R << " && r < " << R + TY << " && x < k.nx && y < k.ny;\n"
-          << "    const int xs = clampi(x, 0, k.nx - 1); const int ys = clampi(y, 0, k.ny - 1);\n"
-          // The halo column each lane also loads (lanes 0..R-1 the left, 64-R..63 the right; the others their own).
-          << "    const int hslot = lane < " << R << " ? lane : (lane >= " << 64 - R << " ? lane + " << 2 * R << " : -1);\n"
-          << "    const int hx = clampi(lane < " << R << " ? x0 - " << R << " + lane : (lane >= " << 64 - R << " ? x0 + lane + " << R
-          << " : x), 0, k.nx - 1);\n"
+          << "    const int ys = clampi(y, 0, k.ny - 1);\n"
           << "    const i64 rowo = (i64)ys * k.nx;\n"
           << "    const int zlim = z1 - 1 + " << R << " < k.nz - 1 ? z1 - 1 + " << R << " : k.nz - 1;\n"
           << "    " << W << " m; m.lds = lds; m.x0 = x0; m.y0 = y0;\n"
           << "    for (int q = 0; q < " << std::max(1, lay_.uniforms) << "; ++q) m.u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
           << ";\n"
-          << "    float* mine = lds + r * " << TW << " + lane + " << R << ";\n"
-          << "    float* halo = lds + r * " << TW << " + (hslot < 0 ? 0 : hslot);\n"
-          << "    struct Pl { float v[" << NW << "]; float h[" << NW << "]; };\n"
-          // Unconditional loads at clamped planes: a load under a branch makes the compiler wait for every load.
-          << "    auto load = [&](Pl& p, int zp) { const i64 o = (i64)clampi(zp, 0, zlim) * k.nxy + rowo;\n";
-        for (int w = 0; w < NW; ++w) {
-            s << "        p.v[" << w << "] = k.now[" << wf[static_cast<std::size_t>(w)] << "][o + xs]; p.h[" << w << "] = k.now["
-              << wf[static_cast<std::size_t>(w)] << "][o + hx];\n";
+          << "    float* row = lds + r * " << TW << ";\n";
+        if (VX == 1) {
+            // Lanes 0..R-1 also load the left halo columns, 64-R..63 the right; the others their own cell again (unused).
+            s << "    const int xs = clampi(x, 0, k.nx - 1);\n"
+              << "    const int hslot = lane < " << R << " ? lane : (lane >= " << 64 - R << " ? lane + " << 2 * R << " : -1);\n"
+              << "    const int hx = clampi(lane < " << R << " ? x0 - " << R << " + lane : (lane >= " << 64 - R << " ? x0 + lane + " << R
+              << " : x), 0, k.nx - 1);\n"
+              << "    struct Pl { float v[" << NW << "]; float h[" << NW << "]; };\n"
+              << "    auto load = [&](Pl& p, int zp) { const i64 o = (i64)clampi(zp, 0, zlim) * k.nxy + rowo;\n";
+            for (int w = 0; w < NW; ++w) {
+                const int f = wf[static_cast<std::size_t>(w)];
+                s << "        p.v[" << w << "] = k.now[" << f << "][o + xs]; p.h[" << w << "] = k.now[" << f << "][o + hx];\n";
+            }
+            s << "    };\n"
+              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((zp + " << RS << ") % " << RS << ") * " << TP << ";\n";
+            for (int w = 0; w < NW; ++w) {
+                s << "        sl[" << w * RF << " + " << R << " + lane] = p.v[" << w << "]; if (hslot >= 0) sl[" << w * RF << " + hslot] = p.h["
+                  << w << "];\n";
+            }
+            s << "    };\n";
+        } else {
+            // Lane 0 also loads the R columns left of the tile, lane 63 the R right of it; the others their own first
+            // cell again (unused): every load is unconditional.
+            s << "    const bool dead = x >= k.nx;\n"
+              << "    const int xs = x < k.nx - 4 ? x : k.nx - 4;\n"
+              << "    const bool hl = lane == 0 || lane == 63;\n"
+              << "    const int hpos = lane == 0 ? " << 4 - R << " : 260;\n"
+              << "    int hx[" << R << "];\n"
+              << "    for (int j = 0; j < " << R << "; ++j) hx[j] = lane == 0 ? clampi(x0 - " << R << " + j, 0, k.nx - 1) : (lane == 63 ? "
+              << "clampi(x0 + 256 + j, 0, k.nx - 1) : xs);\n"
+              << "    struct Pl { float4 v[" << NW << "]; float h[" << NW << "][" << R << "]; };\n"
+              << "    auto load = [&](Pl& p, int zp) { const i64 o = (i64)clampi(zp, 0, zlim) * k.nxy + rowo;\n";
+            for (int w = 0; w < NW; ++w) {
+                const int f = wf[static_cast<std::size_t>(w)];
+                s << "        p.v[" << w << "] = *reinterpret_cast<const float4*>(k.now[" << f << "] + o + xs);\n"
+                  << "        for (int j = 0; j < " << R << "; ++j) p.h[" << w << "][j] = k.now[" << f << "][o + hx[j]];\n";
+            }
+            s << "    };\n"
+              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((zp + " << RS << ") % " << RS << ") * " << TP << ";\n";
+            for (int w = 0; w < NW; ++w) {
+                s << "        { float4 v = p.v[" << w << "]; if (dead) v = make_float4(v.w, v.w, v.w, v.w);\n"
+                  << "          *reinterpret_cast<float4*>(sl + " << w * RF << " + 4 + 4 * lane) = v;\n"
+                  << "          if (hl) for (int j = 0; j < " << R << "; ++j) sl[" << w * RF << " + hpos + j] = p.h[" << w << "][j]; }\n";
+            }
+            s << "    };\n";
         }
-        s << "    };\n"
-          << "    auto put = [&](const Pl& p, int zp) { const int sl = (zp & " << RS - 1 << ") * " << TP << ";\n";
-        for (int w = 0; w < NW; ++w) {
-            s << "        mine[" << w * RF << " + sl] = p.v[" << w << "]; if (hslot >= 0) halo[" << w * RF << " + sl] = p.h[" << w << "];\n";
+        // The cells of a lane: their new values and the stores.
+        std::ostringstream body;
+        body << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
+        for (int c = 0; c < VX; ++c) {
+            body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
-        s << "    };\n"
-          << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); }\n"
+        body << "            const i64 i = IDX(x, y, z);\n";
+        for (int q = 0; q < NT; ++q) {
+            const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
+            body << "            { bool ch = false;\n";
+            for (int c = 0; c < VX; ++c) {
+                body << "              { const bool d = __float_as_uint(o[" << c << "][" << q << "]) != __float_as_uint(n[" << c << "][" << q
+                     << "]); ch = ch || d;";
+                if (in_place) {
+                    body << " if (d) k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "];";
+                }
+                body << " }\n";
+            }
+            if (!in_place) {
+                if (VX == 4) {
+                    body << "              *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
+                         << "], n[2][" << q << "], n[3][" << q << "]);\n";
+                } else {
+                    body << "              k.nxt[" << t << "][i] = n[0][" << q << "];\n";
+                }
+            }
+            body << "              seen[" << q << "] = seen[" << q << "] || ch; }\n";
+        }
+        // A tile's change flags are marked once, after its last plane (a store a plane would be waited on with the loads).
+        s << "    bool seen[" << std::max(1, NT) << "] = {};\n";
+        s << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); }\n"
           << "    Pl A, B, C, D;\n"
-          << "    load(A, z0 + " << R << "); load(B, z0 + " << R + 1 << "); load(C, z0 + " << R + 2 << ");\n"
+          // In this order: the loop's first wait counts A as the oldest load in flight (were A issued last, the
+          // compiler would wait for every load there).
+          << "    load(A, z0 + " << R << "); __builtin_amdgcn_sched_barrier(0); load(B, z0 + " << R + 1
+          << "); __builtin_amdgcn_sched_barrier(0); load(C, z0 + " << R + 2 << "); __builtin_amdgcn_sched_barrier(0);\n"
           << "    auto step = [&](const int z, const Pl& cur, Pl& far) {\n"
           << "        load(far, z + " << R + 3 << ");\n"
           << "        put(cur, z + " << R << ");\n"
           << "        __syncthreads();\n"
-          << "        if (live) {\n" << body << "        }\n"
+          << "        if (live) {\n" << body.str() << "        }\n"
           << "    };\n"
-          << "    for (int z = z0; z < z1; z += 4) {\n"
-          << "        step(z, A, D); if (z + 1 >= z1) break;\n"
-          << "        step(z + 1, B, A); if (z + 2 >= z1) break;\n"
-          << "        step(z + 2, C, B); if (z + 3 >= z1) break;\n"
-          << "        step(z + 3, D, C);\n"
-          << "    }\n}\n";
+          // Whole groups of four planes, then the rest: a break inside the group would join the loop's latch, and
+          // the compiler, counting the loads in flight over every path into the loop's head, would wait for nearly all.
+          << "    int z = z0;\n"
+          << "    for (; z + 4 <= z1; z += 4) {\n"
+          << "        step(z, A, D); step(z + 1, B, A); step(z + 2, C, B); step(z + 3, D, C);\n"
+          << "    }\n"
+          << "    if (z < z1) { step(z, A, D); if (z + 1 < z1) { step(z + 1, B, A); if (z + 2 < z1) step(z + 2, C, B); } }\n";
+        for (int q = 0; q < NT; ++q) {
+            s << "    mark(k, " << targets[static_cast<std::size_t>(q)] << ", seen[" << q << "]);\n";
+        }
+        s << "}\n";
         out_ << s.str();
     }
 

@@ -32,16 +32,23 @@ struct StageCode {
     Op reduce = Op::SumOf;
 };
 
-struct ProgramCode {
-    std::string gather;     // empty: no equations
-    // A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of
-    // tiles_x * tiles_y * tiles_z blocks of 64 x block_y threads with (K, zchunk, tiles_x, tiles_y, tiles_z).
-    // Otherwise a direct gather: a thread a cell, blocks of 64 x 4 over (x, y), one z-plane a block row.
-    bool window = false;
-    int radius = 0;
+// A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of
+// tiles_x * tiles_y * tiles_z blocks of 64 x block_y threads with (K, zchunk, tiles_x, tiles_y, tiles_z); a tile is
+// 64 * vx cells by ty rows by zchunk planes. vx = 4 needs nx a multiple of 4.
+struct WindowKernel {
+    std::string name;
+    int vx = 1;
     int ty = 0;
-    int zchunk = 0;
     int block_y = 0;
+};
+
+struct ProgramCode {
+    std::string gather;     // empty: no equations. The direct gather: a thread a cell, blocks of 64 x 4 over (x, y),
+                            // one z-plane a block row; always there.
+    bool window = false;    // window gathers exist (then one of them runs)
+    int radius = 0;
+    int zchunk = 0;
+    std::vector<WindowKernel> windows;
     bool in_place = false;
     std::vector<int> swaps; // ping-pong: the fields whose two buffers swap after the kernel
     std::vector<int> stages;

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -78,10 +80,15 @@ struct Impl final : LawBackend {
     mutable StepProfile prof;
     mutable bool timings_pending = false;
     std::size_t derive_event = 0;
+    bool derive_timed = false;
     int check_every = 1;
     long long step_count = 0;
     bool lazy = false;
     std::vector<char> stale; // per field: the card holds newer cells than the host
+    // Launch-shape overrides, read once from the environment at construction (for measuring shapes against each
+    // other): FIELD_HIP_TILE="<cells a lane>x<rows>", FIELD_HIP_ZCHUNK=<planes a tile>, FIELD_HIP_DIRECT=1.
+    int tile_vx = 0, tile_ty = 0, zchunk_override = 0;
+    bool force_direct = false;
 
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
@@ -181,13 +188,14 @@ struct Impl final : LawBackend {
     // A term's gather: a thread a cell, or (window kernels) z-marching tiles of 64 x ty cells.
     void launch_gather(const hip::ProgramCode& code, double dt)
     {
-        if (!code.window) {
+        const hip::WindowKernel* w = window_for(code);
+        if (w == nullptr) {
             launch_cells(code.gather, dt);
             return;
         }
-        int zchunk = code.zchunk;
-        int tx = (shape.nx + 63) / 64;
-        int ty = (shape.ny + code.ty - 1) / code.ty;
+        int zchunk = zchunk_override > 0 ? zchunk_override : code.zchunk;
+        int tx = (shape.nx + 64 * w->vx - 1) / (64 * w->vx);
+        int ty = (shape.ny + w->ty - 1) / w->ty;
         int tz = (shape.nz + zchunk - 1) / zchunk;
         const long long total = 1LL * tx * ty * tz;
         if (total > 0x7fffffffLL) {
@@ -195,13 +203,58 @@ struct Impl final : LawBackend {
         }
         std::vector<std::uint64_t> k = args(dt);
         void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz};
-        check(fe_hip_launch(fn(code.gather), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(code.block_y), 1, 0, argv),
-            ("launch " + code.gather).c_str());
+        check(fe_hip_launch(fn(w->name), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(w->block_y), 1, 0, argv),
+            ("launch " + w->name).c_str());
         ++prof.launches;
+    }
+
+    // The window gather for this lattice: four cells a lane when rows are whole float4s, with the taller tile when
+    // that still leaves at least 2048 tiles (eight per CU) to spread; FIELD_HIP_TILE (e.g. "4x2", "1x6") picks one
+    // by its cells a lane and rows. None: the direct gather.
+    const hip::WindowKernel* window_for(const hip::ProgramCode& code) const
+    {
+        if (!code.window || code.windows.empty() || force_direct) {
+            return nullptr;
+        }
+        const hip::WindowKernel* best = nullptr;
+        const int zc = zchunk_override > 0 ? zchunk_override : code.zchunk;
+        const long long tz = (shape.nz + zc - 1) / zc;
+        for (const hip::WindowKernel& w : code.windows) {
+            if (w.vx == 4 && shape.nx % 4 != 0) {
+                continue;
+            }
+            if (tile_vx > 0) {
+                if (w.vx == tile_vx && w.ty == tile_ty) {
+                    return &w;
+                }
+                continue;
+            }
+            const long long tiles = 1LL * ((shape.nx + 64 * w.vx - 1) / (64 * w.vx)) * ((shape.ny + w.ty - 1) / w.ty) * tz;
+            if (best == nullptr || w.vx > best->vx || (w.vx == best->vx && w.ty > best->ty && tiles >= 2048)) {
+                best = &w;
+            }
+        }
+        return best;
+    }
+
+    void read_overrides()
+    {
+        if (const char* t = std::getenv("FIELD_HIP_TILE")) {
+            if (std::sscanf(t, "%dx%d", &tile_vx, &tile_ty) != 2) {
+                tile_vx = tile_ty = 0;
+            }
+        }
+        if (const char* z = std::getenv("FIELD_HIP_ZCHUNK")) {
+            zchunk_override = std::max(0, std::atoi(z));
+        }
+        if (const char* d = std::getenv("FIELD_HIP_DIRECT")) {
+            force_direct = std::atoi(d) != 0;
+        }
     }
 
     void build(std::string& why)
     {
+        read_overrides();
         module = ir::lower(*law);
         const std::string src = hip::generate(module, *law, layout, why);
         if (src.empty()) {
@@ -676,9 +729,14 @@ struct Impl final : LawBackend {
         upload_params();
         upload(fields, edits);
         const std::size_t nf = law->fields.size();
-        check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, nf)), "change flags reset");
         const bool contracts = (step_count % check_every) == 0;
         ++step_count;
+        // The change flags are read back unless the step is lazy and unjudged (then every field a program writes counts
+        // as changed, and the flags are not cleared either).
+        const bool read_flags = !lazy || contracts;
+        if (read_flags) {
+            check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, nf)), "change flags reset");
+        }
         std::vector<Judge> judges;
         int total_slot = 0;
         if (contracts) {
@@ -736,13 +794,18 @@ struct Impl final : LawBackend {
             ev += 2;
         }
         derive_event = ev;
-        check(fe_hip_event_record(event(ev++)), "event");
+        derive_timed = !module.derives.empty();
+        if (derive_timed) {
+            check(fe_hip_event_record(event(ev++)), "event");
+        }
         run_derives(contracts, judges);
-        check(fe_hip_event_record(event(ev++)), "event");
+        if (derive_timed) {
+            check(fe_hip_event_record(event(ev++)), "event");
+        }
         if (contracts) {
             judge(judges, violations);
         }
-        if (lazy && !contracts) {
+        if (!read_flags) {
             // Nothing comes back and nothing is waited for: every field a program writes is taken as changed, and the
             // card's time is read from the events when the profile is asked for.
             changed.assign(nf, 0);
@@ -776,7 +839,10 @@ struct Impl final : LawBackend {
             return;
         }
         timings_pending = false;
-        fe_hip_event_sync(events[derive_event + 1]);
+        if (derive_event == 0 && !derive_timed) {
+            return; // no term and no derive: nothing was timed
+        }
+        fe_hip_event_sync(events[derive_timed ? derive_event + 1 : derive_event - 1]);
         double gpu = 0.0;
         for (std::size_t t = 0; t < prof.terms.size(); ++t) {
             float ms = 0.0f;
@@ -785,7 +851,9 @@ struct Impl final : LawBackend {
             gpu += ms;
         }
         float dms = 0.0f;
-        fe_hip_event_elapsed(events[derive_event], events[derive_event + 1], &dms);
+        if (derive_timed) {
+            fe_hip_event_elapsed(events[derive_event], events[derive_event + 1], &dms);
+        }
         prof.derive_ms = dms;
         prof.gpu_ms = gpu + dms;
     }

@@ -92,3 +92,80 @@ def test_profile_reports_card_time(device):
     p = solver.profile()
     assert p["backend"] == "HIP" and p["launches"] > 0 and p["gpu_ms"] > 0.0
     assert [t["name"] for t in p["terms"]] == ["exchange", "decay"]
+
+
+# Every gather the generator emits for a law, picked through the launch-shape overrides: the direct kernel (a thread a
+# cell), the window kernel with a cell a lane (64 x 6 tiles) and with four cells a lane (256 x 2 and 256 x 6 tiles).
+GATHERS = {"direct": {"FIELD_HIP_DIRECT": "1"}, "w1x6": {"FIELD_HIP_TILE": "1x6"}, "w4x2": {"FIELD_HIP_TILE": "4x2"},
+           "w4x6": {"FIELD_HIP_TILE": "4x6"}, "w4x2z5": {"FIELD_HIP_TILE": "4x2", "FIELD_HIP_ZCHUNK": "5"}}
+
+
+def use_gather(monkeypatch, gather):
+    for key in ("FIELD_HIP_DIRECT", "FIELD_HIP_TILE", "FIELD_HIP_ZCHUNK"):
+        monkeypatch.delenv(key, raising=False)
+    for key, value in GATHERS[gather].items():
+        monkeypatch.setenv(key, value)
+
+
+@pytest.mark.parametrize("gather", sorted(GATHERS))
+@pytest.mark.parametrize("name", golden_io.names())
+def test_every_gather_matches_reference_bits(device, monkeypatch, gather, name):
+    import field_hip
+
+    use_gather(monkeypatch, gather)
+    meta, init, final = golden_io.load(name)
+    print(f"case {name} gather {gather} grid {meta['grid']}", flush=True)
+    _, solver = run_engine(field_hip, meta, init)
+    for field, expected in final.items():
+        assert exact(solver.get_field(field), expected), f"{name}/{field} through {gather}"
+
+
+CONDUCTION = "field T init 0\nparam k = 0.1\nterm conduction\n  d T = diffuse(k, T)\n"
+
+# Rows of several tiles, partial tiles, rows that are not whole float4s, flat and thin lattices.
+LATTICES = [(260, 9, 7), (516, 5, 6), (200, 7, 9), (13, 11, 6), (4, 3, 2), (2, 1, 1), (64, 1, 1), (8, 1, 70), (1, 5, 9)]
+
+
+@pytest.mark.parametrize("gather", sorted(GATHERS))
+@pytest.mark.parametrize("shape", LATTICES, ids=lambda s: "x".join(map(str, s)))
+def test_conduction_matches_the_oracle_on_uneven_lattices(device, monkeypatch, gather, shape):
+    import field_hip
+    from oracle import heat
+
+    use_gather(monkeypatch, gather)
+    nx, ny, nz = shape
+    h, k = 1.0 / 16, 0.1
+    dt = 0.4 * h * h / k / 3
+    t0 = np.random.default_rng(nx * 1000 + ny * 10 + nz).random(nx * ny * nz, dtype=np.float32)
+    print(f"lattice {shape} gather {gather}", flush=True)
+    solver = field_hip.Solver(field_hip.Law(CONDUCTION), shape, h)
+    solver.set_field("T", t0)
+    expect = t0
+    for _ in range(3):
+        solver.step(dt)
+        expect = heat.substep(expect, shape, k, h, dt)
+    assert exact(solver.get_field("T"), expect)
+
+
+@pytest.mark.parametrize("gather", sorted(GATHERS))
+def test_conduction_with_infinities_and_nan_follows_the_oracle(device, monkeypatch, gather):
+    import field_hip
+    from oracle import heat
+
+    use_gather(monkeypatch, gather)
+    shape = (24, 7, 6)
+    h, k = 1.0 / 16, 0.1
+    dt = 0.4 * h * h / k / 3
+    t0 = np.random.default_rng(5).random(24 * 7 * 6, dtype=np.float32)
+    t0[3 * 24 * 7 + 3 * 24 + 5] = np.inf    # interior
+    t0[0] = -np.inf                         # a corner: three faces missing
+    t0[2 * 24 * 7 + 6 * 24 + 23] = np.nan   # an edge cell
+    t0[5 * 24 * 7 + 24 + 0] = np.inf        # a face cell
+    solver = field_hip.Solver(field_hip.Law(CONDUCTION), shape, h)
+    solver.set_field("T", t0)
+    solver.step(dt)
+    got = solver.get_field("T")
+    expect = heat.substep(t0, shape, k, h, dt)
+    nan = np.isnan(expect)
+    assert np.array_equal(np.isnan(got), nan)  # NaN payloads differ between the CPU and the card; which cells do not
+    assert exact(got[~nan], expect[~nan])
