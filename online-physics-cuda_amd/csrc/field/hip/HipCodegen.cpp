@@ -37,11 +37,18 @@ __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(lo, min(v, hi)); }
 // a / h, a / h^2, a / (h * span): when the divisor is a power of two (flags, set by the host) the quotient is the exact
 // product with its reciprocal, so the multiply gives the division's bits without the division's cost.
-__device__ __forceinline__ double div_h(const K& k, double a) { return (k.flags & 1) ? a * k.inv_h : a / k.h; }
-__device__ __forceinline__ double div_hh(const K& k, double a) { return (k.flags & 2) ? a * k.inv_hh : a / (k.h * k.h); }
-__device__ __forceinline__ double div_hs(const K& k, double a, int span)
+// M::kPow2: the caller has checked both flags (a kernel instantiated for that case), so no branch is left.
+template <class M> __device__ __forceinline__ double div_h(const K& k, double a)
 {
-    return (k.flags & 1) && span > 0 ? a * (span == 1 ? k.inv_h : 0.5 * k.inv_h) : a / (k.h * (double)span);
+    return (M::kPow2 || (k.flags & 1)) ? a * k.inv_h : a / k.h;
+}
+template <class M> __device__ __forceinline__ double div_hh(const K& k, double a)
+{
+    return (M::kPow2 || (k.flags & 2)) ? a * k.inv_hh : a / (k.h * k.h);
+}
+template <class M> __device__ __forceinline__ double div_hs(const K& k, double a, int span)
+{
+    return (M::kPow2 || (k.flags & 1)) && span > 0 ? a * (span == 1 ? k.inv_h : 0.5 * k.inv_h) : a / (k.h * (double)span);
 }
 // Where a value is read from (struct Direct, generated after this prelude: the field's array in HBM; a window kernel
 // has its own, an LDS ring of planes).
@@ -215,7 +222,8 @@ public:
         out_ << kPrelude;
         // Reductions' results are read once a kernel into registers (a load inside the z loop would be repeated and,
         // waited for, would drain the loads in flight).
-        out_ << "struct Direct {\n    static constexpr bool kClamps = true;\n    double u[" << nu << "];\n"
+        out_ << "struct Direct {\n    static constexpr bool kClamps = true;\n    static constexpr bool kExact = true;\n"
+             << "    static constexpr bool kPow2 = false;\n    __device__ void flag(bool) const {}\n    double u[" << nu << "];\n"
              << "    __device__ explicit Direct(const K& k) { for (int q = 0; q < " << nu << "; ++q) u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
              << "; }\n"
              << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
@@ -395,7 +403,7 @@ private:
               << "    s = s + " << d(a, "x", cl(a, 1, "y + 1")) << ";\n"
               << "    s = s + " << d(a, "x", "y", cl(a, 2, "z - 1")) << ";\n"
               << "    s = s + " << d(a, "x", "y", cl(a, 2, "z + 1")) << ";\n"
-              << "    const double o = div_hh(k, s - 6.0 * c);\n"
+              << "    const double o = div_hh<M>(k, s - 6.0 * c);\n"
               << "    return fe_rf(finite_d(o) ? o : 0.0);\n";
             break;
         }
@@ -408,7 +416,7 @@ private:
             f << "    const int hi = " << hi << ";\n    const int lo = " << lo << ";\n";
             const std::string ahi = axis == 0 ? d(in.args[0], "hi") : (axis == 1 ? d(in.args[0], "x", "hi") : d(in.args[0], "x", "y", "hi"));
             const std::string alo = axis == 0 ? d(in.args[0], "lo") : (axis == 1 ? d(in.args[0], "x", "lo") : d(in.args[0], "x", "y", "lo"));
-            const std::string body = "div_hs(k, " + ahi + " - " + alo + ", hi - lo)";
+            const std::string body = "div_hs<M>(k, " + ahi + " - " + alo + ", hi - lo)";
             if (axis == 0) {
                 f << "    const double o = " << body << ";\n";
             } else {
@@ -422,12 +430,13 @@ private:
             // kf = 2 ka kb / (ka + kb) (src/field/Law.cpp:986-1018), a missing face skipped.
             const int kk = in.args[0];
             const int fa = in.args[1];
-            // Every neighbour is read at its clamped coordinate, so across a missing face the neighbour is the cell and
-            // the face's term is kf * (c - c): +0 or -0 when c and kf are finite, NaN otherwise. The sum starts at +0
-            // and can never become -0, so adding a zero changes no bit: when the six-term sum is not NaN, every missing
-            // face added a zero and it is the CPU's sum bit for bit. Only a NaN sum is taken again with the missing
-            // faces skipped (then it is whatever the CPU's is). No selects in the common case, and the reads issue
-            // together.
+            // Exact form (M::kExact): the CPU's, missing faces skipped. Fast form: every neighbour is read at its clamped
+            // coordinate, so across a missing face the neighbour is the cell and the face's term is kf * (c - c), +0 or
+            // -0 when c and kf are finite and NaN otherwise. The sum starts at +0 and can never become -0, so adding a
+            // zero changes no bit: when the six-term sum is not NaN it is the CPU's sum bit for bit. A NaN sum flags
+            // the cell (m.flag) and its caller takes it again in the exact form. No selects and no branches: the reads
+            // of neighbouring cells merge. The -x face is written -(kf * (c - n0)), the same bits as kf * (n0 - c), so
+            // it is literally the +x flux of the cell before it (shared when a lane computes both cells).
             f << "    const double c = " << d(fa) << ";\n";
             if (scalar(kk)) {
                 f << "    const double kv = s" << kk << "(k, m);\n"
@@ -450,16 +459,20 @@ private:
                       << ") : 0.0;\n";
                 }
             }
-            auto kfq = [&](int q) { return scalar(kk) ? std::string("kf") : "kf" + std::to_string(q); };
-            f << "    double acc = 0.0;\n";
+            auto term = [&](int q) {
+                const std::string kf = scalar(kk) ? std::string("kf") : "kf" + std::to_string(q);
+                const std::string n = "n" + std::to_string(q);
+                return q == 0 && scalar(kk) ? "-(" + kf + " * (c - " + n + "))" : kf + " * (" + n + " - c)";
+            };
+            f << "    double acc = 0.0;\n    if constexpr (M::kExact) {\n";
             for (int q = 0; q < 6; ++q) {
-                f << "    acc += " << kfq(q) << " * (n" << q << " - c);\n";
+                f << "        if (" << cond[q] << ") acc += " << term(q) << ";\n";
             }
-            f << "    if (acc != acc) {\n        acc = 0.0;\n";
+            f << "    } else {\n";
             for (int q = 0; q < 6; ++q) {
-                f << "        if (" << cond[q] << ") acc += " << kfq(q) << " * (n" << q << " - c);\n";
+                f << "        acc += " << term(q) << ";\n";
             }
-            f << "    }\n    return fe_rf(div_hh(k, acc));\n";
+            f << "        m.flag(acc != acc);\n    }\n    return fe_rf(div_hh<M>(k, acc));\n";
             break;
         }
         case Op::Adv: {
@@ -474,8 +487,8 @@ private:
               << "    const double zl = " << d(fa, "x", "y", cl(fa, 2, "z - 1")) << "; const double zh = "
               << d(fa, "x", "y", cl(fa, 2, "z + 1")) << ";\n";
             auto up = [&](const char* speed, const char* c, const char* cnt, const char* lo, const char* hi) {
-                return std::string("(") + speed + " > 0.0 ? (" + c + " > 0 ? div_h(k, " + speed + " * (c - " + lo + ")) : 0.0) : (" + c +
-                    " + 1 < " + cnt + " ? div_h(k, " + speed + " * (" + hi + " - c)) : 0.0))";
+                return std::string("(") + speed + " > 0.0 ? (" + c + " > 0 ? div_h<M>(k, " + speed + " * (c - " + lo + ")) : 0.0) : (" + c +
+                    " + 1 < " + cnt + " ? div_h<M>(k, " + speed + " * (" + hi + " - c)) : 0.0))";
             };
             f << "    double o = " << up("u", "x", "k.nx", "xl", "xh") << ";\n"
               << "    o = o + " << up("w", "y", "k.ny", "yl", "yh") << ";\n"
@@ -822,19 +835,33 @@ private:
         const int NT = static_cast<int>(targets.size());
         const int SPAN = 64 * VX;
         const std::string W = "W_" + name;
+        const std::string LDS = "L_" + name;
         std::ostringstream s;
-        s << "struct " << W << " {\n    static constexpr bool kClamps = false;\n    const float* lds; int x0; int y0; double u["
-          << std::max(1, lay_.uniforms) << "];\n"
+        // The ring at file scope: reads through it keep its 16-byte alignment, so a lane's four neighbouring cells merge
+        // into one wide LDS read.
+        s << "__shared__ __attribute__((aligned(16))) float " << LDS << "[" << NW * RF << "];\n";
+        // Read through the window in the fast forms (a flagged cell is taken again through WX, the exact forms).
+        s << "template <bool P2> struct " << W << " {\n    static constexpr bool kClamps = false;\n    static constexpr bool kExact = false;\n"
+          << "    static constexpr bool kPow2 = P2;\n"
+          << "    int x0; int y0; mutable bool bad; double u[" << std::max(1, lay_.uniforms) << "];\n"
+          << "    __device__ __forceinline__ void flag(bool b) const { bad = bad || b; }\n"
           << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const {\n";
         for (int w = 0; w < NW; ++w) {
-            s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return lds["
+            s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return " << LDS << "["
               << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + "
               << X0 << ")];\n";
         }
         s << "        else return k.now[F][IDX(x, y, z)];\n    }\n};\n";
-        s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
-          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
-          << "    __shared__ __attribute__((aligned(16))) float lds[" << NW * RF << "];\n"
+        s << "template <bool P2> struct WX_" << name << " {\n    static constexpr bool kClamps = false;\n    static constexpr bool kExact = true;\n"
+          << "    static constexpr bool kPow2 = P2;\n"
+          << "    const " << W << "<P2>& w; const double* u;\n    __device__ void flag(bool) const {}\n"
+          << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return w.template fld<F>(k, x, y, z); }\n"
+          << "};\n";
+        // The kernel's body, instantiated for "h and h^2 are powers of two" (their divisions are multiplications,
+        // with no branch between cells) and for the general case.
+        s << "template <bool P2> __device__ __forceinline__ void run_" << name
+          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
+          << "    float* const lds = " << LDS << ";\n"
           << "    const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z);\n"
           << "    const int x0 = (t % tiles_x) * " << SPAN << "; const int y0 = ((t / tiles_x) % tiles_y) * " << TY << ";\n"
           << "    const int z0 = (t / (tiles_x * tiles_y)) * zchunk; const int z1 = z0 + zchunk < k.nz ? z0 + zchunk : k.nz;\n"
@@ -844,7 +871,7 @@ private:
           << "    const int ys = clampi(y, 0, k.ny - 1);\n"
           << "    const i64 rowo = (i64)ys * k.nx;\n"
           << "    const int zlim = z1 - 1 + " << R << " < k.nz - 1 ? z1 - 1 + " << R << " : k.nz - 1;\n"
-          << "    " << W << " m; m.lds = lds; m.x0 = x0; m.y0 = y0;\n"
+          << "    " << W << "<P2> m; m.x0 = x0; m.y0 = y0;\n"
           << "    for (int q = 0; q < " << std::max(1, lay_.uniforms) << "; ++q) m.u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
           << ";\n"
           << "    float* row = lds + r * " << TW << ";\n";
@@ -896,9 +923,15 @@ private:
         // The cells of a lane: their new values and the stores.
         std::ostringstream body;
         body << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
+        body << "            m.bad = false;\n";
         for (int c = 0; c < VX; ++c) {
             body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
+        body << "            if (m.bad) {\n                const WX_" << name << "<P2> mx{m, m.u};\n";
+        for (int c = 0; c < VX; ++c) {
+            body << "                " << cellfn << "(k, mx, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
+        }
+        body << "            }\n";
         body << "            const i64 i = IDX(x, y, z);\n";
         for (int q = 0; q < NT; ++q) {
             const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
@@ -946,6 +979,10 @@ private:
             s << "    mark(k, " << targets[static_cast<std::size_t>(q)] << ", seen[" << q << "]);\n";
         }
         s << "}\n";
+        s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
+          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
+          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z);\n"
+          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z);\n}\n";
         out_ << s.str();
     }
 
