@@ -101,6 +101,7 @@ def conduction(n: int, steps: int, warmup: int, seed: int = 2026, verify_steps: 
         solver.step(sub_dt)
         per_kernel.append(solver.profile()["terms"][0]["ms"])
     eng_kernel = float(np.median(per_kernel))
+    out["engine_kernel"] = solver.profile()["terms"][0]["kernel"]
     out.update(engine_ms_per_step=eng_whole, engine_kernel_ms=eng_kernel,
                engine_gbs=BYTES_PER_CELL * cells / (eng_kernel * 1e6),
                engine_cell_updates_per_s=cells / (eng_whole * 1e-3))
@@ -129,6 +130,16 @@ def conduction(n: int, steps: int, warmup: int, seed: int = 2026, verify_steps: 
     out["engine_equals_handwritten_bits"] = bool(np.array_equal(engine_t.view(np.uint32), hand_t.view(np.uint32)))
     out["verify_steps"] = verify_steps
     return out
+
+
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes a launch of the engine's gather moves, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (scaled by a copy kernel's; profiles/conduction_256_512_rocprof.md), when they were taken on this kernel and size."""
+    path = os.path.join(ROOT, "profiles", f"conduction_{n}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    return rec["bytes_per_launch"] if rec.get("kernel") == kernel else None
 
 
 def cpu_baseline(n: int, budget_s: float):
@@ -215,11 +226,12 @@ def main() -> int:
                                f"dt=0.4h^2/k/3 a step, ping-pong, lazy readback, contracts off",
                    "engine_equals_handwritten_bits": main_case["engine_equals_handwritten_bits"],
                    "handwritten": {k: main_case[k] for k in ("hand_kernel_ms", "hand_gbs", "hand_ms_per_step")},
-                   "engine_kernel_ms": main_case["engine_kernel_ms"],
+                   "engine_kernel": main_case["engine_kernel"], "engine_kernel_ms": main_case["engine_kernel_ms"],
                    "engine_ms_per_step_contracts_on": main_case["engine_ms_per_step_contracts_on"],
                    "copy_roof_gbs": main_case["copy_gbs"], "large_512": large},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(main_case["engine_kernel"], args.size)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

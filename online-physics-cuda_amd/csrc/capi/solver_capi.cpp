@@ -236,6 +236,8 @@ char* fe_solver_profile_json(fe_solver* s)
         const TermProfile& tp = p.terms[t];
         out << (t ? "," : "") << "{\"name\":";
         json_string(out, tp.name);
+        out << ",\"kernel\":";
+        json_string(out, tp.kernel);
         out << ",\"ms\":" << tp.ms << ",\"launches\":" << tp.launches << ",\"substeps\":" << tp.substeps
             << ",\"learned\":" << (tp.learned ? "true" : "false") << ",\"fell_back\":" << (tp.fell_back ? "true" : "false") << '}';
     }

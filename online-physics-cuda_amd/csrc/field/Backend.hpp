@@ -86,6 +86,7 @@ struct TermProfile {
     bool idle = false;
     bool learned = false;
     bool fell_back = false;
+    std::string kernel; // the gather kernel the term's last substep launched (for profilers' names)
 };
 
 struct StepProfile {

@@ -89,6 +89,7 @@ struct Impl final : LawBackend {
     // other): FIELD_HIP_TILE="<cells a lane>x<rows>", FIELD_HIP_ZCHUNK=<planes a tile>, FIELD_HIP_DIRECT=1.
     int tile_vx = 0, tile_ty = 0, zchunk_override = 0;
     bool force_direct = false;
+    std::string last_gather;
 
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
@@ -189,6 +190,7 @@ struct Impl final : LawBackend {
     void launch_gather(const hip::ProgramCode& code, double dt)
     {
         const hip::WindowKernel* w = window_for(code);
+        last_gather = w == nullptr ? code.gather : w->name;
         if (w == nullptr) {
             launch_cells(code.gather, dt);
             return;
@@ -789,6 +791,8 @@ struct Impl final : LawBackend {
                     violations, judges, total_slot);
             }
             check(fe_hip_event_record(event(ev + 1)), "event");
+            tp.kernel = last_gather;
+            last_gather.clear();
             tp.substeps = substeps[t];
             tp.launches = prof.launches - launches0;
             ev += 2;
