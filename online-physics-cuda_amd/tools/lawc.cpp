@@ -2,6 +2,7 @@
 //
 //   lawc --ir LAW...              the IR as a listing
 //   lawc --ir-json OUT LAW...     the IR as JSON (the reference's format)
+//   lawc --hip OUT LAW...         the generated gfx950 translation unit (the reference's --cuda, main.cpp:221-224)
 #include "fe_field.h"
 
 #include <cstdio>
@@ -15,6 +16,7 @@ int main(int argc, char** argv)
 {
     bool listing = false;
     std::string ir_json;
+    std::string hip_out;
     std::vector<std::string> paths;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -22,8 +24,10 @@ int main(int argc, char** argv)
             listing = true;
         } else if (a == "--ir-json" && i + 1 < argc) {
             ir_json = argv[++i];
+        } else if (a == "--hip" && i + 1 < argc) {
+            hip_out = argv[++i];
         } else if (a == "--help" || a == "-h") {
-            std::printf("lawc [--ir] [--ir-json F] LAW...\n");
+            std::printf("lawc [--ir] [--ir-json F] [--hip F] LAW...\n");
             return 0;
         } else {
             paths.push_back(a);
@@ -68,6 +72,19 @@ int main(int argc, char** argv)
         fe_string_free(s);
         std::printf("IR written to %s\n", ir_json.c_str());
     }
+    int status = 0;
+    if (!hip_out.empty()) {
+        std::vector<char> why(4096);
+        char* src = fe_law_hip_source(law, why.data(), why.size());
+        if (src == nullptr) {
+            std::fprintf(stderr, "no HIP source: %s\n", why.data());
+            status = 1;
+        } else {
+            std::ofstream(hip_out) << src;
+            fe_string_free(src);
+            std::printf("HIP written to %s\n", hip_out.c_str());
+        }
+    }
     fe_law_free(law);
-    return 0;
+    return status;
 }
