@@ -1,0 +1,829 @@
+// Batched Heston ADI on gfx950: thousands of options of uneven lattice sizes stepped together, every step four
+// batched passes over all options, the whole run one captured graph.
+//
+// The scheme and operators are those of the engine's Heston ADI laws (one option a plane; the reference's
+// bench/laws/heston_adi.law:1-51 and bench/finance.py:253-288): x = ln S along x (cells of h = 6 / nx on ln 100 +- 3),
+// var = sigma (j + 1/2) h along y, F = A0 + A1 + A2 with three-point A1 (x) and A2 (y), A0 = rho var gx(gy(.)), the
+// edge cell repeating past the ends. In increments (delta form), for Craig-Sneyd (scheme 0) or Hundsdorfer-Verwer (1):
+//   pass 1 (rows):    T1 = (I - th dt A1)^-1 dt F(U)
+//   pass 2 (columns): D2 = (I - th dt A2)^-1 T1
+//   pass 3 (rows):    T1 = (I - th dt A1)^-1 (CS: dt/2 A0 D2;  HV: dt F(U) - D2 + dt/2 F(D2))
+//   pass 4 (columns): U += D2 + (I - th dt A2)^-1 T1
+// in fp32 (multiply-adds fused).
+//
+// Rows (passes 1 and 3): L lanes a row, K = nx / L cells a lane, in registers: each lane reduces its chunk of the
+// tridiagonal system (a downward and an upward sweep) to one equation in the last unknowns of its own and its
+// neighbours' chunks, the L equations are solved by parallel cyclic reduction across the lanes (shuffles), and each
+// lane back-substitutes its chunk. K (Thomas work a lane) against L (PCR depth) is the split the tuner picks.
+// Columns (passes 2 and 4): a thread a column, Thomas with the option's factorisation of (I - th dt A2) (the same for
+// every column: it depends on the row only), computed once in double on the host; the forward sweep writes d' in
+// place and the backward sweep reads it back. Loads are coalesced across the columns of a block.
+//
+// Options of one lattice shape form a class; a pass is one launch a class, each block a run of rows (columns) of one
+// option from a task table.
+#include "fe_adi.h"
+#include "fe_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(fast)
+
+namespace {
+
+struct HOpt {
+    long long off; // first cell of the option's lattice (row-major, x fastest)
+    int nx, ny;
+    int row;       // its first entry in the row table
+    float h, dt, thdt, r;
+};
+
+struct HRow {
+    float lx, ux, ly, uy; // A1, A2 coefficients of the row
+    float rv;             // rho * var of the row
+    float ya, ycp, ym;    // (I - th dt A2) factorised: lower coefficient, c', 1 / pivot (edges folded)
+};
+
+enum Mode { kFirst = 0, kCsSecond = 1, kHvSecond = 2 };
+
+template <int K, int L>
+struct Row {
+    float v[K];
+};
+
+// Lane p of a group of L: the value of lane p - 1 (its own at p = 0) and of p + 1 (its own at p = L - 1).
+template <int L>
+__device__ __forceinline__ float from_left(float v, int p)
+{
+    const float o = __shfl_up(v, 1, L);
+    return p == 0 ? v : o;
+}
+template <int L>
+__device__ __forceinline__ float from_right(float v, int p)
+{
+    const float o = __shfl_down(v, 1, L);
+    return p == L - 1 ? v : o;
+}
+
+// F restricted to what `mode` needs, at the K cells of a lane in row j: dt F(W) or dt/2 A0 W, from rows jm, j, jp of W.
+template <int K, int L>
+__device__ __forceinline__ void apply(const Row<K, L>& wm, const Row<K, L>& w, const Row<K, L>& wp, const HRow& cf, float h,
+    float r, float span_y, int p, bool mixed_only, float scale, float* out)
+{
+    // gy at the lane's cells and at the cells either side (from the neighbouring lanes); gx(gy) one-sided at the ends.
+    float gy[K];
+    const float iy = span_y > 0.0f ? 1.0f / (h * span_y) : 0.0f;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        gy[q] = (wp.v[q] - wm.v[q]) * iy;
+    }
+    const float gy_left = from_left<L>(gy[K - 1], p);
+    const float gy_right = from_right<L>(gy[0], p);
+    const float w_left = from_left<L>(w.v[K - 1], p);
+    const float w_right = from_right<L>(w.v[0], p);
+    const float ih = 1.0f / h, ih2 = 0.5f / h;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const bool first = p == 0 && q == 0;
+        const bool last = p == L - 1 && q == K - 1;
+        const float gl = q == 0 ? gy_left : gy[q - 1];
+        const float gr = q == K - 1 ? gy_right : gy[q + 1];
+        // central (gr - gl) / 2h; at an end the cell itself stands for the missing one and the span is h
+        const float gxy = (first || last) ? (gr - gl) * ih : (gr - gl) * ih2;
+        float f = cf.rv * gxy;
+        if (!mixed_only) {
+            const float c = w.v[q];
+            const float l = q == 0 ? w_left : w.v[q - 1];
+            const float rr = q == K - 1 ? w_right : w.v[q + 1];
+            f += cf.lx * (l - c) + cf.ux * (rr - c) + cf.ly * (wm.v[q] - c) + cf.uy * (wp.v[q] - c) - r * c;
+        }
+        out[q] = scale * f;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void load_row(const float* p, float (&v)[K])
+{
+    if constexpr (K % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < K; q += 4) {
+            const float4 t = *reinterpret_cast<const float4*>(p + q);
+            v[q] = t.x;
+            v[q + 1] = t.y;
+            v[q + 2] = t.z;
+            v[q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q += 2) {
+            const float2 t = *reinterpret_cast<const float2*>(p + q);
+            v[q] = t.x;
+            v[q + 1] = t.y;
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void store_row(float* p, const float (&v)[K])
+{
+    if constexpr (K % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < K; q += 4) {
+            *reinterpret_cast<float4*>(p + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q += 2) {
+            *reinterpret_cast<float2*>(p + q) = make_float2(v[q], v[q + 1]);
+        }
+    }
+}
+
+// Solves (b on the diagonal, a below, c above; the first row's a and the last row's c folded onto the diagonal) for
+// the row whose right-hand side the L lanes of a group hold, K cells a lane; the answer replaces d.
+template <int K, int L>
+__device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float c, int p)
+{
+    static_assert(K >= 2, "a lane holds two cells at least");
+    float C[K], A[K], D[K];
+    // Downward: x_i + C_i x_{i+1} + A_i x_{s-1} = D_i, x_{s-1} the last unknown of the lane before.
+    {
+        const bool first = p == 0;
+        const float bi = b + (first ? a : 0.0f);
+        const float m = 1.0f / bi;
+        C[0] = c * m;
+        A[0] = first ? 0.0f : a * m;
+        D[0] = d[0] * m;
+    }
+#pragma unroll
+    for (int q = 1; q < K; ++q) {
+        const bool last = p == L - 1 && q == K - 1;
+        const float bi = b + (last ? c : 0.0f);
+        const float m = 1.0f / (bi - a * C[q - 1]);
+        C[q] = last ? 0.0f : c * m;
+        A[q] = -a * A[q - 1] * m;
+        D[q] = (d[q] - a * D[q - 1]) * m;
+    }
+    // Upward: x_q = P_q - Q_q x_e - R_q x_{s-1} for q < K - 1 (x_e the lane's own last unknown).
+    float P[K], Q[K], R[K];
+    P[K - 2] = D[K - 2];
+    Q[K - 2] = C[K - 2];
+    R[K - 2] = A[K - 2];
+#pragma unroll
+    for (int q = K - 3; q >= 0; --q) {
+        P[q] = D[q] - C[q] * P[q + 1];
+        Q[q] = -C[q] * Q[q + 1];
+        R[q] = A[q] - C[q] * R[q + 1];
+    }
+    // One equation a lane in u_p = x_e: the last row, with x_{e+1} (the next lane's first) = P' - Q' u_{p+1} - R' u_p.
+    const float p1 = __shfl_down(P[0], 1, L), q1 = __shfl_down(Q[0], 1, L), r1 = __shfl_down(R[0], 1, L);
+    const bool tail = p == L - 1;
+    const float ce = C[K - 1];
+    float al = A[K - 1];
+    float be = 1.0f - (tail ? 0.0f : ce * r1);
+    float ga = tail ? 0.0f : -ce * q1;
+    float de = D[K - 1] - (tail ? 0.0f : ce * p1);
+    // Parallel cyclic reduction over the L lanes.
+#pragma unroll
+    for (int s = 1; s < L; s <<= 1) {
+        const float alm = __shfl_up(al, s, L), bem = __shfl_up(be, s, L), gam = __shfl_up(ga, s, L), dem = __shfl_up(de, s, L);
+        const float alp = __shfl_down(al, s, L), bep = __shfl_down(be, s, L), gap = __shfl_down(ga, s, L),
+                    dep = __shfl_down(de, s, L);
+        const bool has_m = p >= s, has_p = p + s < L;
+        const float k1 = has_m ? al / bem : 0.0f;
+        const float k2 = has_p ? ga / bep : 0.0f;
+        const float nal = has_m ? -k1 * alm : 0.0f;
+        const float nga = has_p ? -k2 * gap : 0.0f;
+        const float nbe = be - (has_m ? k1 * gam : 0.0f) - (has_p ? k2 * alp : 0.0f);
+        const float nde = de - (has_m ? k1 * dem : 0.0f) - (has_p ? k2 * dep : 0.0f);
+        al = nal;
+        be = nbe;
+        ga = nga;
+        de = nde;
+    }
+    const float u = de / be;
+    const float um = from_left<L>(u, p);
+    const float ul = p == 0 ? 0.0f : um;
+    d[K - 1] = u;
+#pragma unroll
+    for (int q = 0; q < K - 1; ++q) {
+        d[q] = P[q] - Q[q] * u - R[q] * ul;
+    }
+}
+
+struct Task {
+    int opt;
+    int first; // first row (passes 1, 3) or first column (passes 2, 4)
+};
+
+// Passes 1 and 3: a block of WX waves, each wave 64 / L rows, one option's consecutive rows.
+template <int K, int L, int MODE>
+__global__ __launch_bounds__(256) void k_rows(const HOpt* __restrict__ opts, const HRow* __restrict__ rows,
+    const Task* __restrict__ tasks, const float* __restrict__ U, const float* __restrict__ D2, float* __restrict__ T1)
+{
+    const Task t = tasks[blockIdx.x];
+    const HOpt o = opts[t.opt];
+    const int lane = (int)(threadIdx.x & 63u);
+    const int group = (int)(threadIdx.x >> 6) * (64 / L) + lane / L;
+    const int p = lane % L;
+    const int j = t.first + group;
+    if (j >= o.ny) {
+        return; // the whole group: shuffles stay inside groups
+    }
+    const HRow cf = rows[o.row + j];
+    const int jm = j > 0 ? j - 1 : j, jp = j + 1 < o.ny ? j + 1 : j;
+    const float span_y = (float)(jp - jm);
+    const long long col = (long long)p * K;
+    float rhs[K];
+    Row<K, L> wm, w, wp;
+    const float* base = U + o.off + col; // the launch passes D2 here for the Craig-Sneyd correction (dt/2 A0 D2)
+    load_row<K>(base + (long long)jm * o.nx, wm.v);
+    load_row<K>(base + (long long)j * o.nx, w.v);
+    load_row<K>(base + (long long)jp * o.nx, wp.v);
+    if (MODE == kCsSecond) {
+        apply<K, L>(wm, w, wp, cf, o.h, o.r, span_y, p, true, 0.5f * o.dt, rhs);
+    } else {
+        apply<K, L>(wm, w, wp, cf, o.h, o.r, span_y, p, false, o.dt, rhs);
+    }
+    if (MODE == kHvSecond) {
+        // dt F(U) - D2 + dt/2 F(D2)
+        const float* dbase = D2 + o.off + col;
+        load_row<K>(dbase + (long long)jm * o.nx, wm.v);
+        load_row<K>(dbase + (long long)j * o.nx, w.v);
+        load_row<K>(dbase + (long long)jp * o.nx, wp.v);
+        float fd[K];
+        apply<K, L>(wm, w, wp, cf, o.h, o.r, span_y, p, false, 0.5f * o.dt, fd);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            rhs[q] += fd[q] - w.v[q];
+        }
+    }
+    const float cx = -cf.lx - cf.ux - 0.5f * o.r;
+    solve_row<K, L>(rhs, -o.thdt * cf.lx, 1.0f - o.thdt * cx, -o.thdt * cf.ux, p);
+    store_row<K>(T1 + o.off + (long long)j * o.nx + col, rhs);
+}
+
+// Passes 2 and 4: a thread a column of one option. FINAL: U += D2 + x (and D2 is only read); else D2 = x.
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_cols(const HOpt* __restrict__ opts, const HRow* __restrict__ rows,
+    const Task* __restrict__ tasks, float* __restrict__ U, float* __restrict__ D2, float* __restrict__ T1)
+{
+    const Task t = tasks[blockIdx.x];
+    const HOpt o = opts[t.opt];
+    const int i = t.first + (int)threadIdx.x;
+    if (i >= o.nx) {
+        return;
+    }
+    const long long nx = o.nx;
+    float* col = T1 + o.off + i;
+    const HRow* cf = rows + o.row;
+    float prev = 0.0f;
+    for (int j = 0; j < o.ny; ++j) {
+        const float dj = col[j * nx];
+        prev = (dj - (j == 0 ? 0.0f : cf[j].ya * prev)) * cf[j].ym;
+        col[j * nx] = prev;
+    }
+    float next = 0.0f;
+    float* d2 = D2 + o.off + i;
+    float* u = U + o.off + i;
+    for (int j = o.ny - 1; j >= 0; --j) {
+        next = col[j * nx] - cf[j].ycp * next;
+        if (FINAL) {
+            u[j * nx] = u[j * nx] + (d2[j * nx] + next);
+        } else {
+            d2[j * nx] = next;
+        }
+    }
+}
+
+// The cell-averaged put payoff (in double, rounded once).
+__global__ void k_payoff(const HOpt* __restrict__ opts, const double* __restrict__ strikes, int n, float* __restrict__ U)
+{
+    const int o_id = (int)blockIdx.y;
+    if (o_id >= n) {
+        return;
+    }
+    const HOpt o = opts[o_id];
+    const long long cells = (long long)o.nx * o.ny;
+    const double k = strikes[o_id], lk = log(k), h = (double)6.0 / o.nx;
+    const double x0 = log(100.0) - 3.0 + 0.5 * h;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % o.nx);
+        const double x = x0 + i * h;
+        const double lo = fmin(x - 0.5 * h, lk), hi = fmin(x + 0.5 * h, lk);
+        U[o.off + c] = (float)((k * (hi - lo) - (exp(hi) - exp(lo))) / h);
+    }
+}
+
+// Prices: the 4 x 4 Lagrange stencil of each option at (ln 100, v0), weights from the host.
+struct Stencil {
+    int i0, j0;
+    double wx[4], wy[4];
+};
+
+__global__ void k_price(const HOpt* __restrict__ opts, const Stencil* __restrict__ st, int n, const float* __restrict__ U,
+    double* __restrict__ out)
+{
+    const int o_id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (o_id >= n) {
+        return;
+    }
+    const HOpt o = opts[o_id];
+    const Stencil s = st[o_id];
+    double p = 0.0;
+    for (int b = 0; b < 4; ++b) {
+        double row = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            row += s.wx[a] * (double)U[o.off + (long long)(s.j0 + b) * o.nx + s.i0 + a];
+        }
+        p += s.wy[b] * row;
+    }
+    out[o_id] = p;
+}
+
+void hcheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template <typename T>
+struct Dev {
+    T* p = nullptr;
+    std::size_t n = 0;
+    Dev() = default;
+    Dev(const Dev&) = delete;
+    Dev& operator=(const Dev&) = delete;
+    Dev(Dev&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    ~Dev()
+    {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+    }
+    void alloc(std::size_t count)
+    {
+        if (p != nullptr) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        n = count;
+        hcheck(hipMalloc(reinterpret_cast<void**>(&p), std::max<std::size_t>(1, count) * sizeof(T)), "hipMalloc");
+    }
+    void upload(const std::vector<T>& v)
+    {
+        alloc(v.size());
+        hcheck(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+};
+
+using RowKernel = void (*)(const HOpt*, const HRow*, const Task*, const float*, const float*, float*);
+
+template <int K, int L>
+RowKernel row_kernel(int mode)
+{
+    switch (mode) {
+    case kFirst: return k_rows<K, L, kFirst>;
+    case kCsSecond: return k_rows<K, L, kCsSecond>;
+    default: return k_rows<K, L, kHvSecond>;
+    }
+}
+
+RowKernel pick_row_kernel(int k, int l, int mode)
+{
+#define FE_ROWK(KV, LV) if (k == KV && l == LV) return row_kernel<KV, LV>(mode);
+    FE_ROWK(2, 16) FE_ROWK(4, 16) FE_ROWK(8, 16) FE_ROWK(16, 16)
+    FE_ROWK(2, 32) FE_ROWK(4, 32) FE_ROWK(8, 32) FE_ROWK(16, 32)
+    FE_ROWK(2, 64) FE_ROWK(4, 64) FE_ROWK(8, 64) FE_ROWK(16, 64)
+#undef FE_ROWK
+    return nullptr;
+}
+
+} // namespace
+
+struct fe_heston {
+    struct Class {
+        int nx = 0, ny_max = 0;
+        std::vector<int> members;
+        int lanes = 64;  // L: lanes a row
+        int row_waves = 4;  // waves a block in the row passes
+        int col_threads = 64; // threads a block in the column passes
+        Dev<Task> row_tasks, col_tasks;
+        int n_row_tasks = 0, n_col_tasks = 0;
+    };
+    int n = 0;
+    int steps = 0;
+    int scheme = 0;
+    double theta = 0.5;
+    long long cells = 0;
+    std::vector<HOpt> host_opts;
+    std::vector<double> params; // 8 an option
+    std::vector<Class> classes;
+    Dev<HOpt> opts;
+    Dev<HRow> rows;
+    Dev<Stencil> stencils;
+    Dev<double> strikes;
+    Dev<double> prices;
+    Dev<float> U, D2, T1;
+    fe_hip_graph graph = nullptr;
+    std::string error;
+
+    ~fe_heston()
+    {
+        if (graph != nullptr) {
+            fe_hip_graph_destroy(graph);
+        }
+    }
+
+    hipStream_t stream() const { return static_cast<hipStream_t>(fe_hip_stream()); }
+
+    void build_tasks(Class& c)
+    {
+        std::vector<Task> rt, ct;
+        const int rows_a_block = c.row_waves * (64 / c.lanes);
+        for (const int o : c.members) {
+            const int ny = host_opts[static_cast<std::size_t>(o)].ny;
+            for (int j = 0; j < ny; j += rows_a_block) {
+                rt.push_back({o, j});
+            }
+            for (int i = 0; i < c.nx; i += c.col_threads) {
+                ct.push_back({o, i});
+            }
+        }
+        c.row_tasks.upload(rt);
+        c.col_tasks.upload(ct);
+        c.n_row_tasks = static_cast<int>(rt.size());
+        c.n_col_tasks = static_cast<int>(ct.size());
+    }
+
+    void queue_step()
+    {
+        const hipStream_t s = stream();
+        const int second = scheme == 0 ? kCsSecond : kHvSecond;
+        for (int pass = 0; pass < 4; ++pass) {
+            for (Class& c : classes) {
+                if (pass == 0 || pass == 2) {
+                    RowKernel k = pick_row_kernel(c.nx / c.lanes, c.lanes, pass == 0 ? kFirst : second);
+                    // pass 3 reads D2 rows for the mixed term (CS) or U and D2 (HV)
+                    const float* src = pass == 2 && scheme == 0 ? D2.p : U.p;
+                    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(c.n_row_tasks)), dim3(64 * c.row_waves), 0, s, opts.p, rows.p,
+                        c.row_tasks.p, src, D2.p, T1.p);
+                } else if (pass == 1) {
+                    hipLaunchKernelGGL(k_cols<false>, dim3(static_cast<unsigned>(c.n_col_tasks)), dim3(c.col_threads), 0, s, opts.p,
+                        rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+                } else {
+                    hipLaunchKernelGGL(k_cols<true>, dim3(static_cast<unsigned>(c.n_col_tasks)), dim3(c.col_threads), 0, s, opts.p,
+                        rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+                }
+            }
+        }
+        hcheck(hipGetLastError(), "launch");
+    }
+
+    void reset()
+    {
+        const unsigned gy = static_cast<unsigned>(n);
+        hipLaunchKernelGGL(k_payoff, dim3(64, gy), dim3(256), 0, stream(), opts.p, strikes.p, n, U.p);
+        hcheck(hipGetLastError(), "payoff");
+    }
+};
+
+namespace {
+
+int fail(fe_heston* b, const std::string& what)
+{
+    if (b != nullptr) {
+        b->error = what;
+    }
+    return -1;
+}
+
+void set_why(char* why, std::size_t bytes, const std::string& text)
+{
+    if (why != nullptr && bytes > 0) {
+        std::snprintf(why, bytes, "%s", text.c_str());
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+fe_heston* fe_heston_create(int n, const int* nx, const int* ny, const double* options, int steps, int scheme, double theta, char* why,
+    size_t why_bytes)
+{
+    try {
+        if (n <= 0 || n > 65535 || nx == nullptr || ny == nullptr || options == nullptr || steps <= 0 || (scheme != 0 && scheme != 1) ||
+            !(theta > 0.0)) {
+            set_why(why, why_bytes, "invalid arguments");
+            return nullptr;
+        }
+        fe_hip_device_info info{};
+        if (fe_hip_device_open(&info) != 0) {
+            set_why(why, why_bytes, std::string("no HIP device: ") + fe_hip_unavailable());
+            return nullptr;
+        }
+        auto b = std::make_unique<fe_heston>();
+        b->n = n;
+        b->steps = steps;
+        b->scheme = scheme;
+        b->theta = theta;
+        b->params.assign(options, options + 8 * static_cast<std::size_t>(n));
+        std::vector<HRow> rows;
+        std::vector<Stencil> st(static_cast<std::size_t>(n));
+        std::vector<double> strikes(static_cast<std::size_t>(n));
+        std::map<int, int> class_of; // nx -> class index
+        long long off = 0;
+        for (int o = 0; o < n; ++o) {
+            const int X = nx[o], Y = ny[o];
+            const double* q = options + 8 * static_cast<std::size_t>(o);
+            const double kappa = q[0], theta_v = q[1], sigma = q[2], rho = q[3], r = q[4], strike = q[5], mat = q[6], v0 = q[7];
+            bool shape_ok = false;
+            for (const int l : {64, 32, 16}) {
+                const int k = X / l;
+                shape_ok = shape_ok || (X % l == 0 && (k == 2 || k == 4 || k == 8 || k == 16));
+            }
+            if (!shape_ok || Y < 4) {
+                set_why(why, why_bytes, "option " + std::to_string(o) + ": lattice " + std::to_string(X) + "x" + std::to_string(Y) +
+                    " (nx must be 32..1024, a power of two; ny >= 4)");
+                return nullptr;
+            }
+            if (!(sigma > 0.0) || !(mat > 0.0) || !(strike > 0.0)) {
+                set_why(why, why_bytes, "option " + std::to_string(o) + ": sigma, maturity and strike must be positive");
+                return nullptr;
+            }
+            const double h = 6.0 / X;
+            const double dt = mat / steps;
+            const double thdt = theta * dt;
+            HOpt ho{};
+            ho.off = off;
+            ho.nx = X;
+            ho.ny = Y;
+            ho.row = static_cast<int>(rows.size());
+            ho.h = static_cast<float>(h);
+            ho.dt = static_cast<float>(dt);
+            ho.thdt = static_cast<float>(thdt);
+            ho.r = static_cast<float>(r);
+            b->host_opts.push_back(ho);
+            off += static_cast<long long>(X) * Y;
+            // Rows: the operator coefficients (as the oracle's grid) and (I - th dt A2) factorised in double.
+            std::vector<double> lyv(static_cast<std::size_t>(Y)), uyv(static_cast<std::size_t>(Y));
+            double cprev = 0.0;
+            for (int j = 0; j < Y; ++j) {
+                const double v = sigma * (j + 0.5) * h;
+                const double e0 = v < sigma * h ? 1.0 : 0.0;
+                const double bb = kappa * (theta_v - v) / sigma - 0.5 * sigma;
+                HRow hr{};
+                const double lx = 0.5 * v / (h * h) - (r - 0.5 * v) / (2 * h);
+                const double ux = 0.5 * v / (h * h) + (r - 0.5 * v) / (2 * h);
+                const double ly = 0.5 * (v - 0.5 * sigma * h) / (h * h) - (1 - e0) * bb / (2 * h);
+                const double uy = 0.5 * (v + 0.5 * sigma * h) / (h * h) + (1 + e0) * bb / (2 * h);
+                hr.lx = static_cast<float>(lx);
+                hr.ux = static_cast<float>(ux);
+                hr.ly = static_cast<float>(ly);
+                hr.uy = static_cast<float>(uy);
+                hr.rv = static_cast<float>(rho * v);
+                const double a = -thdt * ly, c = -thdt * uy;
+                const double cy = -ly - uy - 0.5 * r;
+                const double bd = 1.0 - thdt * cy + (j == 0 ? a : 0.0) + (j + 1 == Y ? c : 0.0);
+                const double piv = j == 0 ? bd : bd - a * cprev;
+                const double cp = j + 1 == Y ? 0.0 : c / piv;
+                hr.ya = static_cast<float>(a);
+                hr.ycp = static_cast<float>(cp);
+                hr.ym = static_cast<float>(1.0 / piv);
+                cprev = cp;
+                rows.push_back(hr);
+            }
+            // Price stencil: 4-point Lagrange at ln 100 along x and at v0 along y.
+            Stencil s{};
+            const double x0 = std::log(100.0) - 3.0 + 0.5 * h;
+            const double tx = (std::log(100.0) - x0) / h;
+            s.i0 = std::clamp(static_cast<int>(std::floor(tx)) - 1, 0, X - 4);
+            const double ty = (v0 / sigma) / h - 0.5;
+            s.j0 = std::clamp(static_cast<int>(std::floor(ty)) - 1, 0, Y - 4);
+            auto lag = [](double t, double* w) {
+                w[0] = -(t - 1) * (t - 2) * (t - 3) / 6.0;
+                w[1] = t * (t - 2) * (t - 3) / 2.0;
+                w[2] = -t * (t - 1) * (t - 3) / 2.0;
+                w[3] = t * (t - 1) * (t - 2) / 6.0;
+            };
+            lag(tx - s.i0, s.wx);
+            lag(ty - s.j0, s.wy);
+            st[static_cast<std::size_t>(o)] = s;
+            strikes[static_cast<std::size_t>(o)] = strike;
+            auto it = class_of.find(X);
+            if (it == class_of.end()) {
+                it = class_of.emplace(X, static_cast<int>(b->classes.size())).first;
+                b->classes.emplace_back();
+                b->classes.back().nx = X;
+            }
+            fe_heston::Class& cl = b->classes[static_cast<std::size_t>(it->second)];
+            cl.members.push_back(o);
+            cl.ny_max = std::max(cl.ny_max, Y);
+        }
+        b->cells = off;
+        b->opts.upload(b->host_opts);
+        b->rows.upload(rows);
+        b->stencils.upload(st);
+        b->strikes.upload(strikes);
+        b->prices.alloc(static_cast<std::size_t>(n));
+        b->U.alloc(static_cast<std::size_t>(off));
+        b->D2.alloc(static_cast<std::size_t>(off));
+        b->T1.alloc(static_cast<std::size_t>(off));
+        for (fe_heston::Class& c : b->classes) {
+            // default split: four cells a lane at most 64 lanes (the tuner may change it)
+            c.lanes = std::min(64, std::max(16, c.nx / 4));
+            while (c.nx / c.lanes > 16) {
+                c.lanes *= 2;
+            }
+            b->build_tasks(c);
+        }
+        b->reset();
+        hcheck(hipStreamSynchronize(b->stream()), "sync");
+        return b.release();
+    } catch (const std::exception& e) {
+        set_why(why, why_bytes, e.what());
+        return nullptr;
+    }
+}
+
+void fe_heston_free(fe_heston* b) { delete b; }
+
+const char* fe_heston_error(fe_heston* b) { return b != nullptr ? b->error.c_str() : "no batch"; }
+
+long long fe_heston_cells(fe_heston* b) { return b != nullptr ? b->cells : 0; }
+
+int fe_heston_reset(fe_heston* b)
+{
+    if (b == nullptr) {
+        return -1;
+    }
+    try {
+        b->reset();
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
+}
+
+int fe_heston_step(fe_heston* b, int count)
+{
+    if (b == nullptr || count < 0) {
+        return -1;
+    }
+    try {
+        for (int s = 0; s < count; ++s) {
+            b->queue_step();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
+}
+
+int fe_heston_capture(fe_heston* b, int count)
+{
+    if (b == nullptr || count <= 0) {
+        return -1;
+    }
+    try {
+        if (b->graph != nullptr) {
+            fe_hip_graph_destroy(b->graph);
+            b->graph = nullptr;
+        }
+        hcheck(hipStreamSynchronize(b->stream()), "sync");
+        if (fe_hip_graph_begin_capture() != 0) {
+            return fail(b, "graph capture did not start");
+        }
+        try {
+            for (int s = 0; s < count; ++s) {
+                b->queue_step();
+            }
+        } catch (...) {
+            fe_hip_graph graph = nullptr;
+            fe_hip_graph_end_capture(&graph);
+            if (graph != nullptr) {
+                fe_hip_graph_destroy(graph);
+            }
+            throw;
+        }
+        const int r = fe_hip_graph_end_capture(&b->graph);
+        if (r != 0) {
+            b->graph = nullptr;
+            return fail(b, std::string("graph capture: ") + fe_hip_error_string(r));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
+}
+
+int fe_heston_launch(fe_heston* b)
+{
+    if (b == nullptr || b->graph == nullptr) {
+        return fail(b, "no captured graph");
+    }
+    const int r = fe_hip_graph_launch(b->graph);
+    return r == 0 ? 0 : fail(b, std::string("graph launch: ") + fe_hip_error_string(r));
+}
+
+int fe_heston_sync(fe_heston* b)
+{
+    if (b == nullptr) {
+        return -1;
+    }
+    const hipError_t e = hipStreamSynchronize(b->stream());
+    return e == hipSuccess ? 0 : fail(b, hipGetErrorString(e));
+}
+
+int fe_heston_prices(fe_heston* b, double* out)
+{
+    if (b == nullptr || out == nullptr) {
+        return -1;
+    }
+    try {
+        hipLaunchKernelGGL(k_price, dim3(static_cast<unsigned>((b->n + 255) / 256)), dim3(256), 0, b->stream(), b->opts.p, b->stencils.p,
+            b->n, b->U.p, b->prices.p);
+        hcheck(hipGetLastError(), "price");
+        hcheck(hipMemcpyAsync(out, b->prices.p, sizeof(double) * static_cast<std::size_t>(b->n), hipMemcpyDeviceToHost, b->stream()),
+            "prices");
+        hcheck(hipStreamSynchronize(b->stream()), "sync");
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
+}
+
+int fe_heston_lattice(fe_heston* b, int option, float* out)
+{
+    if (b == nullptr || out == nullptr || option < 0 || option >= b->n) {
+        return -1;
+    }
+    const HOpt& o = b->host_opts[static_cast<std::size_t>(option)];
+    const hipError_t e = hipMemcpyAsync(out, b->U.p + o.off, sizeof(float) * static_cast<std::size_t>(o.nx) * o.ny, hipMemcpyDeviceToHost,
+        b->stream());
+    if (e != hipSuccess) {
+        return fail(b, hipGetErrorString(e));
+    }
+    return hipStreamSynchronize(b->stream()) == hipSuccess ? 0 : fail(b, "sync");
+}
+
+int fe_heston_classes(fe_heston* b) { return b != nullptr ? static_cast<int>(b->classes.size()) : 0; }
+
+int fe_heston_get_shape(fe_heston* b, int cls, int* nx, int* lanes, int* row_waves, int* col_threads)
+{
+    if (b == nullptr || cls < 0 || cls >= static_cast<int>(b->classes.size())) {
+        return -1;
+    }
+    const fe_heston::Class& c = b->classes[static_cast<std::size_t>(cls)];
+    *nx = c.nx;
+    *lanes = c.lanes;
+    *row_waves = c.row_waves;
+    *col_threads = c.col_threads;
+    return 0;
+}
+
+int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col_threads)
+{
+    if (b == nullptr || cls < 0 || cls >= static_cast<int>(b->classes.size())) {
+        return -1;
+    }
+    fe_heston::Class& c = b->classes[static_cast<std::size_t>(cls)];
+    const int k = lanes > 0 ? c.nx / lanes : 0;
+    if (!(lanes == 16 || lanes == 32 || lanes == 64) || c.nx % lanes != 0 || !(k == 2 || k == 4 || k == 8 || k == 16) ||
+        !(row_waves == 1 || row_waves == 2 || row_waves == 4) || !(col_threads == 64 || col_threads == 128 || col_threads == 256)) {
+        return fail(b, "shape out of range");
+    }
+    try {
+        if (b->graph != nullptr) {
+            fe_hip_graph_destroy(b->graph);
+            b->graph = nullptr;
+        }
+        hcheck(hipStreamSynchronize(b->stream()), "sync");
+        c.lanes = lanes;
+        c.row_waves = row_waves;
+        c.col_threads = col_threads;
+        b->build_tasks(c);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
+}
+
+} // extern "C"

@@ -93,6 +93,22 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_digest": (ctypes.c_uint64, [c_void_p]),
         "fe_solver_error": (ctypes.c_char_p, [c_void_p]),
         "fe_solver_field_device": (c_void_p, [c_void_p, c_int]),
+        "fe_heston_create": (c_void_p, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_char_p,
+                                        c_size_t]),
+        "fe_heston_free": (None, [c_void_p]),
+        "fe_heston_error": (ctypes.c_char_p, [c_void_p]),
+        "fe_heston_cells": (ctypes.c_longlong, [c_void_p]),
+        "fe_heston_reset": (c_int, [c_void_p]),
+        "fe_heston_step": (c_int, [c_void_p, c_int]),
+        "fe_heston_capture": (c_int, [c_void_p, c_int]),
+        "fe_heston_launch": (c_int, [c_void_p]),
+        "fe_heston_sync": (c_int, [c_void_p]),
+        "fe_heston_prices": (c_int, [c_void_p, c_void_p]),
+        "fe_heston_lattice": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_heston_classes": (c_int, [c_void_p]),
+        "fe_heston_get_shape": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                        ctypes.POINTER(c_int)]),
+        "fe_heston_set_shape": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -392,3 +408,77 @@ class Solver:
 
     def field_device(self, field) -> int:
         return lib().fe_solver_field_device(self.ptr, self._field(field))
+
+
+class HestonBatch:
+    """Options of uneven lattices stepped together on the card by ADI (include/fe_adi.h). options: n x 8 (kappa, theta,
+    sigma, rho, r, strike, maturity, v0); scheme 0 Craig-Sneyd, 1 Hundsdorfer-Verwer."""
+
+    CS, HV = 0, 1
+
+    def __init__(self, nx, ny, options, steps: int, scheme: int = 0, theta: float = 0.5):
+        import numpy as np
+
+        self.nx = np.ascontiguousarray(nx, dtype=np.int32)
+        self.ny = np.ascontiguousarray(ny, dtype=np.int32)
+        self.options = np.ascontiguousarray(options, dtype=np.float64).reshape(-1, 8)
+        self.n = len(self.nx)
+        self.steps = steps
+        why = ctypes.create_string_buffer(2048)
+        self.ptr = lib().fe_heston_create(self.n, self.nx.ctypes.data, self.ny.ctypes.data, self.options.ctypes.data, steps,
+                                          scheme, theta, why, len(why))
+        if not self.ptr:
+            raise FieldHipError("batch refused: " + why.value.decode())
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fe_heston_free(self.ptr)
+            self.ptr = None
+
+    def _check(self, status: int, what: str) -> None:
+        if status != 0:
+            raise FieldHipError(f"{what}: {lib().fe_heston_error(self.ptr).decode()}")
+
+    @property
+    def cells(self) -> int:
+        return int(lib().fe_heston_cells(self.ptr))
+
+    def reset(self) -> None:
+        self._check(lib().fe_heston_reset(self.ptr), "reset")
+
+    def step(self, count: int = 1) -> None:
+        self._check(lib().fe_heston_step(self.ptr, count), "step")
+
+    def capture(self, count: int) -> None:
+        self._check(lib().fe_heston_capture(self.ptr, count), "capture")
+
+    def launch(self) -> None:
+        self._check(lib().fe_heston_launch(self.ptr), "launch")
+
+    def sync(self) -> None:
+        self._check(lib().fe_heston_sync(self.ptr), "sync")
+
+    def prices(self):
+        import numpy as np
+
+        out = np.empty(self.n, np.float64)
+        self._check(lib().fe_heston_prices(self.ptr, out.ctypes.data), "prices")
+        return out
+
+    def lattice(self, option: int):
+        import numpy as np
+
+        out = np.empty(int(self.nx[option]) * int(self.ny[option]), np.float32)
+        self._check(lib().fe_heston_lattice(self.ptr, option, out.ctypes.data), "lattice")
+        return out
+
+    def shapes(self):
+        out = []
+        for c in range(lib().fe_heston_classes(self.ptr)):
+            v = [ctypes.c_int() for _ in range(4)]
+            lib().fe_heston_get_shape(self.ptr, c, *[ctypes.byref(x) for x in v])
+            out.append({"nx": v[0].value, "lanes": v[1].value, "row_waves": v[2].value, "col_threads": v[3].value})
+        return out
+
+    def set_shape(self, cls: int, lanes: int, row_waves: int, col_threads: int) -> None:
+        self._check(lib().fe_heston_set_shape(self.ptr, cls, lanes, row_waves, col_threads), "set_shape")

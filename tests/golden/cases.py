@@ -108,6 +108,61 @@ OPS = {
 }
 
 
+# Heston by ADI, one option a z-plane (its model in static fields), x = ln S along x, var = sv (y + 1/2) h along y.
+# The explicit part f0 = F(V) with the mixed, x and y operators (each of the last two carrying half of -rate V); two
+# implicit sweeps of increments; then a second pair: Craig-Sneyd corrects only the mixed part (cs = 1/2).
+_HESTON_HEAD = """field V init 0 range -1 1000000
+field var init 0 static
+field kap init 1 static
+field th init 0.04 static
+field sv init 0.3 static
+field rho init -0.5 static
+field rate init 0.02 static
+param cs = {theta}
+
+term {name}
+  let edge = step(sv * h - var)
+  let drift = kap * (th - var) / sv - 0.5 * sv
+  let xl = 0.5 * var / (h * h) - (rate - 0.5 * var) / (2 * h)
+  let xu = 0.5 * var / (h * h) + (rate - 0.5 * var) / (2 * h)
+  let yl = 0.5 * (var - 0.5 * sv * h) / (h * h) - (1 - edge) * drift / (2 * h)
+  let yu = 0.5 * (var + 0.5 * sv * h) / (h * h) + (1 + edge) * drift / (2 * h)
+  let xd = 0 - xl - xu - 0.5 * rate
+  let yd = 0 - yl - yu - 0.5 * rate
+  let f0 = rho * var * gx(gy(V)) + xl * (shift(V, -1, 0, 0) - V) + xu * (shift(V, 1, 0, 0) - V) + yl * (shift(V, 0, -1, 0) - V) + yu * (shift(V, 0, 1, 0) - V) - rate * V
+  let p1 = tridiag_x(0 - cs * dt * xl, 1 - cs * dt * xd, 0 - cs * dt * xu, dt * f0)
+  let p2 = tridiag_y(0 - cs * dt * yl, 1 - cs * dt * yd, 0 - cs * dt * yu, p1)
+"""
+HESTON_CS_LAW = _HESTON_HEAD.format(theta=0.5, name="heston_cs") + """  let q1 = tridiag_x(0 - cs * dt * xl, 1 - cs * dt * xd, 0 - cs * dt * xu, 0.5 * dt * rho * var * gx(gy(p2)))
+  let q2 = tridiag_y(0 - cs * dt * yl, 1 - cs * dt * yd, 0 - cs * dt * yu, q1)
+  d V = (p2 + q2) / dt
+"""
+HESTON_HV_LAW = _HESTON_HEAD.format(theta=0.7886751345948129, name="heston_hv") + """  let fp = rho * var * gx(gy(p2)) + xl * (shift(p2, -1, 0, 0) - p2) + xu * (shift(p2, 1, 0, 0) - p2) + yl * (shift(p2, 0, -1, 0) - p2) + yu * (shift(p2, 0, 1, 0) - p2) - rate * p2
+  let q1 = tridiag_x(0 - cs * dt * xl, 1 - cs * dt * xd, 0 - cs * dt * xu, dt * f0 - p2 + 0.5 * dt * fp)
+  let q2 = tridiag_y(0 - cs * dt * yl, 1 - cs * dt * yd, 0 - cs * dt * yu, q1)
+  d V = (p2 + q2) / dt
+"""
+
+# Three options (kappa, theta, sigma, rho, r, strike, maturity, v0) on 32 x 16 cells, ln 100 +- 3 along x.
+HESTON_PLANES = [(1.5, 0.04, 0.3, -0.9, 0.025, 100.0, 0.5, 0.04), (3.0, 0.08, 0.5, -0.3, 0.01, 90.0, 0.5, 0.06),
+                 (0.8, 0.02, 0.2, -0.6, 0.04, 115.0, 0.5, 0.03)]
+
+
+def heston_state(nx, ny, planes):
+    """The law's start: each plane's cell-averaged put payoff in V, its variance lattice and model in the statics."""
+    h = 6.0 / nx
+    x = np.log(100.0) - 3.0 + (np.arange(nx) + 0.5) * h
+    out = {k: np.zeros((len(planes), ny, nx)) for k in ("V", "var", "kap", "th", "sv", "rho", "rate")}
+    for z, (kap, th, sv, rho, r, strike, _, _) in enumerate(planes):
+        lk = np.log(strike)
+        lo, hi = np.minimum(x - 0.5 * h, lk), np.minimum(x + 0.5 * h, lk)
+        out["V"][z] = ((strike * (hi - lo) - (np.exp(hi) - np.exp(lo))) / h)[None, :]
+        out["var"][z] = (sv * (np.arange(ny) + 0.5) * h)[:, None]
+        for key, val in (("kap", kap), ("th", th), ("sv", sv), ("rho", rho), ("rate", r)):
+            out[key][z] = val
+    return {k: v.astype(np.float32).reshape(-1) for k, v in out.items()}
+
+
 def op_law(expr: str) -> str:
     return ("field A init 0 range -1000000000 1000000000\nfield B init 0 range -1000000000 1000000000\n"
             "term t\n  " + expr + "\n")
@@ -209,6 +264,10 @@ CASES = [
     dict(name="multi", law=MULTI_LAW, grid=(18, 11, 7, 0.4), dt=0.25, steps=5, params={},
          init={"a": _uniform(18 * 11 * 7, 14, -1.0, 1.0), "b": _uniform(18 * 11 * 7, 15, 0.0, 1.9),
                "c": _uniform(18 * 11 * 7, 16, 0.0, 2.0)}),
+    dict(name="heston_cs_3", law=HESTON_CS_LAW, grid=(32, 16, 3, 6.0 / 32), dt=0.5 / 8, steps=8, params={},
+         init=heston_state(32, 16, HESTON_PLANES)),
+    dict(name="heston_hv_3", law=HESTON_HV_LAW, grid=(32, 16, 3, 6.0 / 32), dt=0.5 / 8, steps=8, params={},
+         init=heston_state(32, 16, HESTON_PLANES)),
 ] + [
     dict(name="op_" + key, law=op_law(expr), grid=G_OPS, dt=0.02, steps=4, params={},
          init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))

@@ -5,7 +5,7 @@ git-ignored) and run here, in the build container, once per case: a law (text wr
 initial state, N steps of dt. Each case's inputs and the reference's outputs are committed as one small .npz; the
 tests hold the CPU restatement (oracle/) and the HIP backend to them. Nothing here runs on the GPU box.
 
-    python tests/golden/make_golden.py            (from the repository root)
+    python tests/golden/make_golden.py [NAME...]    (from the repository root; no names: every case)
 """
 from __future__ import annotations
 
@@ -53,7 +53,10 @@ def run_reference(case: dict) -> dict:
 def main() -> None:
     if not os.path.exists(LAWC):
         sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    only = set(sys.argv[1:])
     for case in CASES:
+        if only and case["name"] not in only:
+            continue
         out = run_reference(case)
         arrays = {}
         for name, values in case["init"].items():
