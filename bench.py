@@ -132,6 +132,69 @@ def conduction(n: int, steps: int, warmup: int, seed: int = 2026, verify_steps: 
     return out
 
 
+ADI_SHAPES = ((128, 64), (256, 128), (512, 256))
+
+
+def heston_options(n: int, seed: int = 2026):
+    """n options: lattice S x v drawn from 128x64, 256x128, 512x256 (a third each, log-uniform in cells); kappa in
+    [0.5, 5], theta [0.01, 0.1], sigma [0.1, 0.8], rho [-0.9, -0.1], v0 [0.01, 0.1], strike [80, 120], maturity
+    [0.25, 2], r = 0.02, seeded uniform."""
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, 3, n)
+    nx = np.array([ADI_SHAPES[i][0] for i in idx], np.int32)
+    ny = np.array([ADI_SHAPES[i][1] for i in idx], np.int32)
+    opts = np.stack([rng.uniform(0.5, 5, n), rng.uniform(0.01, 0.1, n), rng.uniform(0.1, 0.8, n), rng.uniform(-0.9, -0.1, n),
+                     np.full(n, 0.02), rng.uniform(80, 120, n), rng.uniform(0.25, 2, n), rng.uniform(0.01, 0.1, n)], 1)
+    return nx, ny, opts
+
+
+def heston_adi(n: int, steps: int, sample_per_class: int, cpu: bool):
+    """The batch on the card (Craig-Sneyd, theta 1/2, fp32): tuned, every step captured in one graph, one launch timed;
+    prices of a sample (the same mix) against the FP64 oracle; the FP32 oracle on the host's cores on that sample."""
+    nx, ny, opts = heston_options(n)
+    b = field_hip.HestonBatch(nx, ny, opts, steps, 0, 0.5)
+    cells = b.cells
+    t = time.time()
+    tuned = b.tune(2)
+    tune_s = time.time() - t
+    b.capture(steps)
+    b.launch()          # warm: one whole run
+    b.sync()
+    b.reset()
+    field_hip.sync()
+    ev = field_hip.Events()
+    ev.begin()
+    b.launch()
+    run_ms = ev.end()
+    prices = b.prices()
+    out = {"options": n, "steps": steps, "cells": cells, "scheme": "Craig-Sneyd theta 1/2, fp32, delta form",
+           "run_ms": run_ms, "solves_per_s": n / (run_ms * 1e-3), "cell_updates_per_s": cells * steps / (run_ms * 1e-3),
+           "achieved_gbs_8B": 8.0 * cells * steps / (run_ms * 1e6), "launches_in_graph": 4 * 3 * steps,
+           "tuned": tuned, "tune_s": tune_s}
+    out["frac_hbm_8B"] = out["achieved_gbs_8B"] / HBM_PEAK_GBS
+    sample = np.concatenate([np.flatnonzero(nx == s[0])[:sample_per_class] for s in ADI_SHAPES])
+    if sample.size:
+        from oracle import heston
+        t = time.time()
+        ref, _, _ = heston.batch64(nx[sample], ny[sample], opts[sample], steps)
+        fp64_s = time.time() - t
+        err = np.abs(prices[sample] - ref)
+        out["error_vs_fp64"] = {"options": int(sample.size), "max_abs": float(err.max()),
+                                "max_rel": float(np.max(err / np.maximum(np.abs(ref), 1e-3))), "rms_abs": float(np.sqrt(np.mean(err ** 2))),
+                                "oracle_s": fp64_s}
+        if cpu:
+            cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            t = time.time()
+            heston.batch32(nx[sample], ny[sample], opts[sample], steps)
+            wall = time.time() - t
+            cpu_rate = sample.size / wall
+            out["cpu_baseline"] = {"value": cpu_rate, "unit": "solves/s", "cores": cores, "kind": "port",
+                                   "sample": f"oracle FP32 restatement of the same scheme (OpenMP over options), {sample.size} "
+                                             f"options ({sample_per_class} of each lattice shape), {steps} steps, {wall:.1f} s"}
+            out["speedup_vs_cpu"] = out["solves_per_s"] / cpu_rate
+    return out
+
+
 def measured_traffic(kernel: str, n: int):
     """HBM bytes a launch of the engine's gather moves, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     (scaled by a copy kernel's; profiles/conduction_256_512_rocprof.md), when they were taken on this kernel and size."""
@@ -192,6 +255,10 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true", help="skip the 512^3 case")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adi", action="store_true", help="skip the batched Heston ADI case")
+    ap.add_argument("--adi-options", type=int, default=2048)
+    ap.add_argument("--adi-steps", type=int, default=512)
+    ap.add_argument("--adi-sample", type=int, default=4, help="options of each shape checked against FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
@@ -207,6 +274,10 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_baseline(args.size, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {json.dumps(cpu)}")
+    adi = None
+    if not args.no_adi:
+        adi = heston_adi(args.adi_options, args.adi_steps, args.adi_sample, not args.no_cpu)
+        log(f"heston adi: {json.dumps(adi)}")
 
     achieved = BYTES_PER_CELL * main_case["cells"] / (main_case["engine_kernel_ms"] * 1e6)
     line = {
@@ -228,7 +299,7 @@ def main() -> int:
                    "handwritten": {k: main_case[k] for k in ("hand_kernel_ms", "hand_gbs", "hand_ms_per_step")},
                    "engine_kernel": main_case["engine_kernel"], "engine_kernel_ms": main_case["engine_kernel_ms"],
                    "engine_ms_per_step_contracts_on": main_case["engine_ms_per_step_contracts_on"],
-                   "copy_roof_gbs": main_case["copy_gbs"], "large_512": large},
+                   "copy_roof_gbs": main_case["copy_gbs"], "large_512": large, "heston_adi": adi},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(main_case["engine_kernel"], args.size)},

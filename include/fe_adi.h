@@ -47,6 +47,10 @@ int fe_heston_lattice(fe_heston* b, int option, float* out);
 int fe_heston_classes(fe_heston* b);
 int fe_heston_get_shape(fe_heston* b, int cls, int* nx, int* lanes, int* row_waves, int* col_threads);
 int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col_threads);
+/* The autotuner: for each class, times `reps` steps of that class with every row-sweep split and block, keeps the
+ * fastest, then every column-sweep block; the choice as JSON in `report`. Leaves every lattice at its payoff and drops
+ * a captured graph. */
+int fe_heston_tune(fe_heston* b, int reps, char* report, size_t report_bytes);
 
 #ifdef __cplusplus
 }

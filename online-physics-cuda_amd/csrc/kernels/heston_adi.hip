@@ -62,18 +62,19 @@ struct Row {
     float v[K];
 };
 
-// Lane p of a group of L: the value of lane p - 1 (its own at p = 0) and of p + 1 (its own at p = L - 1).
+// Lane p of a group of L: `send` of lane p - 1 (of p + 1), or `edge` where the row ends there (the edge cell
+// repeating past the end).
 template <int L>
-__device__ __forceinline__ float from_left(float v, int p)
+__device__ __forceinline__ float from_left(float send, float edge, int p)
 {
-    const float o = __shfl_up(v, 1, L);
-    return p == 0 ? v : o;
+    const float o = __shfl_up(send, 1, L);
+    return p == 0 ? edge : o;
 }
 template <int L>
-__device__ __forceinline__ float from_right(float v, int p)
+__device__ __forceinline__ float from_right(float send, float edge, int p)
 {
-    const float o = __shfl_down(v, 1, L);
-    return p == L - 1 ? v : o;
+    const float o = __shfl_down(send, 1, L);
+    return p == L - 1 ? edge : o;
 }
 
 // F restricted to what `mode` needs, at the K cells of a lane in row j: dt F(W) or dt/2 A0 W, from rows jm, j, jp of W.
@@ -88,10 +89,10 @@ __device__ __forceinline__ void apply(const Row<K, L>& wm, const Row<K, L>& w, c
     for (int q = 0; q < K; ++q) {
         gy[q] = (wp.v[q] - wm.v[q]) * iy;
     }
-    const float gy_left = from_left<L>(gy[K - 1], p);
-    const float gy_right = from_right<L>(gy[0], p);
-    const float w_left = from_left<L>(w.v[K - 1], p);
-    const float w_right = from_right<L>(w.v[0], p);
+    const float gy_left = from_left<L>(gy[K - 1], gy[0], p);
+    const float gy_right = from_right<L>(gy[0], gy[K - 1], p);
+    const float w_left = from_left<L>(w.v[K - 1], w.v[0], p);
+    const float w_right = from_right<L>(w.v[0], w.v[K - 1], p);
     const float ih = 1.0f / h, ih2 = 0.5f / h;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -213,8 +214,7 @@ __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float
         de = nde;
     }
     const float u = de / be;
-    const float um = from_left<L>(u, p);
-    const float ul = p == 0 ? 0.0f : um;
+    const float ul = from_left<L>(u, 0.0f, p); // lane 0 has no unknown before it (its R terms are 0)
     d[K - 1] = u;
 #pragma unroll
     for (int q = 0; q < K - 1; ++q) {
@@ -472,12 +472,19 @@ struct fe_heston {
         c.n_col_tasks = static_cast<int>(ct.size());
     }
 
-    void queue_step()
+    void queue_step() { queue_step_of(-1); }
+
+    // One step of every class (cls < 0) or of one class only (the tuner's measurements).
+    void queue_step_of(int cls)
     {
         const hipStream_t s = stream();
         const int second = scheme == 0 ? kCsSecond : kHvSecond;
         for (int pass = 0; pass < 4; ++pass) {
-            for (Class& c : classes) {
+            for (std::size_t ci = 0; ci < classes.size(); ++ci) {
+                if (cls >= 0 && static_cast<int>(ci) != cls) {
+                    continue;
+                }
+                Class& c = classes[ci];
                 if (pass == 0 || pass == 2) {
                     RowKernel k = pick_row_kernel(c.nx / c.lanes, c.lanes, pass == 0 ? kFirst : second);
                     // pass 3 reads D2 rows for the mixed term (CS) or U and D2 (HV)
@@ -494,6 +501,77 @@ struct fe_heston {
             }
         }
         hcheck(hipGetLastError(), "launch");
+    }
+
+    // Card time (ms) of `reps` steps of one class with its current shape, after one untimed step.
+    float time_class(int cls, int reps)
+    {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hcheck(hipEventCreate(&e0), "event");
+        hcheck(hipEventCreate(&e1), "event");
+        queue_step_of(cls);
+        hcheck(hipEventRecord(e0, stream()), "event");
+        for (int r = 0; r < reps; ++r) {
+            queue_step_of(cls);
+        }
+        hcheck(hipEventRecord(e1, stream()), "event");
+        hcheck(hipEventSynchronize(e1), "event");
+        float ms = 0.0f;
+        hcheck(hipEventElapsedTime(&ms, e0, e1), "event");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms;
+    }
+
+    // For each class: the row sweeps' split (lanes a row: Thomas cells a lane against PCR depth) and waves a block,
+    // then the column sweeps' block size, each the fastest of its candidates on this batch; then every lattice back
+    // to its payoff (the measurements stepped it).
+    std::string tune(int reps)
+    {
+        std::ostringstream log;
+        log << "[";
+        for (std::size_t ci = 0; ci < classes.size(); ++ci) {
+            Class& c = classes[ci];
+            float best = 1e30f;
+            int bl = c.lanes, bw = c.row_waves, bc = c.col_threads;
+            for (const int lanes : {16, 32, 64}) {
+                const int k = c.nx / lanes;
+                if (c.nx % lanes != 0 || !(k == 2 || k == 4 || k == 8 || k == 16)) {
+                    continue;
+                }
+                for (const int waves : {1, 2, 4}) {
+                    c.lanes = lanes;
+                    c.row_waves = waves;
+                    build_tasks(c);
+                    const float ms = time_class(static_cast<int>(ci), reps);
+                    if (ms < best) {
+                        best = ms;
+                        bl = lanes;
+                        bw = waves;
+                    }
+                }
+            }
+            c.lanes = bl;
+            c.row_waves = bw;
+            best = 1e30f;
+            for (const int threads : {64, 128, 256}) {
+                c.col_threads = threads;
+                build_tasks(c);
+                const float ms = time_class(static_cast<int>(ci), reps);
+                if (ms < best) {
+                    best = ms;
+                    bc = threads;
+                }
+            }
+            c.col_threads = bc;
+            build_tasks(c);
+            log << (ci ? "," : "") << "{\"nx\":" << c.nx << ",\"options\":" << c.members.size() << ",\"lanes\":" << c.lanes
+                << ",\"row_waves\":" << c.row_waves << ",\"col_threads\":" << c.col_threads << ",\"ms_a_step\":" << best / reps << "}";
+        }
+        log << "]";
+        reset();
+        hcheck(hipStreamSynchronize(stream()), "sync");
+        return log.str();
     }
 
     void reset()
@@ -782,6 +860,24 @@ int fe_heston_lattice(fe_heston* b, int option, float* out)
         return fail(b, hipGetErrorString(e));
     }
     return hipStreamSynchronize(b->stream()) == hipSuccess ? 0 : fail(b, "sync");
+}
+
+int fe_heston_tune(fe_heston* b, int reps, char* report, size_t report_bytes)
+{
+    if (b == nullptr || reps <= 0) {
+        return -1;
+    }
+    try {
+        if (b->graph != nullptr) {
+            fe_hip_graph_destroy(b->graph);
+            b->graph = nullptr;
+        }
+        const std::string r = b->tune(reps);
+        set_why(report, report_bytes, r);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(b, e.what());
+    }
 }
 
 int fe_heston_classes(fe_heston* b) { return b != nullptr ? static_cast<int>(b->classes.size()) : 0; }

@@ -109,6 +109,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_heston_get_shape": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                         ctypes.POINTER(c_int)]),
         "fe_heston_set_shape": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+        "fe_heston_tune": (c_int, [c_void_p, c_int, ctypes.c_char_p, c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -482,3 +483,11 @@ class HestonBatch:
 
     def set_shape(self, cls: int, lanes: int, row_waves: int, col_threads: int) -> None:
         self._check(lib().fe_heston_set_shape(self.ptr, cls, lanes, row_waves, col_threads), "set_shape")
+
+    def tune(self, reps: int = 2):
+        """Picks each class's launch shapes by timing them (the lattices return to their payoffs)."""
+        import json
+
+        report = ctypes.create_string_buffer(8192)
+        self._check(lib().fe_heston_tune(self.ptr, reps, report, len(report)), "tune")
+        return json.loads(report.value.decode())

@@ -97,3 +97,18 @@ def test_refuses_lattices_it_cannot_run(device):
         field_hip.HestonBatch([96], [64], [[1.5, 0.04, 0.3, -0.9, 0.025, 100, 0.5, 0.04]], 8)
     with pytest.raises(field_hip.FieldHipError, match="positive"):
         field_hip.HestonBatch([128], [64], [[1.5, 0.04, 0.0, -0.9, 0.025, 100, 0.5, 0.04]], 8)
+
+
+def test_tuner_picks_valid_shapes_and_results_hold(device):
+    import field_hip
+
+    nx, ny, opts = draw(12, 14)
+    ref_prices, _, _ = heston.batch64(nx, ny, opts, 16)
+    b = field_hip.HestonBatch(nx, ny, opts, 16)
+    report = b.tune(1)
+    print(report, flush=True)
+    assert sorted(r["nx"] for r in report) == sorted(set(int(v) for v in nx))
+    for r in report:
+        assert r["lanes"] in (16, 32, 64) and r["row_waves"] in (1, 2, 4) and r["col_threads"] in (64, 128, 256)
+    b.step(16)
+    assert np.max(np.abs(b.prices() - ref_prices)) < PRICE_ATOL
