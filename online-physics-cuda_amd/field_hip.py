@@ -110,6 +110,20 @@ def _declare(lib: ctypes.CDLL) -> None:
                                         ctypes.POINTER(c_int)]),
         "fe_heston_set_shape": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
         "fe_heston_tune": (c_int, [c_void_p, c_int, ctypes.c_char_p, c_size_t]),
+        "fe_lnet_create": (c_void_p, [c_void_p, c_float, ctypes.c_char_p, c_size_t]),
+        "fe_lnet_free": (None, [c_void_p]),
+        "fe_lnet_error": (ctypes.c_char_p, [c_void_p]),
+        "fe_lnet_param_count": (c_int, []),
+        "fe_lnet_get_params": (c_int, [c_void_p, c_void_p]),
+        "fe_lnet_set_params": (c_int, [c_void_p, c_void_p]),
+        "fe_lnet_faces": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p]),
+        "fe_lnet_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double,
+                                 c_int, c_double, c_void_p, c_void_p]),
+        "fe_lnet_fallbacks": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint)]),
+        "fe_lnet_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
+                                     c_double, c_double, c_double, c_void_p, c_void_p]),
+        "fe_lnet_adam": (c_int, [c_void_p, c_double, c_double]),
+        "fe_lnet_coarsen": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -491,3 +505,76 @@ class HestonBatch:
         report = ctypes.create_string_buffer(8192)
         self._check(lib().fe_heston_tune(self.ptr, reps, report, len(report)), "tune")
         return json.loads(report.value.decode())
+
+
+class LearnedNet:
+    """The learned flux correction of a coarse 2-D two-field lattice (include/fe_learned.h). Device arrays in and out
+    (DeviceArray or raw device pointers); `scale` multiplies the net's outputs into face fluxes."""
+
+    def __init__(self, params, scale: float):
+        import numpy as np
+
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != lib().fe_lnet_param_count():
+            raise FieldHipError(f"{p.size} parameters given, the net has {lib().fe_lnet_param_count()}")
+        why = ctypes.create_string_buffer(1024)
+        self.ptr = lib().fe_lnet_create(p.ctypes.data, scale, why, len(why))
+        if not self.ptr:
+            raise FieldHipError("net refused: " + why.value.decode())
+        self.scale = scale
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fe_lnet_free(self.ptr)
+            self.ptr = None
+
+    def _check(self, status: int, what: str) -> None:
+        if status != 0:
+            raise FieldHipError(f"{what}: {lib().fe_lnet_error(self.ptr).decode()}")
+
+    @staticmethod
+    def _p(a):
+        return a.ptr if isinstance(a, DeviceArray) else a
+
+    def params(self):
+        import numpy as np
+
+        out = np.empty(lib().fe_lnet_param_count(), np.float32)
+        self._check(lib().fe_lnet_get_params(self.ptr, out.ctypes.data), "get_params")
+        return out
+
+    def set_params(self, p) -> None:
+        import numpy as np
+
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(lib().fe_lnet_set_params(self.ptr, p.ctypes.data), "set_params")
+
+    def faces(self, u, v, nx, ny, count, h, out) -> None:
+        self._check(lib().fe_lnet_faces(self.ptr, self._p(u), self._p(v), nx, ny, count, h, self._p(out)), "faces")
+
+    def step(self, u, v, nx, ny, count, h, du, dv, k, dt, u2, v2, learned=True, bound=1e3) -> None:
+        self._check(lib().fe_lnet_step(self.ptr, self._p(u), self._p(v), nx, ny, count, h, du, dv, k, dt, int(learned), bound,
+                                       self._p(u2), self._p(v2)), "step")
+
+    def fallbacks(self) -> int:
+        c = ctypes.c_uint()
+        self._check(lib().fe_lnet_fallbacks(self.ptr, ctypes.byref(c)), "fallbacks")
+        return int(c.value)
+
+    def gradient(self, u0, v0, u1, v1, nx, ny, count, h, du, dv, k, dt, want=True):
+        import numpy as np
+
+        g = np.empty(lib().fe_lnet_param_count(), np.float32) if want else None
+        loss = np.empty(1, np.float32) if want else None
+        self._check(lib().fe_lnet_gradient(self.ptr, self._p(u0), self._p(v0), self._p(u1), self._p(v1), nx, ny, count, h, du, dv,
+                                           k, dt, g.ctypes.data if want else None, loss.ctypes.data if want else None), "gradient")
+        return (float(loss[0]), g) if want else None
+
+    def adam(self, lr: float, clip: float = 1e-2) -> None:
+        self._check(lib().fe_lnet_adam(self.ptr, lr, clip), "adam")
+
+
+def coarsen(fine, fnx, fny, count, factor, coarse) -> None:
+    p = lambda a: a.ptr if isinstance(a, DeviceArray) else a  # noqa: E731
+    if lib().fe_lnet_coarsen(p(fine), fnx, fny, count, factor, p(coarse)) != 0:
+        raise FieldHipError("coarsen: invalid arguments")
