@@ -52,6 +52,10 @@ void fe_string_free(char* s);
 /* The generated gfx950 translation unit of a law (CudaSolver::source, src/field/cuda/CudaSolver.hpp:30; lawc --cuda,
  * tools/lawc/main.cpp:221-224); NULL with the reason in `why` when the law uses an op the backend does not run. */
 char* fe_law_hip_source(const fe_law* law, char* why, size_t why_bytes);
+/* The adjoint of a term (NULL or "": the first) as law text: one step of it is one vector-Jacobian product of one
+ * step of the term (ir::adjoint_law, src/field/ir/Adjoint.cpp:27-225; lawc --adjoint, tools/lawc/main.cpp:274-361);
+ * NULL with the reason in `why`. Free with fe_string_free. */
+char* fe_law_adjoint(const fe_law* law, const char* term, char* why, size_t why_bytes);
 
 /* ---- the solver on the card ----------------------------------------------------------------------------------------- */
 
@@ -87,6 +91,11 @@ uint64_t fe_solver_digest(fe_solver* s);
 const char* fe_solver_error(fe_solver* s);
 /* The device array holding a field's current cells (valid until the next step swaps it). */
 void* fe_solver_field_device(fe_solver* s, int field);
+/* A whole field from device memory (nx*ny*nz floats) into the solver's current cells on the card, queued on the
+ * library's stream; the host copy is fetched before it is next read. For callers that keep states on the card
+ * (checkpoints, adjoint sweeps); call it once the solver has uploaded its fields (a step or refresh_derived). No
+ * counterpart in the reference, whose states travel through the host (tools/lawc/main.cpp:310-323). */
+int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,8 @@
 #include "field/Solver.hpp"
 #include "field/capi_internal.hpp"
 #include "field/hip/HipSolver.hpp"
+#include "field/ir/Adjoint.hpp"
+#include "fe_hip.h"
 
 #include <cstdlib>
 #include <cstring>
@@ -16,6 +18,7 @@ using namespace fieldengine::field;
 
 namespace fieldengine::field {
 void* hip_backend_field_pointer(LawBackend& backend, int field);
+void hip_backend_device_written(LawBackend& backend, int field);
 }
 
 struct fe_solver {
@@ -269,6 +272,34 @@ void* fe_solver_field_device(fe_solver* s, int field)
         return nullptr;
     }
     return hip_backend_field_pointer(*s->solver->backend(), field);
+}
+
+int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
+{
+    if (!valid_field(s, field) || src == nullptr || s->solver->backend() == nullptr) {
+        return -1;
+    }
+    void* dst = hip_backend_field_pointer(*s->solver->backend(), field);
+    const std::size_t bytes = s->solver->fields_const_shape_cells() * sizeof(float);
+    const int r = fe_hip_memcpy_dtod_async(dst, src, bytes);
+    if (r != 0) {
+        s->error = std::string("device copy: ") + fe_hip_error_string(r);
+        return r;
+    }
+    hip_backend_device_written(*s->solver->backend(), field);
+    s->solver->mark_device_written(field);
+    return 0;
+}
+
+char* fe_law_adjoint(const fe_law* law, const char* term, char* why, size_t why_bytes)
+{
+    if (law == nullptr) {
+        return nullptr;
+    }
+    std::string reason;
+    const std::string text = ir::adjoint_law(law->law, term != nullptr ? term : "", reason);
+    put(why, why_bytes, reason);
+    return text.empty() ? nullptr : dup(text);
 }
 
 char* fe_law_hip_source(const fe_law* law, char* why, size_t why_bytes)

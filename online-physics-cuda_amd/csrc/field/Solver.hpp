@@ -35,6 +35,9 @@ public:
     bool use_gpu(std::string& why);
     [[nodiscard]] bool on_gpu() const { return gpu_ != nullptr; }
     [[nodiscard]] LawBackend* backend() const { return gpu_.get(); }
+    // A field's cells were written on the card by the caller: the host copy is fetched before it is next read.
+    void mark_device_written(int field) { stale_[static_cast<std::size_t>(field)] = 1; }
+    [[nodiscard]] std::size_t fields_const_shape_cells() const { return fields_.shape().cells(); }
 
     void refresh_derived();
     void step(double dt);

@@ -892,6 +892,8 @@ struct Impl final : LawBackend {
 
     // Device pointers of a field's current cells (for callers that keep data on the card).
     [[nodiscard]] void* field_pointer(int f) const { return now[static_cast<std::size_t>(f)]; }
+    // A caller wrote field f's cells on the card: the host copy is stale until fetched.
+    void device_written(int f) { stale[static_cast<std::size_t>(f)] = 1; }
 };
 
 } // namespace
@@ -948,6 +950,11 @@ std::string HipSolver::source(const Law& law, std::string& why)
 void* hip_backend_field_pointer(LawBackend& backend, int field)
 {
     return static_cast<Impl&>(backend).field_pointer(field);
+}
+
+void hip_backend_device_written(LawBackend& backend, int field)
+{
+    static_cast<Impl&>(backend).device_written(field);
 }
 
 } // namespace fieldengine::field

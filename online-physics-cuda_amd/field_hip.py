@@ -93,6 +93,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_digest": (ctypes.c_uint64, [c_void_p]),
         "fe_solver_error": (ctypes.c_char_p, [c_void_p]),
         "fe_solver_field_device": (c_void_p, [c_void_p, c_int]),
+        "fe_solver_set_field_device": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_law_adjoint": (c_void_p, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, c_size_t]),
         "fe_heston_create": (c_void_p, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_char_p,
                                         c_size_t]),
         "fe_heston_free": (None, [c_void_p]),
@@ -332,6 +334,14 @@ class Law:
             raise FieldHipError("no HIP source: " + why.value.decode())
         return src
 
+    def adjoint(self, term: str = "") -> str:
+        """The adjoint law of a term (its text): a step of it is one vector-Jacobian product of one step of the term."""
+        why = ctypes.create_string_buffer(4096)
+        text = _take_string(lib().fe_law_adjoint(self.ptr, term.encode(), why, len(why)))
+        if text is None:
+            raise FieldHipError("no adjoint: " + why.value.decode())
+        return text
+
 
 class Solver:
     """A law integrating on the card: the reference's Solver calls over the HIP backend (no CPU path)."""
@@ -423,6 +433,11 @@ class Solver:
 
     def field_device(self, field) -> int:
         return lib().fe_solver_field_device(self.ptr, self._field(field))
+
+    def set_field_device(self, field, src) -> None:
+        """Copies a whole field from device memory (a DeviceArray or a device pointer) into the solver on the card."""
+        ptr = src.ptr if isinstance(src, DeviceArray) else src
+        self._check(lib().fe_solver_set_field_device(self.ptr, self._field(field), ptr), "set_field_device")
 
 
 class HestonBatch:
