@@ -149,8 +149,9 @@ def heston_options(n: int, seed: int = 2026):
 
 
 def heston_adi(n: int, steps: int, sample_per_class: int, cpu: bool):
-    """The batch on the card (Craig-Sneyd, theta 1/2, fp32): tuned, every step captured in one graph, one launch timed;
-    prices of a sample (the same mix) against the FP64 oracle; the FP32 oracle on the host's cores on that sample."""
+    """The batch on the card (Craig-Sneyd, theta 1/2, fp32): tuned, every step captured in one graph, one launch timed.
+    With `cpu`, a sample of the same mix is run by the oracle in FP64 (the error) and in FP32 on the host's cores (the
+    baseline)."""
     nx, ny, opts = heston_options(n)
     b = field_hip.HestonBatch(nx, ny, opts, steps, 0, 0.5)
     cells = b.cells
@@ -172,27 +173,304 @@ def heston_adi(n: int, steps: int, sample_per_class: int, cpu: bool):
            "achieved_gbs_8B": 8.0 * cells * steps / (run_ms * 1e6), "launches_in_graph": 4 * 3 * steps,
            "tuned": tuned, "tune_s": tune_s}
     out["frac_hbm_8B"] = out["achieved_gbs_8B"] / HBM_PEAK_GBS
-    sample = np.concatenate([np.flatnonzero(nx == s[0])[:sample_per_class] for s in ADI_SHAPES])
-    if sample.size:
-        from oracle import heston
-        t = time.time()
-        ref, _, _ = heston.batch64(nx[sample], ny[sample], opts[sample], steps)
-        fp64_s = time.time() - t
-        err = np.abs(prices[sample] - ref)
+    if cpu:
+        sample = np.concatenate([np.flatnonzero(nx == s[0])[:sample_per_class] for s in ADI_SHAPES])
+        base = cpu_baseline_adi(nx[sample], ny[sample], opts[sample], steps)
+        err = np.abs(prices[sample] - base["fp64_prices"])
         out["error_vs_fp64"] = {"options": int(sample.size), "max_abs": float(err.max()),
-                                "max_rel": float(np.max(err / np.maximum(np.abs(ref), 1e-3))), "rms_abs": float(np.sqrt(np.mean(err ** 2))),
-                                "oracle_s": fp64_s}
-        if cpu:
-            cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-            t = time.time()
-            heston.batch32(nx[sample], ny[sample], opts[sample], steps)
-            wall = time.time() - t
-            cpu_rate = sample.size / wall
-            out["cpu_baseline"] = {"value": cpu_rate, "unit": "solves/s", "cores": cores, "kind": "port",
-                                   "sample": f"oracle FP32 restatement of the same scheme (OpenMP over options), {sample.size} "
-                                             f"options ({sample_per_class} of each lattice shape), {steps} steps, {wall:.1f} s"}
-            out["speedup_vs_cpu"] = out["solves_per_s"] / cpu_rate
+                                "max_rel": float(np.max(err / np.maximum(np.abs(base["fp64_prices"]), 1e-3))),
+                                "rms_abs": float(np.sqrt(np.mean(err ** 2)))}
+        out["cpu_baseline"] = base["baseline"]
+        out["speedup_vs_cpu"] = out["solves_per_s"] / base["baseline"]["value"]
     return out
+
+
+def cpu_baseline_adi(nx, ny, opts, steps):
+    """The ORACLE's restatement of the same scheme on the host: FP64 (the prices the card's are judged against) and
+    FP32 on all cores (the baseline), on a sample of the batch's mix."""
+    from oracle import heston
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ref, _, _ = heston.batch64(nx, ny, opts, steps)
+    t = time.time()
+    heston.batch32(nx, ny, opts, steps)
+    wall = time.time() - t
+    return {"fp64_prices": ref,
+            "baseline": {"value": len(nx) / wall, "unit": "solves/s", "cores": cores, "kind": "port",
+                         "sample": f"oracle FP32 restatement of the same scheme (OpenMP over options), {len(nx)} options of the "
+                                   f"batch's mix, {steps} steps, {wall:.1f} s"}}
+
+
+DIRICHLET_LAW = """field T init 0 range -1000000 1000000
+field nb init 0 range 0 6 static
+param k = 0.1
+term conduction
+  d T = diffuse(k, T) - 2 * k * nb * T / (h * h)
+  max_dt = 0.15 * h * h / k
+"""
+
+BLACK_SCHOLES_LAW = """field V init 0 range -1 1000000
+param sig = 0.2
+param r = 0.05
+term black_scholes
+  d V = 0.5 * sig * sig * (shift(V, 1, 0, 0) - 2 * V + shift(V, -1, 0, 0)) / (h * h) + (r - 0.5 * sig * sig) * gx(V) - r * V
+  max_dt = 0.4 * h * h / (sig * sig)
+"""
+
+
+def sine_start(n: int, noise: float, seed: int = 5):
+    """prod sin(pi x_i) at cell centres of [0, 1]^3, plus seeded uniform noise of amplitude `noise`; and the count of
+    each cell's boundary faces (zero Dirichlet: each such face holds a ghost of -T)."""
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    t = np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+    if noise:
+        t = t + noise * np.random.default_rng(seed).random(t.shape)
+    i = np.arange(n)
+    edge = ((i == 0) | (i == n - 1)).astype(np.float32)
+    nb = edge[None, None, :] + edge[None, :, None] + edge[:, None, None]
+    return t.astype(np.float32).reshape(-1), nb.astype(np.float32).reshape(-1)
+
+
+def dirichlet_heat(cpu: bool):
+    """Zero-Dirichlet 3-D conduction (k = 0.1, dt = 0.4 h^2/k, substeps from max_dt = 0.15 h^2/k): the error against the
+    decaying sine mode exp(-3 pi^2 k t) prod sin(pi x) and its order under refinement 16 -> 32 -> 64 at a fixed end
+    time (200 steps of 64^3), and the 64^3 run with noise: a fixed-order compensated total every step."""
+    k = 0.1
+    t_end = 200 * 0.4 * (1.0 / 64) ** 2 / k
+    errors = {}
+    for n in (16, 32, 64):
+        h = 1.0 / n
+        steps = int(np.ceil(t_end / (0.4 * h * h / k) - 1e-9))
+        t0, nb = sine_start(n, 0.0)
+        s = field_hip.Solver(field_hip.Law(DIRICHLET_LAW), (n, n, n), h)
+        s.set_field("T", t0)
+        s.set_field("nb", nb)
+        s.set_lazy_readback(True)
+        s.check_contracts_every(1 << 30)
+        for _ in range(steps):
+            s.step(t_end / steps)
+        exact = np.exp(-3 * np.pi ** 2 * k * t_end) * t0.astype(np.float64)
+        errors[n] = float(np.max(np.abs(s.get_field("T") - exact)))
+    orders = [float(np.log2(errors[16] / errors[32])), float(np.log2(errors[32] / errors[64]))]
+    n, h = 64, 1.0 / 64
+    dt = 0.4 * h * h / k
+    t0, nb = sine_start(n, 1e-3)
+    s = field_hip.Solver(field_hip.Law(DIRICHLET_LAW), (n, n, n), h)
+    s.set_field("T", t0)
+    s.set_field("nb", nb)
+    s.set_lazy_readback(True)
+    s.check_contracts_every(1 << 30)
+    totals = []
+    ev = field_hip.Events()
+    ev.begin()
+    for _ in range(200):
+        s.step(dt)
+        totals.append(s.total("T")[0])
+    ms = ev.end()
+    out = {"errors_max_vs_exact": errors, "orders": orders, "substeps_per_step": s.last_substeps(),
+           "steps": 200, "cells": n ** 3, "ms_per_step_with_total": ms / 200,
+           "total_first_last": [totals[0], totals[-1]], "totals_decreasing": bool(np.all(np.diff(totals) < 0))}
+    if cpu:
+        final = s.get_field("T")
+        ref = cpu_baseline_lawc(DIRICHLET_LAW, (n, n, n, h), {"T": t0, "nb": nb}, 200, dt, {}, "T")
+        out["equals_reference_cpu_bits"] = bool(np.array_equal(final.view(np.uint32), ref["final"].view(np.uint32)))
+        out["cpu_baseline"] = ref["baseline"]
+    return out
+
+
+def black_scholes(steps: int, cpu: bool):
+    """The Black-Scholes law in x = ln S on 4096 cells x 512 planes (every plane an option), sig 0.2, r 0.05, put K =
+    100 averaged over each cell, one substep a step at dt = max_dt = 0.4 h^2 / sig^2."""
+    nx, planes = 4096, 512
+    h = 8.0 / nx
+    x0 = np.log(100.0) - 4.0
+    edges = x0 + h * np.arange(nx + 1)
+    lo, hi = edges[:-1], edges[1:]
+    a = np.minimum(hi, np.log(100.0))
+    pay = np.where(a > lo, (100.0 * (a - lo) - (np.exp(a) - np.exp(lo))) / h, 0.0)
+    v0 = np.tile(pay, planes).astype(np.float32)
+    dt = 0.4 * h * h / 0.04
+    s = field_hip.Solver(field_hip.Law(BLACK_SCHOLES_LAW), (nx, 1, planes), h)
+    s.set_field("V", v0)
+    s.set_lazy_readback(True)
+    s.check_contracts_every(1 << 30)
+    for _ in range(10):
+        s.step(dt)
+    mid = s.get_field("V")
+    ev = field_hip.Events()
+    ev.begin()
+    for _ in range(steps):
+        s.step(dt)
+    ms = ev.end() / steps
+    cells = nx * planes
+    out = {"cells": cells, "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "gbs_8B": 8.0 * cells / (ms * 1e6),
+           "kernel": s.profile()["terms"][0]["kernel"]}
+    if cpu:
+        ref = cpu_baseline_lawc(BLACK_SCHOLES_LAW, (nx, 1, planes, h), {"V": v0}, 10, dt, {}, "V")
+        rel = np.max(np.abs(mid.astype(np.float64) - ref["final"]) / np.maximum(np.abs(ref["final"]), 1e-30))
+        out["max_rel_vs_reference_cpu_10_steps"] = float(rel)
+        out["equals_reference_cpu_bits"] = bool(np.array_equal(mid.view(np.uint32), ref["final"].view(np.uint32)))
+        out["cpu_baseline"] = ref["baseline"]
+    return out
+
+
+def cpu_baseline_lawc(law: str, grid, init: dict, steps: int, dt: float, params: dict, field: str):
+    """The reference's own CPU engine (oracle/_ref/lawc, built from its sources) on the same law, start and steps, on
+    this host's cores: its final field (what the card's is compared with) and its time a step."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    lawc = os.path.join(ROOT, "oracle", "_ref", "lawc")
+    if not os.path.exists(lawc):
+        return {"final": np.zeros(1, np.float32), "baseline": None}
+    nx, ny, nz, h = grid
+    tmp = tempfile.mkdtemp()
+    try:
+        path = os.path.join(tmp, "law.law")
+        open(path, "w").write(law)
+        init_dir = os.path.join(tmp, "init")
+        os.makedirs(init_dir)
+        for name, values in init.items():
+            np.asarray(values, np.float32).tofile(os.path.join(init_dir, name + ".bin"))
+        out_dir, js = os.path.join(tmp, "out"), os.path.join(tmp, "t.json")
+        cmd = [lawc, "--grid", str(nx), str(ny), str(nz), repr(float(h)), "--init", init_dir, "--steps", str(steps), "--dt",
+               repr(float(dt)), "--dump", out_dir, "--json", js, "--warmup", "1"]
+        for name, value in params.items():
+            cmd += ["--param", name, repr(float(value))]
+        cmd.append(path)
+        t = time.time()
+        subprocess.run(cmd, check=True, capture_output=True, env=dict(os.environ, OMP_NUM_THREADS=str(cores)), timeout=600)
+        wall = time.time() - t
+        final = np.fromfile(os.path.join(out_dir, f"step_{steps}", field + ".bin"), dtype=np.float32)
+        ms = json.load(open(js))["median_ms"]
+        cells = nx * ny * nz
+        return {"final": final,
+                "baseline": {"value": cells / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "reference",
+                             "sample": f"reference lawc, OpenMP {cores} threads, {nx}x{ny}x{nz}, {steps} steps, median "
+                                       f"{ms:.3f} ms/step ({wall:.1f} s wall)"}}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def learned_coarse(cpu: bool, train_steps: int, test_seeds: int):
+    """The learned coarse lattice (diffusion-reaction, Du 1e-3, Dv 5e-3, k 5e-3): 256 training seeds of the fine 512^2
+    law run by the engine, 100 snapshots each coarsened to 128^2 on the card; the net trained on the card; then on
+    unseen seeds the coarse law with and without the net, and the plain law at 256^2 and 512^2, all against the fine
+    law in FP64 (the oracle), each with its card time for the same simulated time."""
+    import learned_coarse as lc
+    from oracle import learned as L
+    s = lc.Setup()
+    stats = {}
+    data = lc.generate(s, range(256 + test_seeds), base_seed=7000, stats=stats)
+    scale = lc.target_scale(data)
+    net = field_hip.LearnedNet(L.init_params(1, last_scale=0.0), scale)
+    t = time.time()
+    losses = lc.train(net, data, train_steps, batch=16, lr=3e-3, train_seeds=256, log_every=max(1, train_steps // 5))
+    train_s = time.time() - t
+    test = slice(256, 256 + test_seeds)
+    snaps = s.snapshots
+    t_end = snaps * s.dt_coarse
+    times = {}
+    tl, tp = [], []
+    lu, lv = lc.rollout(net, data, test, snaps, learned=True, times=tl)
+    pu, pv = lc.rollout(net, data, test, snaps, learned=False, times=tp)
+    times["learned_128"], times["plain_128"] = tl[-1], tp[-1]
+    u0, v0 = lc.initial(s, range(256, 256 + test_seeds), base_seed=7000)
+    runs = {"learned_128": (lu, lv), "plain_128": (pu, pv)}
+    for n, sub in ((256, 3 * snaps), (512, s.every * snaps)):
+        f = n // s.coarse
+        tt = []
+        fu, fv = lc.plain_run(lc.upsample(u0, f).reshape(-1), lc.upsample(v0, f).reshape(-1), n, test_seeds, s, t_end, sub, tt)
+        times[f"plain_{n}"] = tt[-1]
+        runs[f"plain_{n}"] = (L.coarsen(fu, n, n, test_seeds, f).astype(np.float32), L.coarsen(fv, n, n, test_seeds, f).astype(np.float32))
+    out = {"fine": s.fine, "coarse": s.coarse, "train_seeds": 256, "test_seeds": test_seeds, "snapshots": snaps,
+           "data": stats, "train_steps": train_steps, "train_s": train_s, "loss_log": losses, "scale": scale,
+           "card_ms_for_t_end": times, "fallbacks": net.fallbacks()}
+    if cpu:
+        truth = cpu_baseline_learned(s, u0, v0, test_seeds, t_end)
+        errs = {}
+        for name, (ru, rv) in runs.items():
+            errs[name] = float(np.sqrt(np.mean((ru - truth["u"]) ** 2) + np.mean((rv - truth["v"]) ** 2)))
+        out["rms_error_vs_fp64_fine"] = errs
+        out["cpu_baseline"] = truth["baseline"]
+        # equal error: the cheapest plain resolution whose error is no larger than the learned lattice's
+        fit = [n for n in (128, 256, 512) if errs[f"plain_{n}"] <= errs["learned_128"]]
+        if fit:
+            n = fit[0]
+            out["equal_error"] = {"plain": n, "speedup_learned_vs_plain": times[f"plain_{n}"] / times["learned_128"]}
+        out["speedup_learned_128_vs_plain_512"] = times["plain_512"] / times["learned_128"]
+    return out
+
+
+def cpu_baseline_learned(s, u0, v0, count: int, t_end: float):
+    """The fine law in FP64 on the host (the ORACLE's restatement, OpenMP): the truth the coarse lattices are judged
+    against, and its time as the CPU baseline."""
+    import learned_coarse as lc
+    from oracle import diffreact, learned as L
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    steps = s.every * s.snapshots
+    t = time.time()
+    U, V = diffreact.run64(lc.upsample(u0, s.factor), lc.upsample(v0, s.factor), s.fine, s.fine, count, s.h_fine, s.du, s.dv,
+                           s.k, t_end / steps, steps)
+    wall = time.time() - t
+    return {"u": L.coarsen(U, s.fine, s.fine, count, s.factor), "v": L.coarsen(V, s.fine, s.fine, count, s.factor),
+            "baseline": {"value": 2 * s.fine ** 2 * count * steps / wall, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+                         "sample": f"oracle FP64 fine law, {count} seeds of {s.fine}^2, {steps} steps ({wall:.1f} s)"}}
+
+
+TRANSPORT_LAW = """field a init 0
+field b init 0
+field c init 0
+param Da = 0.05
+param Db = 0.02
+param Dc = 0.01
+term transport
+  d a = diffuse(Da, a) - 0.3 * (a - shift(a, -1, 0, 0)) / h + 0.2 * (shift(a, 0, 1, 0) - a) / h - 0.1 * (a - shift(a, 0, 0, -1)) / h - a * b + 0.1 * c
+  d b = diffuse(Db, b) - 0.3 * (b - shift(b, -1, 0, 0)) / h + 0.2 * (shift(b, 0, 1, 0) - b) / h - 0.1 * (b - shift(b, 0, 0, -1)) / h - a * b + 0.1 * c
+  d c = diffuse(Dc, c) - 0.3 * (c - shift(c, -1, 0, 0)) / h + 0.2 * (shift(c, 0, 1, 0) - c) / h - 0.1 * (c - shift(c, 0, 0, -1)) / h + a * b - 0.1 * c
+"""
+
+
+def adjoint_transport(n: int, cpu: bool):
+    """The gradient of L = sum c^2 after 64 steps of the three-field transport law on n^3, with respect to the start and
+    the three diffusivities: the generated adjoint law, checkpoints every 8 steps, recomputation, on the card."""
+    import adjoint
+    h = 1.0 / n
+    dt = 0.8 * h * h / (6 * 0.05)
+    rng = np.random.default_rng(11)
+    init = {f: rng.uniform(0.2, 1.0, n ** 3).astype(np.float32) for f in ("a", "b", "c")}
+    res = adjoint.vjp(TRANSPORT_LAW, (n, n, n), h, {"Da": 0.05, "Db": 0.02, "Dc": 0.01}, init, dt, 64, "c", every=8)
+    cells = n ** 3
+    # a backward step: the adjoint law reads 3 primal and 6 adjoint fields and writes 6 (60 B a cell); the recomputed
+    # forward steps (56 of them) read and write 3 fields (24 B a cell)
+    bytes_back = (64 * 60 + 56 * 24) * cells
+    out = {"n": n, "steps": 64, "every": 8, "loss": res.loss, "param_grads": res.param_grads, "forward_ms": res.forward_ms,
+           "backward_ms": res.backward_ms, "backward_cell_updates_per_s": 64 * cells / (res.backward_ms * 1e-3),
+           "backward_gbs": bytes_back / (res.backward_ms * 1e6), **res.stats}
+    out["backward_frac_hbm"] = out["backward_gbs"] / HBM_PEAK_GBS
+    if cpu:
+        out["fd_check"] = cpu_baseline_adjoint()
+    return out
+
+
+def cpu_baseline_adjoint():
+    """Central differences of the ORACLE's FP64 run on a small lattice against the card's checkpointed adjoint."""
+    import adjoint
+    from oracle import transport
+    shape, h, dt, steps = (10, 9, 8), 0.125, 0.01, 12
+    D = [0.05, 0.02, 0.01]
+    rng = np.random.default_rng(3)
+    init = {f: rng.uniform(0.2, 1.0, int(np.prod(shape))) for f in ("a", "b", "c")}
+    res = adjoint.vjp(transport.LAW, shape, h, {"Da": D[0], "Db": D[1], "Dc": D[2]}, init, dt, steps, "c", every=4)
+    worst = 0.0
+    for i, name in enumerate(("Da", "Db", "Dc")):
+        e = 1e-6 * D[i]
+        up, dn = list(D), list(D)
+        up[i] += e
+        dn[i] -= e
+        fd = (transport.loss(shape, h, dt, steps, up, init["a"], init["b"], init["c"])
+              - transport.loss(shape, h, dt, steps, dn, init["a"], init["b"], init["c"])) / (2 * e)
+        worst = max(worst, abs(res.param_grads[name] - fd) / abs(fd))
+    return {"lattice": list(shape), "steps": steps, "every": 4, "max_rel_error_param_grads": worst}
 
 
 def measured_traffic(kernel: str, n: int):
@@ -259,6 +537,9 @@ def main() -> int:
     ap.add_argument("--adi-options", type=int, default=2048)
     ap.add_argument("--adi-steps", type=int, default=512)
     ap.add_argument("--adi-sample", type=int, default=4, help="options of each shape checked against FP64")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Dirichlet, Black-Scholes, learned and adjoint cases")
+    ap.add_argument("--learned-steps", type=int, default=3000)
+    ap.add_argument("--adjoint-size", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
@@ -278,6 +559,15 @@ def main() -> int:
     if not args.no_adi:
         adi = heston_adi(args.adi_options, args.adi_steps, args.adi_sample, not args.no_cpu)
         log(f"heston adi: {json.dumps(adi)}")
+    extra = {}
+    if not args.no_extra:
+        for name, fn in (("dirichlet_heat_64", lambda: dirichlet_heat(not args.no_cpu)),
+                         ("black_scholes_4096x512", lambda: black_scholes(200, not args.no_cpu)),
+                         ("learned_coarse", lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4)),
+                         ("adjoint_transport", lambda: adjoint_transport(args.adjoint_size, not args.no_cpu))):
+            t = time.time()
+            extra[name] = fn()
+            log(f"{name} ({time.time() - t:.1f} s): {json.dumps(extra[name], default=float)}")
 
     achieved = BYTES_PER_CELL * main_case["cells"] / (main_case["engine_kernel_ms"] * 1e6)
     line = {
@@ -299,13 +589,13 @@ def main() -> int:
                    "handwritten": {k: main_case[k] for k in ("hand_kernel_ms", "hand_gbs", "hand_ms_per_step")},
                    "engine_kernel": main_case["engine_kernel"], "engine_kernel_ms": main_case["engine_kernel_ms"],
                    "engine_ms_per_step_contracts_on": main_case["engine_ms_per_step_contracts_on"],
-                   "copy_roof_gbs": main_case["copy_gbs"], "large_512": large, "heston_adi": adi},
+                   "copy_roof_gbs": main_case["copy_gbs"], "large_512": large, "heston_adi": adi, **extra},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(main_case["engine_kernel"], args.size)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=float), flush=True)
     return 0
 
 

@@ -96,6 +96,11 @@ void* fe_solver_field_device(fe_solver* s, int field);
  * (checkpoints, adjoint sweeps); call it once the solver has uploaded its fields (a step or refresh_derived). No
  * counterpart in the reference, whose states travel through the host (tools/lawc/main.cpp:310-323). */
 int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
+/* A field's total and total magnitude as the contracts compute them on the card: fixed-order compensated sums in
+ * double (a block a fixed range of cells, the blocks folded in order), so two runs give the same bits; waits for the
+ * card. The reference reports the totals its conservation judge reads (src/field/Solver.cpp:31-39,
+ * src/field/cuda/CudaSolver.cpp:515-571). magnitude may be NULL. */
+int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude);
 
 #ifdef __cplusplus
 }

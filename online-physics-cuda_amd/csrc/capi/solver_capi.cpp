@@ -19,6 +19,7 @@ using namespace fieldengine::field;
 namespace fieldengine::field {
 void* hip_backend_field_pointer(LawBackend& backend, int field);
 void hip_backend_device_written(LawBackend& backend, int field);
+void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag);
 }
 
 struct fe_solver {
@@ -289,6 +290,20 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
     hip_backend_device_written(*s->solver->backend(), field);
     s->solver->mark_device_written(field);
     return 0;
+}
+
+int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude)
+{
+    if (!valid_field(s, field) || sum == nullptr || s->solver->backend() == nullptr) {
+        return -1;
+    }
+    return guarded(s, [&] {
+        double m = 0.0;
+        hip_backend_total(*s->solver->backend(), field, *sum, m);
+        if (magnitude != nullptr) {
+            *magnitude = m;
+        }
+    });
 }
 
 char* fe_law_adjoint(const fe_law* law, const char* term, char* why, size_t why_bytes)

@@ -894,6 +894,18 @@ struct Impl final : LawBackend {
     [[nodiscard]] void* field_pointer(int f) const { return now[static_cast<std::size_t>(f)]; }
     // A caller wrote field f's cells on the card: the host copy is stale until fetched.
     void device_written(int f) { stale[static_cast<std::size_t>(f)] = 1; }
+
+    // The field's fixed-order compensated total and total magnitude, from the card (waits for it).
+    void field_total(int f, double& sum, double& mag)
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        total(f, kTotals - 2);
+        double out[2] = {0.0, 0.0};
+        check(fe_hip_memcpy_dtoh_async(out, totals.as<double>() + (kTotals - 2), sizeof(out)), "totals readback");
+        check(fe_hip_stream_sync(), "sync");
+        sum = out[0];
+        mag = out[1];
+    }
 };
 
 } // namespace
@@ -955,6 +967,11 @@ void* hip_backend_field_pointer(LawBackend& backend, int field)
 void hip_backend_device_written(LawBackend& backend, int field)
 {
     static_cast<Impl&>(backend).device_written(field);
+}
+
+void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag)
+{
+    static_cast<Impl&>(backend).field_total(field, sum, mag);
 }
 
 } // namespace fieldengine::field

@@ -94,6 +94,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_error": (ctypes.c_char_p, [c_void_p]),
         "fe_solver_field_device": (c_void_p, [c_void_p, c_int]),
         "fe_solver_set_field_device": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_solver_total": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
         "fe_law_adjoint": (c_void_p, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, c_size_t]),
         "fe_heston_create": (c_void_p, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_char_p,
                                         c_size_t]),
@@ -433,6 +434,12 @@ class Solver:
 
     def field_device(self, field) -> int:
         return lib().fe_solver_field_device(self.ptr, self._field(field))
+
+    def total(self, field):
+        """(total, total magnitude) of a field, fixed-order compensated on the card."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().fe_solver_total(self.ptr, self._field(field), ctypes.byref(a), ctypes.byref(b)), "total")
+        return a.value, b.value
 
     def set_field_device(self, field, src) -> None:
         """Copies a whole field from device memory (a DeviceArray or a device pointer) into the solver on the card."""
