@@ -741,7 +741,8 @@ private:
     }
 
     // How far from its cell the gather reads each field through what it inlines (stages are read where they lie).
-    // False when a read's reach is not bounded by the instructions' radii (the transposes walk runs of cells).
+    // Every op's reads lie within its radius: a transpose's runs at the lattice's edges stay within its offset
+    // (shift_t) or one cell (gx_t, gy_t, gz_t) of the cell, at in-range coordinates.
     bool field_reach(const ir::Program& p, std::map<int, int>& out, int& cost) const
     {
         bool bounded = true;
@@ -761,9 +762,6 @@ private:
                 int& r = out[in.index];
                 r = std::max(r, off);
                 return;
-            }
-            if (in.op == Op::ShiftT || in.op == Op::GxT || in.op == Op::GyT || in.op == Op::GzT) {
-                bounded = false;
             }
             for (const int a : in.args) {
                 walk(a, off + in.radius, false);
