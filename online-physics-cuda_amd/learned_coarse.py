@@ -178,6 +178,21 @@ def train(net: field_hip.LearnedNet, data: Dataset, steps: int, batch: int, lr: 
     return losses
 
 
+def eval_loss(net: field_hip.LearnedNet, data: Dataset, seeds: slice, every: int = 5) -> float:
+    """The training loss averaged over snapshot pairs t = 0, every, 2 every, ... of `seeds` (a fixed set: losses
+    before and after training compare)."""
+    s = data.setup
+    c = s.coarse
+    count = seeds.stop - seeds.start
+    total, n = 0.0, 0
+    for t in range(0, s.snapshots, every):
+        loss, _ = net.gradient(data.at("u", t, seeds.start), data.at("v", t, seeds.start), data.at("u", t + 1, seeds.start),
+                               data.at("v", t + 1, seeds.start), c, c, count, s.h_coarse, s.du, s.dv, s.k, s.dt_coarse)
+        total += loss
+        n += 1
+    return total / n
+
+
 def rollout(net, data: Dataset, seeds: slice, steps: int, learned: bool, times=None):
     """The coarse law (with the net when `learned`) from snapshot 0 of `seeds`, `steps` snapshots; returns the final
     u, v (host) and, when `times` is a list, appends the card time of the run (ms)."""
