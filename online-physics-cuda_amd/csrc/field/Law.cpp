@@ -1084,10 +1084,15 @@ bool load_net(NetDecl& net, std::string& why)
         why = net.file + ": " + std::to_string(weights.size()) + " weights, the net has " + std::to_string(net.parameter_count());
         return false;
     }
-    net.act = act;
+    if (act != net.act) {
+        why = net.file + ": trained with another activation than the law declares"; // src/field/nn/Net.cpp:324-327
+        return false;
+    }
     net.weights = std::move(weights);
-    net.in_shift = shift.empty() ? std::vector<float>(static_cast<std::size_t>(net.inputs()), 0.0f) : shift;
-    net.in_scale = scale.empty() ? std::vector<float>(static_cast<std::size_t>(net.inputs()), 1.0f) : scale;
+    shift.resize(static_cast<std::size_t>(net.inputs()), 0.0f);
+    scale.resize(static_cast<std::size_t>(net.inputs()), 1.0f);
+    net.in_shift = std::move(shift);
+    net.in_scale = std::move(scale);
     net.out_scale = out_scale;
     net.loaded = true;
     return true;

@@ -198,7 +198,7 @@ public:
         for (std::size_t id = 0; id < m_.insts.size(); ++id) {
             const Op op = m_.insts[id].op;
             if (op == Op::Distance || op == Op::Poisson || op == Op::Fall || op == Op::Spread || op == Op::Topple ||
-                op == Op::Mlp || op == Op::Flux || op == Op::LDiffuse || op == Op::Local) {
+                op == Op::Local) {
                 if (used(static_cast<int>(id))) {
                     why = std::string("the HIP backend does not run ") + ir::op_name(op) + " yet";
                     return "";
@@ -215,10 +215,10 @@ public:
         const int np = std::max(1, static_cast<int>(law_.param_values.size()));
         const int nu = std::max(1, lay_.uniforms);
         out_ << "struct K {\n    float* now[" << nf << "];\n    float* nxt[" << nf << "];\n"
-             << "    const double* U;\n    float* A;\n    double* S;\n    double* L;\n    unsigned* C;\n"
+             << "    const double* U;\n    float* A;\n    double* S;\n    double* L;\n    unsigned* C;\n    const float* N;\n"
              << "    long long nxy;\n    long long cells;\n    double h;\n    double dt;\n    double inv_h;\n    double inv_hh;\n"
              << "    int nx;\n    int ny;\n    int nz;\n    int flags;\n    double P[" << np << "];\n};\n";
-        lay_.kernel_arg_bytes = (2 * nf + 5) * 8 + 6 * 8 + 4 * 4 + 8 * np;
+        lay_.kernel_arg_bytes = (2 * nf + 6) * 8 + 6 * 8 + 4 * 4 + 8 * np;
         out_ << kPrelude;
         // Reductions' results are read once a kernel into registers (a load inside the z loop would be repeated and,
         // waited for, would drain the loads in flight).
@@ -228,6 +228,16 @@ public:
              << "; }\n"
              << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
              << "};\n";
+        std::vector<std::size_t> offsets;
+        (void)net_blob(law_, offsets);
+        if (!law_.nets.empty()) {
+            // std::min / std::max on floats
+            out_ << "__device__ __forceinline__ float fe_minf(float a, float b) { return b < a ? b : a; }\n"
+                 << "__device__ __forceinline__ float fe_maxf(float a, float b) { return a < b ? b : a; }\n";
+        }
+        for (std::size_t n = 0; n < law_.nets.size(); ++n) {
+            net_function(static_cast<int>(n), offsets[n]);
+        }
         for (std::size_t id = 0; id < m_.insts.size(); ++id) {
             if (used(static_cast<int>(id))) {
                 emit(static_cast<int>(id));
@@ -356,6 +366,16 @@ private:
             case Op::MaxOf: case Op::MinOf: case Op::SumOf: {
                 const int u = lay_.uslot[static_cast<std::size_t>(id)];
                 f << "return " << (u >= 0 ? "m.u[" + std::to_string(u) + "]" : std::string("0.0")) << ";";
+                break;
+            }
+            case Op::Mlp: {
+                // every input one number: the net once (src/field/Law.cpp:1310-1318)
+                const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
+                f << "float xin[" << in.args.size() << "]; ";
+                for (std::size_t q = 0; q < in.args.size(); ++q) {
+                    f << "xin[" << q << "] = (float)s" << in.args[q] << "(k, m); ";
+                }
+                f << "float yout[" << net.outputs() << "]; net_" << in.index << "(k, xin, yout); return (double)yout[0];";
                 break;
             }
             default: f << "return " << arith(in, id) << ";"; break;
@@ -523,6 +543,19 @@ private:
         case Op::Tridiag:
             f << "    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n";
             break;
+        case Op::Mlp: {
+            // the net at the cell, inputs the arguments there as floats (src/field/Law.cpp:1320-1333)
+            const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
+            f << "    float xin[" << in.args.size() << "];\n";
+            for (std::size_t q = 0; q < in.args.size(); ++q) {
+                f << "    xin[" << q << "] = (float)" << d(in.args[q]) << ";\n";
+            }
+            f << "    float yout[" << net.outputs() << "];\n    net_" << in.index << "(k, xin, yout);\n    return yout[0];\n";
+            break;
+        }
+        case Op::Flux: case Op::LDiffuse:
+            net_faces(in, f);
+            break;
         default:
             f << "    return fe_rf(" << arith(in, id) << ");\n";
             break;
@@ -534,6 +567,140 @@ private:
                  << "(const K& k, const M& m, int x, int y, int z) { (void)m; return AT(" << lay_.slot[static_cast<std::size_t>(id)]
                  << ", x, y, z); }\n";
         }
+    }
+
+    // net_<n>(k, in, out): the CPU's forward pass (src/field/nn/Net.cpp:96-152) in fp32 in its order: inputs
+    // normalised ((x - shift) * scale), each layer acc = b[o] then += W[o][i] x[i] for i in order, the activation on
+    // every layer but the last, outputs times out_scale. No multiply-adds are fused (the CPU has none), so nets of
+    // relu and linear layers give the CPU's bits; tanh and silu go through the card's own tanhf and expf (within an ulp
+    // of the C library's).
+    void net_function(int n, std::size_t off)
+    {
+        const NetDecl& net = law_.nets[static_cast<std::size_t>(n)];
+        const int nin = net.inputs();
+        std::ostringstream f;
+        f << "__device__ __forceinline__ void net_" << n << "(const K& k, const float* in, float* out)\n{\n"
+          << "    const float* P = k.N + " << off << ";\n"
+          << "    float h0[" << nin << "];\n"
+          << "#pragma unroll\n    for (int i = 0; i < " << nin << "; ++i) h0[i] = (in[i] - P[i]) * P[" << nin << " + i];\n";
+        std::size_t w = static_cast<std::size_t>(2 * nin + 1);
+        for (int l = 0; l < net.layers(); ++l) {
+            const int ni = net.sizes[static_cast<std::size_t>(l)], no = net.sizes[static_cast<std::size_t>(l) + 1];
+            const bool last = l + 1 == net.layers();
+            f << "    float h" << l + 1 << "[" << no << "];\n"
+              << "#pragma unroll\n    for (int o = 0; o < " << no << "; ++o) {\n"
+              << "        const float* row = P + " << w << " + o * " << ni << ";\n"
+              << "        float acc = P[" << w + static_cast<std::size_t>(ni) * static_cast<std::size_t>(no) << " + o];\n"
+              << "#pragma unroll\n        for (int i = 0; i < " << ni << "; ++i) acc += row[i] * h" << l << "[i];\n";
+            if (last) {
+                f << "        h" << l + 1 << "[o] = acc;\n";
+            } else {
+                switch (net.act) {
+                case Activation::Tanh: f << "        h" << l + 1 << "[o] = tanhf(acc);\n"; break;
+                case Activation::Relu: f << "        h" << l + 1 << "[o] = acc > 0.0f ? acc : 0.0f;\n"; break;
+                case Activation::Silu: f << "        h" << l + 1 << "[o] = acc / (1.0f + expf(-acc));\n"; break;
+                case Activation::Linear: f << "        h" << l + 1 << "[o] = acc;\n"; break;
+                case Activation::Gelu:
+                    f << "        h" << l + 1 << "[o] = 0.5f * acc * (1.0f + tanhf(0.7978845608028654f * (acc + 0.044715f * acc * acc * acc)));\n";
+                    break;
+                }
+            }
+            f << "    }\n";
+            w += static_cast<std::size_t>(ni + 1) * static_cast<std::size_t>(no);
+        }
+        f << "#pragma unroll\n    for (int o = 0; o < " << net.outputs() << "; ++o) out[o] = h" << net.layers() << "[o] * P[" << 2 * nin
+          << "];\n}\n";
+        out_ << f.str();
+    }
+
+    // flux(NET, F, a..., lo, hi) and ldiffuse(NET, k, F) at a cell: its six faces' fluxes in the CPU's order
+    // (src/field/Law.cpp:1379-1389), each computed here (a face's flux comes out the same in both its cells: the same
+    // inputs, the same arithmetic), out = sum (inflow - outflow) in float, over h.
+    void net_faces(const ir::Inst& in, std::ostringstream& f)
+    {
+        const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
+        const bool flux = in.op == Op::Flux;
+        if (flux) {
+            // face i -> j: the net both ways round, half the difference, limited (src/field/Law.cpp:673-707)
+            const std::size_t mm = in.args.size() - 2;
+            const double lo = at(in.args[mm]).number, hi = at(in.args[mm + 1]).number;
+            f << "    auto face = [&](int xi, int yi, int zi, int xj, int yj, int zj) -> float {\n"
+              << "        float xij[" << 2 * mm << "], xji[" << 2 * mm << "], y[" << net.outputs() << "];\n";
+            for (std::size_t q = 0; q < mm; ++q) {
+                f << "        xij[" << q << "] = (float)" << d(in.args[q], "xi", "yi", "zi") << "; xij[" << mm + q << "] = (float)"
+                  << d(in.args[q], "xj", "yj", "zj") << ";\n"
+                  << "        xji[" << q << "] = xij[" << mm + q << "]; xji[" << mm + q << "] = xij[" << q << "];\n";
+            }
+            f << "        net_" << in.index << "(k, xij, y); const float gij = y[0];\n"
+              << "        net_" << in.index << "(k, xji, y); const float gji = y[0];\n"
+              << "        float r = 0.5f * (gij - gji);\n"
+              << "        if (k.dt > 0.0) {\n"
+              << "            const float per = (float)(0.999 * k.h / (6.0 * k.dt));\n"
+              << "            const float fi = xij[0], fj = xij[" << mm << "];\n";
+            const bool has_lo = lo > -1e29, has_hi = hi < 1e29;
+            f << "            if (r > 0.0f) {\n";
+            if (has_lo) {
+                f << "                r = fe_minf(r, fe_maxf(0.0f, (fi - (float)" << literal(lo) << ") * per));\n";
+            }
+            if (has_hi) {
+                f << "                r = fe_minf(r, fe_maxf(0.0f, ((float)" << literal(hi) << " - fj) * per));\n";
+            }
+            f << "            } else if (r < 0.0f) {\n";
+            if (has_lo) {
+                f << "                r = fe_maxf(r, -fe_maxf(0.0f, (fj - (float)" << literal(lo) << ") * per));\n";
+            }
+            if (has_hi) {
+                f << "                r = fe_maxf(r, -fe_maxf(0.0f, ((float)" << literal(hi) << " - fi) * per));\n";
+            }
+            f << "            }\n        }\n        return r;\n    };\n";
+        } else {
+            // ldiffuse: the face's window F[i-1] F[i] | F[j] F[j+1] weighted by what the net sees (src/field/Law.cpp:709-738)
+            const int ka = in.args[0], fa = in.args[1];
+            f << "    auto raw = [&](const float* w, float kf) -> float {\n"
+              << "        const float centre = 0.5f * (w[1] + w[2]);\n        float spread = 1e-6f;\n"
+              << "        for (int a = 0; a < 4; ++a) spread = fe_maxf(spread, fabsf(w[a] - centre));\n"
+              << "        float x[5];\n        for (int a = 0; a < 4; ++a) x[a] = (w[a] - centre) / spread;\n"
+              << "        const float dtf = (float)k.dt, hf = (float)k.h;\n"
+              << "        x[4] = kf * dtf / (hf * hf);\n"
+              << "        float y[4];\n        net_" << in.index << "(k, x, y);\n"
+              << "        const float mean = 0.25f * (y[0] + y[1] + y[2] + y[3]);\n"
+              << "        const float base[4] = {0.0f, -1.0f, 1.0f, 0.0f};\n        float acc = 0.0f;\n"
+              << "        for (int a = 0; a < 4; ++a) acc += (base[a] + (y[a] - mean)) * w[a];\n"
+              << "        return -kf / hf * acc;\n    };\n"
+              << "    auto face = [&](int axis, int xi, int yi, int zi) -> float {\n"
+              << "        const int cnt = axis == 0 ? k.nx : (axis == 1 ? k.ny : k.nz);\n"
+              << "        const int ci = axis == 0 ? xi : (axis == 1 ? yi : zi);\n"
+              << "        const int sx = axis == 0, sy = axis == 1, sz = axis == 2;\n"
+              << "        const int xj = xi + sx, yj = yi + sy, zj = zi + sz;\n"
+              << "        const int xm = ci > 0 ? xi - sx : xi, ym = ci > 0 ? yi - sy : yi, zm = ci > 0 ? zi - sz : zi;\n"
+              << "        const int xp = ci + 2 < cnt ? xj + sx : xj, yp = ci + 2 < cnt ? yj + sy : yj, zp = ci + 2 < cnt ? zj + sz : zj;\n"
+              << "        const float w[4] = {(float)" << d(fa, "xm", "ym", "zm") << ", (float)" << d(fa, "xi", "yi", "zi") << ", (float)"
+              << d(fa, "xj", "yj", "zj") << ", (float)" << d(fa, "xp", "yp", "zp") << "};\n"
+              << "        const double ka = " << d(ka, "xi", "yi", "zi") << ", kb = " << d(ka, "xj", "yj", "zj") << ";\n"
+              << "        const float kf = (float)((ka + kb) > 0.0 ? 2.0 * ka * kb / (ka + kb) : 0.0);\n"
+              << "        const float mirror[4] = {w[3], w[2], w[1], w[0]};\n"
+              << "        const float ij = raw(w, kf), ji = raw(mirror, kf);\n"
+              << "        return 0.5f * (ij - ji);\n    };\n";
+        }
+        // the cell: in through its -x, -y, -z faces, out through +x, +y, +z, in that order
+        if (flux) {
+            f << "    float out = 0.0f;\n"
+              << "    if (x > 0) out += face(x - 1, y, z, x, y, z);\n"
+              << "    if (x + 1 < k.nx) out -= face(x, y, z, x + 1, y, z);\n"
+              << "    if (y > 0) out += face(x, y - 1, z, x, y, z);\n"
+              << "    if (y + 1 < k.ny) out -= face(x, y, z, x, y + 1, z);\n"
+              << "    if (z > 0) out += face(x, y, z - 1, x, y, z);\n"
+              << "    if (z + 1 < k.nz) out -= face(x, y, z, x, y, z + 1);\n";
+        } else {
+            f << "    float out = 0.0f;\n"
+              << "    if (x > 0) out += face(0, x - 1, y, z);\n"
+              << "    if (x + 1 < k.nx) out -= face(0, x, y, z);\n"
+              << "    if (y > 0) out += face(1, x, y - 1, z);\n"
+              << "    if (y + 1 < k.ny) out -= face(1, x, y, z);\n"
+              << "    if (z > 0) out += face(2, x, y, z - 1);\n"
+              << "    if (z + 1 < k.nz) out -= face(2, x, y, z);\n";
+        }
+        f << "    return out / (float)k.h;\n";
     }
 
     // An element-wise op in double over its arguments at (x, y, z), as the CPU's lambdas compute it
@@ -997,6 +1164,26 @@ std::string generate(const ir::Module& m, const Law& law, LawLayout& layout, std
 {
     layout = LawLayout{};
     return Generator(m, law, layout).run(why);
+}
+
+std::vector<float> net_blob(const Law& law, std::vector<std::size_t>& offsets)
+{
+    std::vector<float> blob;
+    offsets.clear();
+    for (const NetDecl& net : law.nets) {
+        offsets.push_back(blob.size());
+        const int nin = net.inputs();
+        for (int i = 0; i < nin; ++i) {
+            blob.push_back(i < static_cast<int>(net.in_shift.size()) ? net.in_shift[static_cast<std::size_t>(i)] : 0.0f);
+        }
+        for (int i = 0; i < nin; ++i) {
+            blob.push_back(i < static_cast<int>(net.in_scale.size()) ? net.in_scale[static_cast<std::size_t>(i)] : 1.0f);
+        }
+        blob.push_back(net.out_scale);
+        blob.insert(blob.end(), net.weights.begin(), net.weights.end());
+        blob.resize(blob.size() + ((4 - blob.size() % 4) % 4), 0.0f); // each net 16-byte aligned
+    }
+    return blob;
 }
 
 std::vector<std::string> compile_options()

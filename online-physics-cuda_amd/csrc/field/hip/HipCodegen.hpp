@@ -75,4 +75,9 @@ std::string generate(const ir::Module& m, const Law& law, LawLayout& layout, std
 // hipRTC options every generated source is compiled with (see the prelude for why each is there).
 std::vector<std::string> compile_options();
 
+// The law's nets as one float blob for the card, net after net: in_shift[inputs] (0 where not given),
+// in_scale[inputs] (1 where not given), out_scale, then the weights (per layer W (out x in, row-major), b);
+// offsets[n] is where net n starts.
+std::vector<float> net_blob(const Law& law, std::vector<std::size_t>& offsets);
+
 } // namespace fieldengine::field::hip

@@ -94,7 +94,8 @@ struct Impl final : LawBackend {
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
     std::vector<void*> now, nxt;
-    Buffer P, U, A, S, L, C, records, firsts, totals;
+    Buffer P, U, A, S, L, C, N, records, firsts, totals;
+    std::vector<float> net_host; // the law's nets as the kernels read them (HipCodegen.cpp: net_blob)
     std::vector<double> uploaded_params;
     bool params_valid = false;
     std::vector<fe_hip_event> events;
@@ -136,7 +137,7 @@ struct Impl final : LawBackend {
     {
         const int nf = std::max(1, static_cast<int>(law->fields.size()));
         const int np = std::max(1, static_cast<int>(law->param_values.size()));
-        std::vector<std::uint64_t> k(static_cast<std::size_t>(2 * nf + 5 + 6 + 2 + np), 0);
+        std::vector<std::uint64_t> k(static_cast<std::size_t>(2 * nf + 6 + 6 + 2 + np), 0);
         std::size_t w = 0;
         for (int f = 0; f < nf; ++f) {
             k[w++] = reinterpret_cast<std::uint64_t>(f < static_cast<int>(now.size()) ? now[static_cast<std::size_t>(f)] : nullptr);
@@ -149,6 +150,7 @@ struct Impl final : LawBackend {
         k[w++] = reinterpret_cast<std::uint64_t>(S.p);
         k[w++] = reinterpret_cast<std::uint64_t>(L.p);
         k[w++] = reinterpret_cast<std::uint64_t>(C.p);
+        k[w++] = reinterpret_cast<std::uint64_t>(N.p);
         const long long nxy = static_cast<long long>(shape.nx) * shape.ny;
         const long long cells = nxy * shape.nz;
         std::memcpy(&k[w++], &nxy, 8);
@@ -319,6 +321,12 @@ struct Impl final : LawBackend {
         S.allocate(tridiag ? 2 * cells * 8 : 8);
         L.allocate(static_cast<std::size_t>(3 * kReduceBlocks) * 8);
         C.allocate(std::max<std::size_t>(1, nf) * 4);
+        std::vector<std::size_t> net_offsets;
+        net_host = hip::net_blob(*law, net_offsets);
+        N.allocate(std::max<std::size_t>(1, net_host.size()) * 4);
+        if (!net_host.empty()) {
+            check(fe_hip_memcpy_htod_async(N.p, net_host.data(), net_host.size() * 4), "nets upload");
+        }
         records.allocate(std::max<std::size_t>(1, nf) * 2 * 4);
         firsts.allocate(std::max<std::size_t>(1, nf) * 8);
         totals.allocate(static_cast<std::size_t>(kTotals) * 8);

@@ -241,6 +241,50 @@ def _ops_init(n, seed):
 
 G_OPS = (20, 14, 9, 0.5)
 
+
+def net_text(name, sizes, act, seed, out_scale=1.0, shift=None, scale=None):
+    """A .net file in the reference's text format (sections sizes / act / in_shift / in_scale / out_scale / weights;
+    per layer W (out x in, row-major) then b), seeded uniform weights of Glorot scale with a nonzero last layer."""
+    rng = np.random.default_rng(seed)
+    weights = []
+    for ni, no in zip(sizes[:-1], sizes[1:]):
+        lim = np.sqrt(6.0 / (ni + no))
+        weights += list(rng.uniform(-lim, lim, ni * no).astype(np.float32)) + list(rng.uniform(-0.2, 0.2, no).astype(np.float32))
+    shift = [0.0] * sizes[0] if shift is None else shift
+    scale = [1.0] * sizes[0] if scale is None else scale
+    f32 = lambda v: repr(float(np.float32(v)))  # noqa: E731  shortest text that reads back as the same float
+    lines = [f"net {name}", "sizes " + " ".join(str(n) for n in sizes), f"act {act}",
+             "in_shift " + " ".join(f32(v) for v in shift), "in_scale " + " ".join(f32(v) for v in scale),
+             f"out_scale {f32(out_scale)}", "weights"]
+    for i in range(0, len(weights), 8):
+        lines.append(" ".join(f32(v) for v in weights[i:i + 8]))
+    return "\n".join(lines) + "\n"
+
+
+def net_law(decls: str, body: str, a_range: str = "-1000000000 1000000000") -> str:
+    return (decls + "field A init 0 range " + a_range + "\nfield B init 0 range -1000000000 1000000000\n"
+            "param kk = 0.3\nterm t\n" + body)
+
+
+# Law nets (src/field/Law.cpp:1298-1443). relu and linear layers give the CPU's bits; tanh and silu go through the
+# card's tanhf / expf, held to a tolerance (atol) instead.
+NET_CASES = [
+    dict(name="net_mlp_relu", law=net_law("net q in 2 hidden 8 8 out 2 act relu\n",
+                                          "  d B = mlp(q, A, B) + 0.1 * mlp(q, kk, 2)\n  d A = 0.5 * mlp(q, B, A)\n"),
+         nets={"q": net_text("q", [2, 8, 8, 2], "relu", 31, 0.7, [0.5, 0.2], [2.0, 1.5])}),
+    dict(name="net_flux_relu", law=net_law("net g in 4 hidden 8 out 1 act relu\n", "  d A = flux(g, A, B)\n", "0 1"),
+         nets={"g": net_text("g", [4, 8, 1], "relu", 32, 1.3, [0.5, 0.3, 0.5, 0.3])}, dt=0.05),
+    dict(name="net_flux_limited", law=net_law("net g in 2 hidden 6 out 1 act relu\n", "  d A = flux(g, A)\n", "0 1"),
+         nets={"g": net_text("g", [2, 6, 1], "relu", 33, 40.0)}, dt=0.05),
+    dict(name="net_ldiffuse_relu", law=net_law("net w in 5 hidden 8 out 4 act relu\n",
+                                               "  d B = ldiffuse(w, kk, B)\n  d A = ldiffuse(w, 0.5 + B, A)\n"),
+         nets={"w": net_text("w", [5, 8, 4], "relu", 34, 0.3)}),
+    dict(name="net_flux_tanh", law=net_law("net g in 4 hidden 8 out 1 act tanh\n", "  d A = flux(g, A, B)\n"),
+         nets={"g": net_text("g", [4, 8, 1], "tanh", 35)}, atol=1e-6),
+    dict(name="net_ldiffuse_silu", law=net_law("net w in 5 hidden 8 out 4 act silu\n", "  d A = ldiffuse(w, kk, A)\n"),
+         nets={"w": net_text("w", [5, 8, 4], "silu", 36, 0.3)}, atol=1e-6),
+]
+
 CASES = [
     dict(name="heat_20x14x9", law=HEAT_LAW, grid=(20, 14, 9, 0.5), dt=0.9, steps=4,
          params={"k": 0.1}, init={"T": _uniform(20 * 14 * 9, 1)}),
@@ -272,4 +316,7 @@ CASES = [
     dict(name="op_" + key, law=op_law(expr), grid=G_OPS, dt=0.02, steps=4, params={},
          init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))
     for i, (key, expr) in enumerate(OPS.items())
+] + [
+    dict(grid=G_OPS, steps=4, params={}, init=_ops_init(int(np.prod(G_OPS[:3])), 200 + i), **dict(dict(dt=0.02), **case))
+    for i, case in enumerate(NET_CASES)
 ]

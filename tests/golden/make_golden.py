@@ -41,7 +41,12 @@ def run_reference(case: dict) -> dict:
         for name, value in case.get("params", {}).items():
             cmd += ["--param", name, repr(float(value))]
         cmd.append(law_path)
-        res = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        # A law's nets load from assets/nets/NAME.net under the working directory (src/field/Law.cpp:1706).
+        for name, text in case.get("nets", {}).items():
+            os.makedirs(os.path.join(tmp, "assets", "nets"), exist_ok=True)
+            with open(os.path.join(tmp, "assets", "nets", name + ".net"), "w") as f:
+                f.write(text)
+        res = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp, env=dict(os.environ, OMP_NUM_THREADS="4"))
         if res.returncode != 0:
             raise RuntimeError(f"lawc failed for {case['name']}: {res.stdout}\n{res.stderr}")
         final_dir = os.path.join(out_dir, f"step_{case['steps']}")
@@ -65,6 +70,9 @@ def main() -> None:
             arrays["final_" + name] = values
         meta = {k: case[k] for k in ("name", "law", "grid", "dt", "steps")}
         meta["params"] = case.get("params", {})
+        for key in ("nets", "atol"):
+            if key in case:
+                meta[key] = case[key]
         meta["reference_stdout"] = out["stdout"]
         meta["reference_stderr"] = out["stderr"]
         arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)

@@ -1,8 +1,10 @@
 """Reads the golden vectors (tests/golden/*.npz) the reference's CPU engine wrote."""
 from __future__ import annotations
 
+import contextlib
 import json
 import os
+import tempfile
 
 import numpy as np
 
@@ -19,3 +21,35 @@ def load(name: str):
 
 def names(prefix: str = ""):
     return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith(prefix))
+
+
+def make_law(field_hip, meta):
+    """The case's law, its nets read from NAME.net files under assets/nets/ of the working directory, as the
+    reference's lawc read them when it wrote the case."""
+    nets = meta.get("nets", {})
+    if not nets:
+        return field_hip.Law(meta["law"])
+    with tempfile.TemporaryDirectory() as tmp, _cwd(tmp):
+        os.makedirs(os.path.join("assets", "nets"))
+        for name, text in nets.items():
+            with open(os.path.join("assets", "nets", name + ".net"), "w") as f:
+                f.write(text)
+        return field_hip.Law(meta["law"])
+
+
+@contextlib.contextmanager
+def _cwd(path):
+    old = os.getcwd()
+    os.chdir(path)
+    try:
+        yield
+    finally:
+        os.chdir(old)
+
+
+def same(meta, got, want) -> bool:
+    """The reference's bits, or within the case's atol where it names one (nets through the card's tanhf / expf)."""
+    atol = meta.get("atol")
+    if atol is None:
+        return got.shape == want.shape and bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
+    return got.shape == want.shape and bool(np.allclose(got, want, rtol=0.0, atol=atol))

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def run_engine(field_hip, meta, init, lazy=False, contracts=1):
     nx, ny, nz, h = meta["grid"]
-    law = field_hip.Law(meta["law"])
+    law = golden_io.make_law(field_hip, meta)
     solver = field_hip.Solver(law, (nx, ny, nz), h)
     for name, value in meta["params"].items():
         solver.set_param(name, value)
@@ -41,11 +41,11 @@ def test_engine_matches_reference_bits(device, name):
     law, solver = run_engine(field_hip, meta, init)
     for field, expected in final.items():
         got = solver.get_field(field)
-        assert exact(got, expected), f"{name}/{field}: max |diff| {np.max(np.abs(got - expected))}"
+        assert golden_io.same(meta, got, expected), f"{name}/{field}: max |diff| {np.max(np.abs(got - expected))}"
     assert solver.violations() == [] or all(v["kind"] != "conservation" for v in solver.violations())
 
 
-@pytest.mark.parametrize("name", ["heat_20x14x9", "multi", "diffreact_rk4"])
+@pytest.mark.parametrize("name", ["heat_20x14x9", "multi", "diffreact_rk4", "net_flux_tanh", "net_ldiffuse_relu"])
 def test_two_runs_and_lazy_readback_agree_to_the_bit(device, name):
     import field_hip
 
@@ -117,7 +117,7 @@ def test_every_gather_matches_reference_bits(device, monkeypatch, gather, name):
     print(f"case {name} gather {gather} grid {meta['grid']}", flush=True)
     _, solver = run_engine(field_hip, meta, init)
     for field, expected in final.items():
-        assert exact(solver.get_field(field), expected), f"{name}/{field} through {gather}"
+        assert golden_io.same(meta, solver.get_field(field), expected), f"{name}/{field} through {gather}"
 
 
 CONDUCTION = "field T init 0\nparam k = 0.1\nterm conduction\n  d T = diffuse(k, T)\n"

@@ -49,7 +49,7 @@ def test_generated_source_compiles_for_gfx950(name):
     import field_hip
 
     meta, _, _ = golden_io.load(name)
-    law = field_hip.Law(meta["law"])
+    law = golden_io.make_law(field_hip, meta)
     src = law.hip_source()
     assert "void g_t0(K k" in src
     opts = ["-O3", "-std=c++17", "-ffp-contract=off", "-fgpu-flush-denormals-to-zero", "-Xclang", "-target-feature",
