@@ -195,16 +195,7 @@ public:
 
     std::string run(std::string& why)
     {
-        for (std::size_t id = 0; id < m_.insts.size(); ++id) {
-            const Op op = m_.insts[id].op;
-            if (op == Op::Distance || op == Op::Poisson || op == Op::Fall || op == Op::Spread || op == Op::Topple ||
-                op == Op::Local) {
-                if (used(static_cast<int>(id))) {
-                    why = std::string("the HIP backend does not run ") + ir::op_name(op) + " yet";
-                    return "";
-                }
-            }
-        }
+        (void)why; // every op of the IR has a form here
         allocate();
         lay_.fields = static_cast<int>(law_.fields.size());
         const int nf = std::max(1, lay_.fields);
@@ -318,6 +309,20 @@ private:
                 s.kind = StageCode::Kind::Tridiag;
                 s.axis = in.index;
                 lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
+            } else if (in.op == Op::Distance) {
+                s.kind = StageCode::Kind::Distance;
+                lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots;
+                lay_.aux_slots += 6;
+                s.words = lay_.stage_words;
+                lay_.stage_words += 4;
+            } else if (in.op == Op::Poisson) {
+                s.kind = StageCode::Kind::Poisson;
+                s.iterations = in.iterations;
+                s.scalar_source = scalar(in.args[0]);
+                lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots;
+                lay_.aux_slots += 3;
+                s.words = lay_.stage_words;
+                lay_.stage_words += 4;
             } else {
                 s.kind = StageCode::Kind::Stored;
                 lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
@@ -543,6 +548,81 @@ private:
         case Op::Tridiag:
             f << "    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n";
             break;
+        case Op::Fall: {
+            // F moving down into the room below (src/field/Law.cpp:1183-1210): the flux out of a cell through its
+            // floor, in double; the cell loses its own and gains the one above's.
+            if (scalar(in.args[0])) {
+                f << "    return 0.0f;\n";
+                break;
+            }
+            const int fa = in.args[0], ba = in.args[1], sa = in.args[2];
+            f << "    if (k.nz < 2) return 0.0f;\n"
+              << "    auto down = [&](int zz) -> double {\n"
+              << "        if (zz == 0) return 0.0;\n"
+              << "        const double room = fe_clamp(1.0 - " << d(ba, "x", "y", "zz - 1") << " - " << d(fa, "x", "y", "zz - 1") << ", 0.0, 1.0);\n"
+              << "        return div_h<M>(k, fe_max(0.0, " << d(sa, "x", "y", "zz") << ")) * fe_max(0.0, " << d(fa, "x", "y", "zz")
+              << ") * room;\n    };\n"
+              << "    float out = fe_rf(0.0 - down(z));\n"
+              << "    if (z + 1 < k.nz) out = fe_rf((double)out + down(z + 1));\n"
+              << "    return out;\n";
+            break;
+        }
+        case Op::Spread: {
+            // Sideways, downhill, into the room on the receiving side (src/field/Law.cpp:1212-1244): the flux across
+            // each x and y face in double, summed into the cell in the CPU's order, rounding to float at each step.
+            if (scalar(in.args[0])) {
+                f << "    return 0.0f;\n";
+                break;
+            }
+            const int fa = in.args[0], ba = in.args[1], ka = in.args[2];
+            f << "    auto across = [&](int ax, int ay, int cx, int cy) -> double {\n"
+              << "        const double fa = " << d(fa, "ax", "ay", "z") << ", fc = " << d(fa, "cx", "cy", "z") << ";\n"
+              << "        const bool to_c = fa > fc;\n        const int rx = to_c ? cx : ax, ry = to_c ? cy : ay;\n"
+              << "        const double room = fe_clamp(1.0 - " << d(ba, "rx", "ry", "z") << " - " << d(fa, "rx", "ry", "z") << ", 0.0, 1.0);\n"
+              << "        const double kf = 0.5 * (fe_max(0.0, " << d(ka, "ax", "ay", "z") << ") + fe_max(0.0, " << d(ka, "cx", "cy", "z")
+              << "));\n"
+              << "        return div_hh<M>(k, kf) * (fa - fc) * room;\n    };\n"
+              << "    float out = 0.0f;\n"
+              << "    if (y > 0) out = fe_rf((double)out + across(x, y - 1, x, y));\n"
+              << "    if (x > 0) out = fe_rf((double)out + across(x - 1, y, x, y));\n"
+              << "    out = fe_rf((double)out - (x + 1 < k.nx ? across(x, y, x + 1, y) : 0.0));\n"
+              << "    out = fe_rf((double)out - (y + 1 < k.ny ? across(x, y, x, y + 1) : 0.0));\n"
+              << "    return out;\n";
+            break;
+        }
+        case Op::Topple: {
+            // A pile slides diagonally down off what holds it (src/field/Law.cpp:1246-1296): what the cell sends down
+            // its four sides, and what the four cells above-and-beside send toward this column, in double.
+            if (scalar(in.args[0])) {
+                f << "    return 0.0f;\n";
+                break;
+            }
+            const int fa = in.args[0], ba = in.args[1], ra = in.args[2];
+            f << "    if (k.nz < 2) return 0.0f;\n"
+              << "    auto room = [&](int qx, int qy, int qz) -> double {\n"
+              << "        return fe_clamp(1.0 - " << d(ba, "qx", "qy", "qz") << " - " << d(fa, "qx", "qy", "qz") << ", 0.0, 1.0);\n    };\n"
+              << "    auto send = [&](int qx, int qy, int qz, int side) -> double {\n"
+              << "        const double here = " << d(fa, "qx", "qy", "qz") << ";\n"
+              << "        if (here <= 0.0 || qz == 0) return 0.0;\n"
+              << "        if (room(qx, qy, qz - 1) > 0.01) return 0.0;\n"
+              << "        int jx = qx, jy = qy;\n"
+              << "        if (side == 0) { if (qx == 0) return 0.0; jx = qx - 1; }\n"
+              << "        else if (side == 1) { if (qx + 1 >= k.nx) return 0.0; jx = qx + 1; }\n"
+              << "        else if (side == 2) { if (qy == 0) return 0.0; jy = qy - 1; }\n"
+              << "        else { if (qy + 1 >= k.ny) return 0.0; jy = qy + 1; }\n"
+              << "        return div_h<M>(k, 0.25 * fe_max(0.0, " << d(ra, "qx", "qy", "qz") << ")) * here * fe_min(room(jx, jy, qz), room(jx, jy, qz - 1));\n"
+              << "    };\n"
+              << "    double out = 0.0;\n"
+              << "    out -= send(x, y, z, 0);\n    out -= send(x, y, z, 1);\n    out -= send(x, y, z, 2);\n    out -= send(x, y, z, 3);\n"
+              << "    if (z + 1 < k.nz) {\n"
+              << "        if (x + 1 < k.nx) out += send(x + 1, y, z + 1, 0);\n"
+              << "        if (x > 0) out += send(x - 1, y, z + 1, 1);\n"
+              << "        if (y + 1 < k.ny) out += send(x, y + 1, z + 1, 2);\n"
+              << "        if (y > 0) out += send(x, y - 1, z + 1, 3);\n"
+              << "    }\n"
+              << "    return fe_rf(out);\n";
+            break;
+        }
         case Op::Mlp: {
             // the net at the cell, inputs the arguments there as floats (src/field/Law.cpp:1320-1333)
             const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
@@ -799,7 +879,130 @@ private:
                  << "}\n";
             break;
         }
+        case StageCode::Kind::Distance:
+            distance_kernels(s, in, slot);
+            break;
+        case StageCode::Kind::Poisson:
+            poisson_kernels(s, in, slot);
+            break;
         }
+    }
+
+    // distance(a) (src/field/Law.cpp:1042-1096): the set a >= 0.5, the squared distances (in cells) from every cell to
+    // the nearest cell inside and outside it, and from those the signed distance in metres, the surface half a cell
+    // from the last solid cell's centre and the lattice's outside counted as outside. The squared distances are
+    // whole numbers, exact in float, so the exact minimum is the CPU's value whatever the method: here each cell scans
+    // outward along one axis at a time until no farther cell can beat what it holds. The last step is the CPU's
+    // double arithmetic.
+    void distance_kernels(const StageCode& s, const ir::Inst& in, int slot)
+    {
+        const std::string set = std::to_string(slot + 1);
+        out_ << "extern \"C\" __global__ void " << s.kernel << "_mark(K k, unsigned* w) { CELL_HEAD const Direct m(k);\n"
+             << "    const float in = " << d(in.args[0]) << " >= 0.5 ? 1.0f : 0.0f;\n"
+             << "    float& held = AT(" << set << ", x, y, z);\n"
+             << "    const bool changed = __float_as_uint(held) != __float_as_uint(in) || w[3] == 0u;\n"
+             << "    held = in;\n"
+             << "    const u64 bc = __ballot(changed), bi = __ballot(in != 0.0f), bo = __ballot(in == 0.0f);\n"
+             << "    const int lane = (int)__lane_id();\n"
+             << "    if (changed && lane == __ffsll((long long)bc) - 1) w[0] = 1u;\n"
+             << "    if (in != 0.0f && lane == __ffsll((long long)bi) - 1) w[1] = 1u;\n"
+             << "    if (in == 0.0f && lane == __ffsll((long long)bo) - 1) w[2] = 1u;\n"
+             << "}\n";
+        // One axis for both sets; from_in < 0: the first axis, seeded from the set (0 on it, inf off it).
+        out_ << "extern \"C\" __global__ void " << s.kernel
+             << "_scan(K k, const unsigned* w, int axis, int from_in, int from_out, int to_in, int to_out) {\n"
+             << "    if (w[0] == 0u) return;\n    CELL_HEAD\n"
+             << "    const int n = axis == 0 ? k.nx : (axis == 1 ? k.ny : k.nz);\n"
+             << "    const int q = axis == 0 ? x : (axis == 1 ? y : z);\n"
+             << "    const i64 step = axis == 0 ? 1 : (axis == 1 ? (i64)k.nx : k.nxy);\n"
+             << "    const i64 here = IDX(x, y, z);\n"
+             << "    const float inf = __builtin_inff();\n"
+             << "    const float* fi = k.A + (i64)(from_in < 0 ? " << set << " : from_in) * k.cells;\n"
+             << "    const float* fo = k.A + (i64)(from_in < 0 ? " << set << " : from_out) * k.cells;\n"
+             << "    float bi = inf, bo = inf;\n"
+             << "    for (int r = 0; r < n; ++r) {\n"
+             << "        const float rr = (float)(r * r);\n"
+             << "        if (rr >= bi && rr >= bo) break;\n"
+             << "        const int lo = q - r, hi = q + r;\n"
+             << "        if (lo < 0 && hi >= n) break;\n"
+             << "        if (lo >= 0) {\n"
+             << "            const i64 j = here - (i64)r * step;\n"
+             << "            const float a = from_in < 0 ? (fi[j] != 0.0f ? 0.0f : inf) : fi[j];\n"
+             << "            const float b = from_in < 0 ? (fo[j] == 0.0f ? 0.0f : inf) : fo[j];\n"
+             << "            bi = fminf(bi, a + rr); bo = fminf(bo, b + rr);\n"
+             << "        }\n"
+             << "        if (hi < n) {\n"
+             << "            const i64 j = here + (i64)r * step;\n"
+             << "            const float a = from_in < 0 ? (fi[j] != 0.0f ? 0.0f : inf) : fi[j];\n"
+             << "            const float b = from_in < 0 ? (fo[j] == 0.0f ? 0.0f : inf) : fo[j];\n"
+             << "            bi = fminf(bi, a + rr); bo = fminf(bo, b + rr);\n"
+             << "        }\n"
+             << "    }\n"
+             << "    k.A[(i64)to_in * k.cells + here] = bi; k.A[(i64)to_out * k.cells + here] = bo;\n"
+             << "}\n";
+        out_ << "extern \"C\" __global__ void " << s.kernel << "_signed(K k, unsigned* w) {\n"
+             << "    if (w[0] == 0u) return;\n    CELL_HEAD\n"
+             << "    const bool any_in = w[1] != 0u, any_out = w[2] != 0u;\n"
+             << "    const double far = (double)(k.nx + k.ny + k.nz) * k.h;\n"
+             << "    double dd;\n"
+             << "    if (AT(" << set << ", x, y, z) != 0.0f) {\n"
+             << "        double border = __builtin_inf();\n"
+             << "        if (k.nx > 1) border = fe_min(fe_min(border, (double)(x + 1)), (double)(k.nx - x));\n"
+             << "        if (k.ny > 1) border = fe_min(fe_min(border, (double)(y + 1)), (double)(k.ny - y));\n"
+             << "        if (k.nz > 1) border = fe_min(fe_min(border, (double)(z + 1)), (double)(k.nz - z));\n"
+             << "        const double cells = fe_min(any_out ? sqrt((double)AT(" << slot + 3 << ", x, y, z)) : far / k.h, border);\n"
+             << "        dd = -(cells - 0.5) * k.h;\n"
+             << "    } else {\n"
+             << "        dd = any_in ? (sqrt((double)AT(" << slot + 2 << ", x, y, z)) - 0.5) * k.h : far;\n"
+             << "    }\n"
+             << "    AT(" << slot << ", x, y, z) = fe_rf(dd);\n"
+             << "    if (x == 0 && y == 0 && z == 0) w[3] = 1u;\n"
+             << "}\n";
+    }
+
+    // poisson(s[, n]) (src/field/Law.cpp:1139-1181): n Jacobi sweeps of lap(phi) = s with phi = 0 outside the lattice,
+    // warm-started from the last answer (zero at first), each sweep's sum in double in the CPU's order; skipped when
+    // the source holds the bits it held at the last solve.
+    void poisson_kernels(const StageCode& s, const ir::Inst& in, int slot)
+    {
+        const int src = in.args[0];
+        if (s.scalar_source) {
+            out_ << "extern \"C\" __global__ void " << s.kernel << "_mark(K k, unsigned* w) {\n"
+                 << "    const Direct m(k);\n"
+                 << "    const u64 b = (u64)__double_as_longlong(" << d(src) << ");\n"
+                 << "    const bool changed = w[3] == 0u || w[1] != (unsigned)b || w[2] != (unsigned)(b >> 32);\n"
+                 << "    w[1] = (unsigned)b; w[2] = (unsigned)(b >> 32);\n"
+                 << "    if (changed) w[0] = 1u;\n"
+                 << "}\n";
+        } else {
+            out_ << "extern \"C\" __global__ void " << s.kernel << "_mark(K k, unsigned* w) { CELL_HEAD const Direct m(k);\n"
+                 << "    const float v = " << v(src) << ";\n"
+                 << "    float& held = AT(" << slot + 2 << ", x, y, z);\n"
+                 << "    const bool changed = __float_as_uint(held) != __float_as_uint(v) || w[3] == 0u;\n"
+                 << "    held = v;\n"
+                 << "    const u64 bc = __ballot(changed);\n"
+                 << "    if (changed && (int)__lane_id() == __ffsll((long long)bc) - 1) w[0] = 1u;\n"
+                 << "}\n";
+        }
+        out_ << "extern \"C\" __global__ void " << s.kernel << "_sweep(K k, const unsigned* w, int from, int to) {\n"
+             << "    if (w[0] == 0u) return;\n    CELL_HEAD\n"
+             << (s.scalar_source ? "    const Direct m(k);\n" : "")
+             << "    const float* phi = k.A + (i64)from * k.cells;\n"
+             << "    const i64 i = IDX(x, y, z);\n"
+             << "    double sum = 0.0;\n"
+             << "    if (k.nx > 1) { if (x > 0) sum += phi[i - 1]; if (x + 1 < k.nx) sum += phi[i + 1]; }\n"
+             << "    if (k.ny > 1) { if (y > 0) sum += phi[i - k.nx]; if (y + 1 < k.ny) sum += phi[i + k.nx]; }\n"
+             << "    if (k.nz > 1) { if (z > 0) sum += phi[i - k.nxy]; if (z + 1 < k.nz) sum += phi[i + k.nxy]; }\n"
+             << "    const int axes = (k.nx > 1 ? 1 : 0) + (k.ny > 1 ? 1 : 0) + (k.nz > 1 ? 1 : 0);\n"
+             << "    const double denom = 2.0 * (double)(axes > 1 ? axes : 1);\n"
+             << "    const double s = " << (s.scalar_source ? d(src) : "(double)AT(" + std::to_string(slot + 2) + ", x, y, z)") << ";\n"
+             << "    k.A[(i64)to * k.cells + i] = fe_rf((sum - k.h * k.h * s) / denom);\n"
+             << "}\n";
+        out_ << "extern \"C\" __global__ void " << s.kernel << "_copy(K k, unsigned* w, int from) {\n"
+             << "    if (w[0] == 0u) return;\n    CELL_HEAD\n"
+             << "    if (from != " << slot << ") AT(" << slot << ", x, y, z) = k.A[(i64)from * k.cells + IDX(x, y, z)];\n"
+             << "    if (x == 0 && y == 0 && z == 0) w[3] = 1u;\n"
+             << "}\n";
     }
 
     ProgramCode program(const ir::Program& p, const std::string& tag, bool derive)

@@ -24,12 +24,25 @@
 namespace fieldengine::field::hip {
 
 struct StageCode {
-    enum class Kind { Stored, Reduce, Tridiag };
+    // Distance: the signed distance to the surface of where the argument is >= 0.5, recomputed only when that set
+    // changed. Slots from the instruction's slot s: s the answer, s+1 the set (1 inside, 0 outside), s+2 / s+3 the
+    // squared distances to the nearest inside / outside cell, s+4 / s+5 scratch. Kernels: <kernel>_mark (the set and
+    // whether it changed), <kernel>_scan (one axis of the squared distance transform), <kernel>_signed.
+    // Poisson: Jacobi sweeps warm-started from the last answer, skipped when the source is what it was at the last
+    // solve. Slots: s the answer, s+1 the other half of the ping-pong, s+2 the source (a cell source). Kernels:
+    // <kernel>_mark, <kernel>_sweep, <kernel>_copy.
+    // Both keep four words of state (words .. words+3 of the solver's stage words, zero at the start): [0] changed,
+    // [1] / [2] Distance: some cell inside / outside, Poisson with a scalar source: its double's bits; [3] solved
+    // once. A solve begins by clearing [0] (Distance: [0..2]) with k_zero.
+    enum class Kind { Stored, Reduce, Tridiag, Distance, Poisson };
     Kind kind = Kind::Stored;
     int inst = -1;
     std::string kernel;     // the stage's kernel (Reduce: its per-block pass; the final pass is k_reduce_final)
     int axis = 0;           // Tridiag: the axis its lines run along
     Op reduce = Op::SumOf;
+    int iterations = 0;     // Poisson: sweeps a solve
+    int words = -1;         // Distance, Poisson: the first of its four state words
+    bool scalar_source = false; // Poisson: the source is one number (its _mark kernel runs as one thread)
 };
 
 // A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of
@@ -60,6 +73,7 @@ struct LawLayout {
     int fields = 0;
     int aux_slots = 0;      // float slots of A, one lattice each
     int uniforms = 0;       // doubles of U (reductions)
+    int stage_words = 0;    // u32 state words of the Distance and Poisson stages, zero at the start
     std::vector<int> slot;  // per instruction: its slot in A, or -1
     std::vector<int> uslot; // per instruction: its value in U, or -1
     std::map<int, StageCode> stages;

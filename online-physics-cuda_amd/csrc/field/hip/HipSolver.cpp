@@ -94,7 +94,7 @@ struct Impl final : LawBackend {
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
     std::vector<void*> now, nxt;
-    Buffer P, U, A, S, L, C, N, records, firsts, totals;
+    Buffer P, U, A, S, L, C, N, W, records, firsts, totals;
     std::vector<float> net_host; // the law's nets as the kernels read them (HipCodegen.cpp: net_blob)
     std::vector<double> uploaded_params;
     bool params_valid = false;
@@ -327,6 +327,15 @@ struct Impl final : LawBackend {
         if (!net_host.empty()) {
             check(fe_hip_memcpy_htod_async(N.p, net_host.data(), net_host.size() * 4), "nets upload");
         }
+        // The Distance and Poisson stages' state words start at zero, and a Poisson stage's answer (its warm start) too.
+        W.allocate(static_cast<std::size_t>(std::max(1, layout.stage_words)) * 4);
+        check(fe_hip_memset_d32_async(W.p, 0u, static_cast<std::size_t>(std::max(1, layout.stage_words))), "stage words reset");
+        for (const auto& [id, s] : layout.stages) {
+            if (s.kind == hip::StageCode::Kind::Poisson) {
+                char* answer = A.as<char>() + static_cast<std::size_t>(layout.slot[static_cast<std::size_t>(id)]) * cells * 4;
+                check(fe_hip_memset_d32_async(answer, 0u, cells), "poisson warm start reset");
+            }
+        }
         records.allocate(std::max<std::size_t>(1, nf) * 2 * 4);
         firsts.allocate(std::max<std::size_t>(1, nf) * 8);
         totals.allocate(static_cast<std::size_t>(kTotals) * 8);
@@ -423,6 +432,46 @@ struct Impl final : LawBackend {
             std::vector<std::uint64_t> k = args(dt);
             void* argv[] = {k.data()};
             check(fe_hip_launch(fn(s.kernel), static_cast<unsigned>((lines + 63) / 64), 1, 1, 64, 1, 1, 0, argv), "tridiag launch");
+            ++prof.launches;
+            break;
+        }
+        case hip::StageCode::Kind::Distance: {
+            unsigned* w = W.as<unsigned>() + s.words;
+            const int slot = layout.slot[static_cast<std::size_t>(id)];
+            int clear = 3;
+            void* zargs[] = {&w, &clear};
+            check(fe_hip_launch(fn("k_zero"), 1, 1, 1, 64, 1, 1, 0, zargs), "distance reset");
+            launch_cells(s.kernel + "_mark", dt, {&w});
+            // x from the set into (s+2, s+3), y into (s+4, s+5), z back into (s+2, s+3)
+            const int passes[3][4] = {{-1, -1, slot + 2, slot + 3}, {slot + 2, slot + 3, slot + 4, slot + 5}, {slot + 4, slot + 5, slot + 2, slot + 3}};
+            for (int axis = 0; axis < 3; ++axis) {
+                int a = axis, fi = passes[axis][0], fo = passes[axis][1], ti = passes[axis][2], to = passes[axis][3];
+                launch_cells(s.kernel + "_scan", dt, {&w, &a, &fi, &fo, &ti, &to});
+            }
+            launch_cells(s.kernel + "_signed", dt, {&w});
+            ++prof.launches;
+            break;
+        }
+        case hip::StageCode::Kind::Poisson: {
+            unsigned* w = W.as<unsigned>() + s.words;
+            const int slot = layout.slot[static_cast<std::size_t>(id)];
+            int clear = 1;
+            void* zargs[] = {&w, &clear};
+            check(fe_hip_launch(fn("k_zero"), 1, 1, 1, 64, 1, 1, 0, zargs), "poisson reset");
+            if (s.scalar_source) {
+                std::vector<std::uint64_t> k = args(dt);
+                void* margs[] = {k.data(), &w};
+                check(fe_hip_launch(fn(s.kernel + "_mark"), 1, 1, 1, 1, 1, 1, 0, margs), "poisson mark");
+                ++prof.launches;
+            } else {
+                launch_cells(s.kernel + "_mark", dt, {&w});
+            }
+            int from = slot, to = slot + 1;
+            for (int it = 0; it < s.iterations; ++it) {
+                launch_cells(s.kernel + "_sweep", dt, {&w, &from, &to});
+                std::swap(from, to);
+            }
+            launch_cells(s.kernel + "_copy", dt, {&w, &from});
             ++prof.launches;
             break;
         }

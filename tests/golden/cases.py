@@ -105,7 +105,22 @@ OPS = {
     "reductions": "d A = 0.01 * (maxof(B) - minof(A)) + 0.0001 * sumof(B)",
     "transposes": "d A = 0.1 * (shift_t(A, 1, 0, -1) + gx_t(B) - gy_t(A) + gz_t(A))",
     "rim_mul_zero": "d A = rim * (A - B) + 0 * lap(A) - 0.25 * A * B",
+    "fall": "d B = fall(B, A, 2) + 0.5 * fall(A, B, 1)",
+    "spread": "d B = spread(B, A * 0.5, 0.3) + spread(A, B, A)",
+    "topple": "d B = topple(B, A, 1.5)\n  d A = topple(A, B, 0.5 + B)",
+    "distance": "d A = 0.1 * distance(B) + 0.05 * distance(A)",
+    "poisson": "d A = 0.01 * poisson(B - 0.4, 5) + 0.02 * poisson(A, 3) + 0.001 * poisson(0.3, 2)",
 }
+
+# A set that moves one cell along x a step (dt 1: B becomes shift(B)), so the distance is recomputed every step, and
+# a Poisson source that stops changing after the first step (its later solves are skipped).
+MOVING_LAW = """field A init 0 range -1000000000 1000000000
+field B init 0 range -1000000000 1000000000
+field S init 0 range -1000000000 1000000000 static
+term t
+  d B = shift(B, 1, 0, 0) - B
+  d A = 0.01 * distance(B) + 0.001 * poisson(S, 4)
+"""
 
 
 # Heston by ADI, one option a z-plane (its model in static fields), x = ln S along x, var = sv (y + 1/2) h along y.
@@ -316,6 +331,9 @@ CASES = [
     dict(name="op_" + key, law=op_law(expr), grid=G_OPS, dt=0.02, steps=4, params={},
          init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))
     for i, (key, expr) in enumerate(OPS.items())
+] + [
+    dict(name="op_distance_moving", law=MOVING_LAW, grid=(12, 7, 5, 0.25), dt=1.0, steps=5, params={},
+         init=dict(_ops_init(12 * 7 * 5, 150), S=_uniform(12 * 7 * 5, 151, -1.0, 1.0))),
 ] + [
     dict(grid=G_OPS, steps=4, params={}, init=_ops_init(int(np.prod(G_OPS[:3])), 200 + i), **dict(dict(dt=0.02), **case))
     for i, case in enumerate(NET_CASES)
