@@ -75,6 +75,10 @@ int fe_solver_set_param(fe_solver* s, const char* name, double value); /* -1: no
 /* One step of dt: every term in order (substeps from max_dt), then every derive (Solver::step, Solver.hpp:149). 0, or
  * nonzero with the reason in fe_solver_error(). Contract breaches are not errors: see fe_solver_violations_json. */
 int fe_solver_step(fe_solver* s, double dt);
+/* `steps` steps of dt in one call: the state that many fe_solver_step calls leave, violations gathered over all of
+ * them. No counterpart in the reference (its callers loop over Solver::step): with lazy readback, pairs of steps that
+ * need no contract check and no read-back run as one captured graph, replayed. */
+int fe_solver_run(fe_solver* s, double dt, int steps);
 int fe_solver_refresh_derived(fe_solver* s);               /* Solver::refresh_derived (Solver.hpp:148) */
 int fe_solver_set_lazy_readback(fe_solver* s, int on);     /* Solver::set_lazy_readback (Solver.hpp:138) */
 int fe_solver_check_contracts_every(fe_solver* s, int steps); /* Solver::check_contracts_every (Solver.hpp:171) */

@@ -42,9 +42,10 @@ int fe_lnet_step(fe_lnet* n, const float* u, const float* v, int nx, int ny, int
     double dt, int learned, double bound, float* u2, float* v2);
 int fe_lnet_fallbacks(fe_lnet* n, unsigned* out);
 /* The loss's gradient for the pair (s0 -> s1 over dt) into the net's gradient buffer (and grad_out, loss_out when not
- * NULL: then it waits for the card). Fixed-order sums: the same bits every run. */
+ * NULL: then it waits for the card). The loss is `weight` times the mean squared miss. Fixed-order sums: the same bits
+ * every run. */
 int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* u1, const float* v1, int nx, int ny, int count, double h,
-    double du, double dv, double k, double dt, float* grad_out, float* loss_out);
+    double du, double dv, double k, double dt, double weight, float* grad_out, float* loss_out);
 /* Adam on the last gradient (beta 0.9, 0.999, eps 1e-8), each update clipped to +-clip; repacks the fragments. */
 int fe_lnet_adam(fe_lnet* n, double lr, double clip);
 /* Block averages of `factor` x `factor` cells, lattice by lattice (coarse: count * (fnx/factor) * (fny/factor)). */

@@ -152,6 +152,11 @@ int fe_solver_step(fe_solver* s, double dt)
     return guarded(s, [&] { s->solver->step(dt); });
 }
 
+int fe_solver_run(fe_solver* s, double dt, int steps)
+{
+    return guarded(s, [&] { s->solver->run(dt, steps); });
+}
+
 int fe_solver_refresh_derived(fe_solver* s)
 {
     return guarded(s, [&] { s->solver->refresh_derived(); });

@@ -109,6 +109,20 @@ public:
     virtual void step(double dt, bool respect_max_dt, bool learned, Law& law, FieldSet& fields, FieldEdits& edits,
         std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps) = 0;
     virtual void derive(Law& law, FieldSet& fields, FieldEdits& edits, std::vector<char>& changed) = 0;
+    // `steps` steps of dt, as that many calls of step() take them: changed is the union over the steps, violations all
+    // of theirs, substeps the last step's. No counterpart in the reference: a backend may replay what it captured.
+    virtual void run(double dt, int steps, bool respect_max_dt, bool learned, Law& law, FieldSet& fields, FieldEdits& edits,
+        std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps)
+    {
+        std::vector<char> any(law.fields.size(), 0);
+        for (int i = 0; i < steps; ++i) {
+            step(dt, respect_max_dt, learned, law, fields, edits, changed, violations, substeps);
+            for (std::size_t f = 0; f < changed.size() && f < any.size(); ++f) {
+                any[f] = any[f] || changed[f];
+            }
+        }
+        changed = any;
+    }
     virtual void check_contracts_every(int steps) = 0;
     virtual void set_lazy(bool on) { (void)on; }
     virtual void fetch(FieldSet& fields, const std::vector<char>& which) { (void)fields; (void)which; }

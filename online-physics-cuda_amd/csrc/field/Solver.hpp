@@ -41,6 +41,9 @@ public:
 
     void refresh_derived();
     void step(double dt);
+    // `steps` steps of dt in one call (the backend may replay them from a captured graph); the same state as that many
+    // step() calls, violations gathered over all of them.
+    void run(double dt, int steps);
     void set_lazy_readback(bool on);
     [[nodiscard]] bool lazy_readback() const { return lazy_; }
     void check_contracts_every(int steps);

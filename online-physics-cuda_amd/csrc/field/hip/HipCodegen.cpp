@@ -545,7 +545,7 @@ private:
               << "    return fe_rf(s);\n";
             break;
         }
-        case Op::Tridiag:
+        case Op::Tridiag: case Op::Distance: case Op::Poisson: // computed over the lattice by their stage kernels
             f << "    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n";
             break;
         case Op::Fall: {
@@ -806,8 +806,8 @@ private:
         case Op::Clamp: return "([](double q, double lo, double hi) { return fe_clamp(q, lo, fe_max(lo, hi)); })(" + a(0) + ", " +
             a(1) + ", " + a(2) + ")";
         default:
-            (void)id;
-            return "0.0";
+            throw std::runtime_error(std::string("no element-wise form for ") + ir::op_name(in.op) + " (instruction " +
+                std::to_string(id) + ")");
         }
     }
 
@@ -1366,7 +1366,12 @@ private:
 std::string generate(const ir::Module& m, const Law& law, LawLayout& layout, std::string& why)
 {
     layout = LawLayout{};
-    return Generator(m, law, layout).run(why);
+    try {
+        return Generator(m, law, layout).run(why);
+    } catch (const std::exception& e) {
+        why = e.what();
+        return "";
+    }
 }
 
 std::vector<float> net_blob(const Law& law, std::vector<std::size_t>& offsets)

@@ -102,6 +102,7 @@ struct Impl final : LawBackend {
 
     ~Impl() override
     {
+        drop_pairs();
         for (fe_hip_event e : events) {
             fe_hip_event_destroy(e);
         }
@@ -780,6 +781,173 @@ struct Impl final : LawBackend {
         std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps) override
     {
         const std::lock_guard<std::mutex> lock(device_mutex());
+        step_locked(dt, respect_max_dt, learned, l, fields, edits, changed, violations, substeps);
+    }
+
+    // Captured runs of plain steps: a graph of an even number of whole steps (every ping-pong field back where it
+    // began), keyed by where the fields lie when it starts, its length, dt, max_dt's use and the params its launches
+    // carry by value. Runs of kLongRun steps where no contract falls due, pairs otherwise.
+    static constexpr int kLongRun = 32;
+    struct Pair {
+        std::vector<void*> start;
+        int steps = 2;
+        double dt = 0.0;
+        bool respect = true;
+        std::vector<double> params;
+        fe_hip_graph graph = nullptr;
+        std::vector<int> substeps;
+    };
+    std::vector<Pair> pairs;
+
+    void drop_pairs()
+    {
+        for (Pair& q : pairs) {
+            if (q.graph != nullptr) {
+                fe_hip_graph_destroy(q.graph);
+            }
+        }
+        pairs.clear();
+    }
+
+    // A step the card can take from a graph: lazy, no contracts due, no guarded learned term, and substeps that need
+    // no reduction read back (max_dt of params, h and constants only, or not respected).
+    bool replayable(bool respect, bool learned, const FieldEdits& edits) const
+    {
+        if (!lazy || edits.all || edits.shifted()) {
+            return false;
+        }
+        for (const std::vector<char>& planes : edits.planes) {
+            if (std::any_of(planes.begin(), planes.end(), [](char c) { return c != 0; })) {
+                return false;
+            }
+        }
+        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+            if (learned && !module.learned[t].equations.empty()) {
+                return false;
+            }
+            if (respect && layout.terms[t].max_dt >= 0 && !layout.terms[t].limit_stages.empty()) {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    void run(double dt, int steps, bool respect, bool learned, Law& l, FieldSet& fields, FieldEdits& edits,
+        std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps) override
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        law = &l;
+        const std::size_t nf = law->fields.size();
+        std::vector<char> any(nf, 0);
+        auto merge = [&] {
+            for (std::size_t f = 0; f < nf && f < changed.size(); ++f) {
+                any[f] = any[f] || changed[f];
+            }
+        };
+        int i = 0;
+        while (i < steps) {
+            // steps from here before the next one that checks contracts (step_count % check_every == 0)
+            const long long every = check_every;
+            const long long due = ((step_count + every - 1) / every) * every;
+            const long long free_steps = std::min<long long>(due - step_count, steps - i);
+            const int length = free_steps >= kLongRun ? kLongRun : (free_steps >= 2 ? 2 : 0);
+            if (length > 0 && replayable(respect, learned, edits)) {
+                replay_pair(dt, respect, length, substeps);
+                step_count += length;
+                changed.assign(nf, 0);
+                for (const auto* progs : {&module.terms, &module.derives}) {
+                    for (const ir::Program& p : *progs) {
+                        for (const int f : p.writes) {
+                            changed[static_cast<std::size_t>(f)] = 1;
+                        }
+                    }
+                }
+                for (std::size_t f = 0; f < nf; ++f) {
+                    stale[f] = stale[f] || changed[f];
+                }
+                merge();
+                i += length;
+                continue;
+            }
+            step_locked(dt, respect, learned, l, fields, edits, changed, violations, substeps);
+            merge();
+            i += 1;
+        }
+        changed = any;
+    }
+
+    // `length` (even) plain steps from the graph captured for where the fields lie now (captured first if there is
+    // none).
+    void replay_pair(double dt, bool respect, int length, std::vector<int>& substeps)
+    {
+        upload_params();
+        Pair* hit = nullptr;
+        for (Pair& q : pairs) {
+            if (q.start == now && q.steps == length && q.dt == dt && q.respect == respect && q.params == law->param_values) {
+                hit = &q;
+            }
+        }
+        if (hit == nullptr) {
+            if (pairs.size() >= 8) {
+                drop_pairs();
+            }
+            Pair q;
+            q.start = now;
+            q.steps = length;
+            q.dt = dt;
+            q.respect = respect;
+            q.params = law->param_values;
+            const std::vector<void*> now0 = now, nxt0 = nxt;
+            check(fe_hip_graph_begin_capture(), "graph capture");
+            try {
+                for (int s = 0; s < length; ++s) {
+                    plain_step(dt, respect, q.substeps);
+                }
+            } catch (...) {
+                fe_hip_graph end = nullptr;
+                if (fe_hip_graph_end_capture(&end) == 0 && end != nullptr) {
+                    fe_hip_graph_destroy(end);
+                }
+                now = now0;
+                nxt = nxt0;
+                throw;
+            }
+            check(fe_hip_graph_end_capture(&q.graph), "graph capture");
+            if (now != now0) {
+                throw std::runtime_error("a captured run of steps did not bring the fields back where they began");
+            }
+            pairs.push_back(std::move(q));
+            hit = &pairs.back();
+        }
+        check(fe_hip_graph_launch(hit->graph), "graph launch");
+        substeps = hit->substeps;
+        prof = StepProfile{};
+        prof.backend = "HIP";
+        prof.terms.resize(module.terms.size());
+        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+            prof.terms[t].name = module.terms[t].name;
+            prof.terms[t].substeps = t < substeps.size() ? substeps[t] : 1;
+            prof.terms[t].ms = -1.0; // not timed: the steps ran as one graph
+        }
+        timings_pending = false;
+    }
+
+    // One step's launches and nothing else: no contracts, no events, no waits (what a graph captures).
+    void plain_step(double dt, bool respect, std::vector<int>& substeps)
+    {
+        std::vector<Violation> none;
+        std::vector<Judge> judges;
+        int slot = 0;
+        substeps.assign(module.terms.size(), 1);
+        for (std::size_t t = 0; t < module.terms.size(); ++t) {
+            substeps[t] = run_program(module.terms[t], layout.terms[t], module.terms[t].name, dt, respect, false, none, judges, slot);
+        }
+        run_derives(false, judges);
+    }
+
+    void step_locked(double dt, bool respect_max_dt, bool learned, Law& l, FieldSet& fields, FieldEdits& edits,
+        std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps)
+    {
         const auto t0 = std::chrono::steady_clock::now();
         law = &l;
         prof = StepProfile{};

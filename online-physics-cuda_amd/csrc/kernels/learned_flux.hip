@@ -334,7 +334,7 @@ __global__ void k_clear(unsigned* p, int n)
 // plain(s0); e = dL/dcorrection into E (two a cell), and the loss's terms into loss_parts (one a cell, summed in order
 // by k_sum).
 __global__ void k_residual(const float* __restrict__ u0, const float* __restrict__ v0, const float* __restrict__ u1, const float* __restrict__ v1,
-    const float* __restrict__ F, Lattice L, Reaction R, float dt, float* __restrict__ E, float* __restrict__ loss_parts)
+    const float* __restrict__ F, Lattice L, Reaction R, float dt, float weight, float* __restrict__ E, float* __restrict__ loss_parts)
 {
     const long long cells = (long long)L.nx * L.ny;
     const long long n = cells * L.count;
@@ -349,7 +349,7 @@ __global__ void k_residual(const float* __restrict__ u0, const float* __restrict
     correction(F, L, lat, x, y, cu, cv);
     const float tu = (u1[i] - u0[i]) / dt - ru, tv = (v1[i] - v0[i]) / dt - rv;
     const float du = cu - tu, dv = cv - tv;
-    const float inv = 1.0f / (2.0f * (float)n);
+    const float inv = weight / (2.0f * (float)n);
     E[2 * i] = 2.0f * du * inv;
     E[2 * i + 1] = 2.0f * dv * inv;
     loss_parts[i] = (du * du + dv * dv) * inv;
@@ -816,9 +816,10 @@ int fe_lnet_fallbacks(fe_lnet* n, unsigned* out)
 }
 
 int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* u1, const float* v1, int nx, int ny, int count, double h,
-    double du, double dv, double k, double dt, float* grad_out, float* loss_out)
+    double du, double dv, double k, double dt, double weight, float* grad_out, float* loss_out)
 {
-    if (n == nullptr || u0 == nullptr || v0 == nullptr || u1 == nullptr || v1 == nullptr || nx < 2 || ny < 2 || count < 1) {
+    if (n == nullptr || u0 == nullptr || v0 == nullptr || u1 == nullptr || v1 == nullptr || nx < 2 || ny < 2 || count < 1 ||
+        !(weight > 0.0)) {
         return -1;
     }
     try {
@@ -833,8 +834,8 @@ int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* 
         const int waves = fe_lnet::kTrainBlocks * kWavesTrain;
         n->partials.alloc(static_cast<std::size_t>(waves) * kParams);
         hipLaunchKernelGGL(k_faces, dim3(fe_lnet::kFaceBlocks), dim3(256), 0, s, n->frags.p, n->params.p, u0, v0, L, n->scale, faces, n->F.p);
-        hipLaunchKernelGGL(k_residual, dim3(blocks_for(cells, 256)), dim3(256), 0, s, u0, v0, u1, v1, n->F.p, L, R, static_cast<float>(dt), n->E.p,
-            n->loss_parts.p);
+        hipLaunchKernelGGL(k_residual, dim3(blocks_for(cells, 256)), dim3(256), 0, s, u0, v0, u1, v1, n->F.p, L, R, static_cast<float>(dt),
+            static_cast<float>(weight), n->E.p, n->loss_parts.p);
         hipLaunchKernelGGL(k_backward, dim3(fe_lnet::kTrainBlocks), dim3(64 * kWavesTrain), 0, s, n->frags.p, n->params.p, u0, v0, n->E.p, L,
             n->scale, faces, n->partials.p);
         hipLaunchKernelGGL(k_sum, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, n->partials.p, waves, kParams, n->grad.p);

@@ -81,6 +81,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_get_field": (c_int, [c_void_p, c_int, c_void_p]),
         "fe_solver_set_param": (c_int, [c_void_p, ctypes.c_char_p, c_double]),
         "fe_solver_step": (c_int, [c_void_p, c_double]),
+        "fe_solver_run": (c_int, [c_void_p, c_double, c_int]),
         "fe_solver_refresh_derived": (c_int, [c_void_p]),
         "fe_solver_set_lazy_readback": (c_int, [c_void_p, c_int]),
         "fe_solver_check_contracts_every": (c_int, [c_void_p, c_int]),
@@ -124,7 +125,7 @@ def _declare(lib: ctypes.CDLL) -> None:
                                  c_int, c_double, c_void_p, c_void_p]),
         "fe_lnet_fallbacks": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint)]),
         "fe_lnet_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
-                                     c_double, c_double, c_double, c_void_p, c_void_p]),
+                                     c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
         "fe_lnet_adam": (c_int, [c_void_p, c_double, c_double]),
         "fe_lnet_coarsen": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     }
@@ -394,6 +395,10 @@ class Solver:
     def step(self, dt: float) -> None:
         self._check(lib().fe_solver_step(self.ptr, float(dt)), "step")
 
+    def run(self, dt: float, steps: int) -> None:
+        """`steps` steps of dt in one call (with lazy readback, pairs of plain steps replay a captured graph)."""
+        self._check(lib().fe_solver_run(self.ptr, float(dt), int(steps)), "run")
+
     def refresh_derived(self) -> None:
         self._check(lib().fe_solver_refresh_derived(self.ptr), "refresh_derived")
 
@@ -583,13 +588,14 @@ class LearnedNet:
         self._check(lib().fe_lnet_fallbacks(self.ptr, ctypes.byref(c)), "fallbacks")
         return int(c.value)
 
-    def gradient(self, u0, v0, u1, v1, nx, ny, count, h, du, dv, k, dt, want=True):
+    def gradient(self, u0, v0, u1, v1, nx, ny, count, h, du, dv, k, dt, want=True, weight=1.0):
         import numpy as np
 
         g = np.empty(lib().fe_lnet_param_count(), np.float32) if want else None
         loss = np.empty(1, np.float32) if want else None
         self._check(lib().fe_lnet_gradient(self.ptr, self._p(u0), self._p(v0), self._p(u1), self._p(v1), nx, ny, count, h, du, dv,
-                                           k, dt, g.ctypes.data if want else None, loss.ctypes.data if want else None), "gradient")
+                                           k, dt, float(weight), g.ctypes.data if want else None,
+                                           loss.ctypes.data if want else None), "gradient")
         return (float(loss[0]), g) if want else None
 
     def adam(self, lr: float, clip: float = 1e-2) -> None:

@@ -157,20 +157,46 @@ def target_scale(data: Dataset, seeds: int = 4) -> float:
     return float(s.h_coarse * np.sqrt(0.5 * (np.mean(tu ** 2) + np.mean(tv ** 2))))
 
 
+def initial_params(seed: int, last_scale: float = 0.0) -> np.ndarray:
+    """The net's starting weights in the layout of include/fe_learned.h: per layer W (fan_in x fan_out, row-major) then
+    b; Glorot-uniform W, zero b, the last layer times last_scale (0: the net starts at no correction)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out, scale in ((18, 64, 1.0), (64, 64, 1.0), (64, 2, last_scale)):
+        lim = np.sqrt(6.0 / (fan_in + fan_out))
+        parts.append((rng.uniform(-lim, lim, (fan_in, fan_out)) * scale).reshape(-1))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def plain_losses(data: Dataset, seeds: slice, scale: float) -> np.ndarray:
+    """The loss of the plain coarse law (the net at no correction) on each snapshot pair t -> t + 1 of `seeds`."""
+    probe = field_hip.LearnedNet(initial_params(1, 0.0), scale)
+    s = data.setup
+    c = s.coarse
+    count = seeds.stop - seeds.start
+    out = np.empty(s.snapshots)
+    for t in range(s.snapshots):
+        out[t], _ = probe.gradient(data.at("u", t, seeds.start), data.at("v", t, seeds.start), data.at("u", t + 1, seeds.start),
+                                   data.at("v", t + 1, seeds.start), c, c, count, s.h_coarse, s.du, s.dv, s.k, s.dt_coarse)
+    return out
+
+
 def train(net: field_hip.LearnedNet, data: Dataset, steps: int, batch: int, lr: float, train_seeds: int, seed: int = 0,
-          log_every: int = 0, clip: float = 1e-2):
-    """Adam on random (snapshot, block of `batch` seeds among the first `train_seeds`) pairs; lr decays by 10 over the
-    run. Returns the losses logged."""
+          log_every: int = 0, clip: float = 1e-2, first: int = 0, weights=None):
+    """Adam on random (snapshot t >= first, block of `batch` seeds among the first `train_seeds`) pairs, pair t's loss
+    times weights[t] when given; lr decays by 10 over the run. Returns the losses logged."""
     rng = np.random.default_rng(seed)
     s = data.setup
     c = s.coarse
     losses = []
     for i in range(steps):
-        t = int(rng.integers(0, s.snapshots))
+        t = int(rng.integers(first, s.snapshots))
         s0 = int(rng.integers(0, train_seeds - batch + 1))
         want = bool(log_every) and (i % log_every == 0 or i == steps - 1)
         out = net.gradient(data.at("u", t, s0), data.at("v", t, s0), data.at("u", t + 1, s0), data.at("v", t + 1, s0), c, c, batch,
-                           s.h_coarse, s.du, s.dv, s.k, s.dt_coarse, want=want)
+                           s.h_coarse, s.du, s.dv, s.k, s.dt_coarse, want=want,
+                           weight=1.0 if weights is None else float(weights[t]))
         if want:
             losses.append((i, out[0]))
         net.adam(lr * 10 ** (-i / max(1, steps)), clip)

@@ -281,6 +281,31 @@ def net_law(decls: str, body: str, a_range: str = "-1000000000 1000000000") -> s
             "param kk = 0.3\nterm t\n" + body)
 
 
+# Many ops in one law: stages (distance, poisson, a net flux, a tridiagonal solve, reductions) beside inlined stencils,
+# several terms, derives and a clamped field, as a world-style law mixes them.
+KITCHEN_LAW = """net g in 4 hidden 6 out 1 act relu
+field rock init 0 range 0 1 static
+field water init 0 range 0 1
+field heat init 0 range -1000 1000
+field smoke init 0 range 0 2 clamp
+derive wet = water * (1 - rock)
+derive wet += 0.1 * smoke
+
+param kh = 0.2
+param g0 = 1.5
+
+term flow
+  let room = 1 - rock - water
+  d water = fall(water, rock, g0) + spread(water, rock, 0.1) + topple(water, rock, 1) + 0.05 * flux(g, water, heat)
+  d smoke = 0.02 * room * pos(heat - 0.5) - 0.01 * smoke + 0.1 * lap(smoke)
+  conserves water
+  max_dt = 0.5 * h
+
+term warm
+  d heat = diffuse(kh + 0.1 * rock, heat) + adv(heat, 0.2, -0.1, 0.05) - 0.01 * distance(rock) + 0.001 * poisson(water - 0.3, 6)
+  d heat = 0.001 * (maxof(water) - sumof(smoke) / 1000) + 0.01 * tridiag_x(0 - 0.1, 1.2, 0 - 0.1, heat) - 0.01 * heat
+"""
+
 # Law nets (src/field/Law.cpp:1298-1443). relu and linear layers give the CPU's bits; tanh and silu go through the
 # card's tanhf / expf, held to a tolerance (atol) instead.
 NET_CASES = [
@@ -332,6 +357,11 @@ CASES = [
          init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))
     for i, (key, expr) in enumerate(OPS.items())
 ] + [
+    dict(name="kitchen", law=KITCHEN_LAW, grid=(24, 18, 12, 0.5), dt=0.1, steps=6, params={},
+         nets={"g": net_text("g", [4, 6, 1], "relu", 40, 0.8, [0.5, 0.2, 0.5, 0.2])},
+         init={"rock": (np.random.default_rng(41).random(24 * 18 * 12) < 0.3).astype(np.float32),
+               "water": _uniform(24 * 18 * 12, 42, 0.0, 0.6), "heat": _uniform(24 * 18 * 12, 43, -1.0, 2.0),
+               "smoke": _uniform(24 * 18 * 12, 44, 0.0, 0.5)}),
     dict(name="op_distance_moving", law=MOVING_LAW, grid=(12, 7, 5, 0.25), dt=1.0, steps=5, params={},
          init=dict(_ops_init(12 * 7 * 5, 150), S=_uniform(12 * 7 * 5, 151, -1.0, 1.0))),
 ] + [
