@@ -218,6 +218,7 @@ public:
              << "    __device__ explicit Direct(const K& k) { for (int q = 0; q < " << nu << "; ++q) u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
              << "; }\n"
              << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
+             << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
              << "};\n";
         std::vector<std::size_t> offsets;
         (void)net_blob(law_, offsets);
@@ -341,10 +342,20 @@ private:
         return std::string(is_leaf ? leaf[a] : direct[a]) + v + ")";
     }
 
-    static std::string v(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z")
+    // A value at a point. Where the point is a runtime coordinate (ops that walk or choose their cells), a field is
+    // read through vr<id>: straight from the window, without the checks that serve known offsets from registers.
+    std::string v(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z") const
     {
-        return "v" + std::to_string(id) + "(k, m, " + x + ", " + y + ", " + z + ")";
+        const bool leaf = at(id).op == Op::Field && lay_.stages.count(id) == 0;
+        return std::string(leaf && runtime_ > 0 ? "vr" : "v") + std::to_string(id) + "(k, m, " + x + ", " + y + ", " + z + ")";
     }
+
+    struct Runtime {
+        int& n;
+        explicit Runtime(int& c) : n(c) { ++n; }
+        ~Runtime() { --n; }
+    };
+    int runtime_ = 0;
 
     // An argument as a double at a point: a scalar as computed, a cell value widened from its float.
     std::string d(int id, const std::string& x = "x", const std::string& y = "y", const std::string& z = "z") const
@@ -396,7 +407,9 @@ private:
         f << head << "\n{\n";
         switch (in.op) {
         case Op::Field:
-            f << "    return m.template fld<" << in.index << ">(k, x, y, z);\n";
+            f << "    return m.template fld<" << in.index << ">(k, x, y, z);\n}\n"
+              << "template <class M> __device__ __forceinline__ float vr" << n << "(const K& k, const M& m, int x, int y, int z)\n{\n"
+              << "    return m.template fld_any<" << in.index << ">(k, x, y, z);\n";
             break;
         case Op::Rim:
             f << "    return (x == 0 || y == 0 || z == 0 || x + 1 == k.nx || y + 1 == k.ny || z + 1 == k.nz) ? 1.0f : 0.0f;\n";
@@ -522,6 +535,7 @@ private:
             break;
         }
         case Op::ShiftT: {
+            const Runtime walk(runtime_);
             f << "    int lx, hx, ly, hy, lz, hz;\n"
               << "    run_to(x, " << in.offset[0] << ", k.nx, lx, hx); run_to(y, " << in.offset[1] << ", k.ny, ly, hy); run_to(z, "
               << in.offset[2] << ", k.nz, lz, hz);\n"
@@ -532,6 +546,7 @@ private:
             break;
         }
         case Op::GxT: case Op::GyT: case Op::GzT: {
+            const Runtime walk(runtime_);
             const int axis = in.op == Op::GxT ? 0 : (in.op == Op::GyT ? 1 : 2);
             const char* c = axis == 0 ? "x" : (axis == 1 ? "y" : "z");
             const char* cnt = axis == 0 ? "k.nx" : (axis == 1 ? "k.ny" : "k.nz");
@@ -549,6 +564,7 @@ private:
             f << "    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n";
             break;
         case Op::Fall: {
+            const Runtime walk(runtime_);
             // F moving down into the room below (src/field/Law.cpp:1183-1210): the flux out of a cell through its
             // floor, in double; the cell loses its own and gains the one above's.
             if (scalar(in.args[0])) {
@@ -568,6 +584,7 @@ private:
             break;
         }
         case Op::Spread: {
+            const Runtime walk(runtime_);
             // Sideways, downhill, into the room on the receiving side (src/field/Law.cpp:1212-1244): the flux across
             // each x and y face in double, summed into the cell in the CPU's order, rounding to float at each step.
             if (scalar(in.args[0])) {
@@ -591,6 +608,7 @@ private:
             break;
         }
         case Op::Topple: {
+            const Runtime walk(runtime_);
             // A pile slides diagonally down off what holds it (src/field/Law.cpp:1246-1296): what the cell sends down
             // its four sides, and what the four cells above-and-beside send toward this column, in double.
             if (scalar(in.args[0])) {
@@ -633,9 +651,11 @@ private:
             f << "    float yout[" << net.outputs() << "];\n    net_" << in.index << "(k, xin, yout);\n    return yout[0];\n";
             break;
         }
-        case Op::Flux: case Op::LDiffuse:
+        case Op::Flux: case Op::LDiffuse: {
+            const Runtime walk(runtime_);
             net_faces(in, f);
             break;
+        }
         default:
             f << "    return fe_rf(" << arith(in, id) << ");\n";
             break;
@@ -1209,29 +1229,59 @@ private:
         // into one wide LDS read.
         s << "__shared__ __attribute__((aligned(16))) float " << LDS << "[" << NW * RF << "];\n";
         // Read through the window in the fast forms (a flagged cell is taken again through WX, the exact forms).
+        // A read at an offset from the lane's own cells that the compiler knows (a stencil's fixed neighbours, once
+        // inlined) is served from registers when it lies in the lane's own column: the cells the lane loaded itself
+        // for planes z-R..z+R (`own`), as the LDS ring holds them. Every other read goes through the ring.
+        const int NOWN = 2 * R + 1;
         s << "template <bool P2> struct " << W << " {\n    static constexpr bool kClamps = false;\n    static constexpr bool kExact = false;\n"
           << "    static constexpr bool kPow2 = P2;\n"
-          << "    int x0; int y0; mutable bool bad; double u[" << std::max(1, lay_.uniforms) << "];\n"
+          << "    int x0; int y0; int xc; int yc; int zc; mutable bool bad; double u[" << std::max(1, lay_.uniforms) << "];\n"
+          << "    float own[" << NW << "][" << NOWN << "][" << VX << "];\n"
           << "    __device__ __forceinline__ void flag(bool b) const { bad = bad || b; }\n"
           << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const {\n";
+        for (int w = 0; w < NW; ++w) {
+            // Selects, not an index: a known offset folds to one register, an unknown one costs selects, never
+            // private memory.
+            s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") {\n"
+              << "            const int dx = x - xc, dy = y - yc, dz = z - zc;\n"
+              << "            if (dy == 0 && (unsigned)(dz + " << R << ") <= " << 2 * R << "u && (unsigned)dx < " << VX << "u) {\n"
+              << "                float v = own[" << w << "][0][0];\n"
+              << "#pragma unroll\n                for (int j = 0; j < " << 2 * R + 1 << "; ++j)\n"
+              << "#pragma unroll\n                    for (int c = 0; c < " << VX << "; ++c) v = (dz + " << R << " == j && dx == c) ? own[" << w
+              << "][j][c] : v;\n"
+              << "                return v;\n            }\n"
+              << "            return " << LDS << "[" << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R
+              << ") * " << TW << " + (x - x0 + " << X0 << ")];\n        }\n";
+        }
+        s << "        else return k.now[F][IDX(x, y, z)];\n    }\n"
+          << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const {\n";
         for (int w = 0; w < NW; ++w) {
             s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return " << LDS << "["
               << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + "
               << X0 << ")];\n";
         }
         s << "        else return k.now[F][IDX(x, y, z)];\n    }\n};\n";
+        // The exact view holds a copy of the window's state, not its address: an address taken would keep the
+        // window's registers (its own cells, its offsets) in memory.
         s << "template <bool P2> struct WX_" << name << " {\n    static constexpr bool kClamps = false;\n    static constexpr bool kExact = true;\n"
           << "    static constexpr bool kPow2 = P2;\n"
-          << "    const " << W << "<P2>& w; const double* u;\n    __device__ void flag(bool) const {}\n"
+          << "    " << W << "<P2> w; double u[" << std::max(1, lay_.uniforms) << "];\n"
+          << "    __device__ explicit WX_" << name << "(const " << W << "<P2>& src) : w(src) { for (int q = 0; q < "
+          << std::max(1, lay_.uniforms) << "; ++q) u[q] = src.u[q]; }\n"
+          << "    __device__ void flag(bool) const {}\n"
           << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return w.template fld<F>(k, x, y, z); }\n"
+          << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const { return w.template fld_any<F>(k, x, y, z); }\n"
           << "};\n";
         // The kernel's body, instantiated for "h and h^2 are powers of two" (their divisions are multiplications,
         // with no branch between cells) and for the general case.
         s << "template <bool P2> __device__ __forceinline__ void run_" << name
-          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
+          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK) {\n"
           << "    float* const lds = " << LDS << ";\n"
           << "    const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z);\n"
-          << "    const int x0 = (t % tiles_x) * " << SPAN << "; const int y0 = ((t / tiles_x) % tiles_y) * " << TY << ";\n"
+          // x0 passes through an empty asm: the compiler then knows nothing of its low bits, so a neighbour's x + c
+          // stays an add (not an or), and its offset from the lane's own cell, x + c - x, folds to c.
+          << "    int x0 = (t % tiles_x) * " << SPAN << "; asm volatile(\"\" : \"+s\"(x0)); const int y0 = ((t / tiles_x) % tiles_y) * "
+          << TY << ";\n"
           << "    const int z0 = (t / (tiles_x * tiles_y)) * zchunk; const int z1 = z0 + zchunk < k.nz ? z0 + zchunk : k.nz;\n"
           << "    const int lane = (int)threadIdx.x; const int r = (int)threadIdx.y;\n"
           << "    const int x = x0 + " << VX << " * lane; const int y = y0 + r - " << R << ";\n"
@@ -1239,7 +1289,7 @@ private:
           << "    const int ys = clampi(y, 0, k.ny - 1);\n"
           << "    const i64 rowo = (i64)ys * k.nx;\n"
           << "    const int zlim = z1 - 1 + " << R << " < k.nz - 1 ? z1 - 1 + " << R << " : k.nz - 1;\n"
-          << "    " << W << "<P2> m; m.x0 = x0; m.y0 = y0;\n"
+          << "    " << W << "<P2> m; m.x0 = x0; m.y0 = y0; m.xc = x; m.yc = y;\n"
           << "    for (int q = 0; q < " << std::max(1, lay_.uniforms) << "; ++q) m.u[q] = " << (lay_.uniforms > 0 ? "k.U[q]" : "0.0")
           << ";\n"
           << "    float* row = lds + r * " << TW << ";\n";
@@ -1260,6 +1310,11 @@ private:
             for (int w = 0; w < NW; ++w) {
                 s << "        sl[" << w * RF << " + " << R << " + lane] = p.v[" << w << "]; if (hslot >= 0) sl[" << w * RF << " + hslot] = p.h["
                   << w << "];\n";
+            }
+            s << "    };\n"
+              << "    auto keep = [&](const Pl& p, const int slot) {\n";
+            for (int w = 0; w < NW; ++w) {
+                s << "        m.own[" << w << "][slot][0] = p.v[" << w << "];\n";
             }
             s << "    };\n";
         } else {
@@ -1286,27 +1341,35 @@ private:
                   << "          *reinterpret_cast<float4*>(sl + " << w * RF << " + 4 + 4 * lane) = v;\n"
                   << "          if (hl) for (int j = 0; j < " << R << "; ++j) sl[" << w * RF << " + hpos + j] = p.h[" << w << "][j]; }\n";
             }
+            s << "    };\n"
+              << "    auto keep = [&](const Pl& p, const int slot) {\n";
+            for (int w = 0; w < NW; ++w) {
+                s << "        m.own[" << w << "][slot][0] = p.v[" << w << "].x; m.own[" << w << "][slot][1] = p.v[" << w << "].y; m.own[" << w
+                  << "][slot][2] = p.v[" << w << "].z; m.own[" << w << "][slot][3] = p.v[" << w << "].w;\n";
+            }
             s << "    };\n";
         }
         // The cells of a lane: their new values and the stores.
         std::ostringstream body;
         body << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
-        body << "            m.bad = false;\n";
+        body << "            m.bad = false; m.zc = z;\n";
         for (int c = 0; c < VX; ++c) {
             body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
-        body << "            if (m.bad) {\n                const WX_" << name << "<P2> mx{m, m.u};\n";
+        body << "            if (m.bad) {\n                const WX_" << name << "<P2> mx(m);\n";
         for (int c = 0; c < VX; ++c) {
             body << "                " << cellfn << "(k, mx, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
         body << "            }\n";
         body << "            const i64 i = IDX(x, y, z);\n";
+        // Change tracking (TRACK) only where the step reads the flags back; an in-place store still skips unchanged
+        // cells.
         for (int q = 0; q < NT; ++q) {
             const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
             body << "            { bool ch = false;\n";
             for (int c = 0; c < VX; ++c) {
-                body << "              { const bool d = __float_as_uint(o[" << c << "][" << q << "]) != __float_as_uint(n[" << c << "][" << q
-                     << "]); ch = ch || d;";
+                body << "              if (TRACK || " << (in_place ? "true" : "false") << ") { const bool d = __float_as_uint(o[" << c << "]["
+                     << q << "]) != __float_as_uint(n[" << c << "][" << q << "]); ch = ch || d;";
                 if (in_place) {
                     body << " if (d) k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "];";
                 }
@@ -1324,7 +1387,8 @@ private:
         }
         // A tile's change flags are marked once, after its last plane (a store a plane would be waited on with the loads).
         s << "    bool seen[" << std::max(1, NT) << "] = {};\n";
-        s << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); }\n"
+        s << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); keep(p, zp - z0 + " << R
+          << "); }\n"
           << "    Pl A, B, C, D;\n"
           // In this order: the loop's first wait counts A as the oldest load in flight (were A issued last, the
           // compiler would wait for every load there).
@@ -1333,8 +1397,11 @@ private:
           << "    auto step = [&](const int z, const Pl& cur, Pl& far) {\n"
           << "        load(far, z + " << R + 3 << ");\n"
           << "        put(cur, z + " << R << ");\n"
+          << "        keep(cur, " << 2 * R << ");\n"
           << "        __syncthreads();\n"
           << "        if (live) {\n" << body.str() << "        }\n"
+          << "        for (int w = 0; w < " << NW << "; ++w) for (int j = 0; j < " << 2 * R << "; ++j) for (int c = 0; c < " << VX
+          << "; ++c) m.own[w][j][c] = m.own[w][j + 1][c];\n"
           << "    };\n"
           // Whole groups of four planes, then the rest: a break inside the group would join the loop's latch, and
           // the compiler, counting the loads in flight over every path into the loop's head, would wait for nearly all.
@@ -1344,13 +1411,13 @@ private:
           << "    }\n"
           << "    if (z < z1) { step(z, A, D); if (z + 1 < z1) { step(z + 1, B, A); if (z + 2 < z1) step(z + 2, C, B); } }\n";
         for (int q = 0; q < NT; ++q) {
-            s << "    mark(k, " << targets[static_cast<std::size_t>(q)] << ", seen[" << q << "]);\n";
+            s << "    if (TRACK) mark(k, " << targets[static_cast<std::size_t>(q)] << ", seen[" << q << "]);\n";
         }
         s << "}\n";
         s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
-          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z) {\n"
-          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z);\n"
-          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z);\n}\n";
+          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int track) {\n"
+          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z, track != 0);\n"
+          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z, track != 0);\n}\n";
         out_ << s.str();
     }
 

@@ -90,6 +90,7 @@ struct Impl final : LawBackend {
     int tile_vx = 0, tile_ty = 0, zchunk_override = 0;
     bool force_direct = false;
     std::string last_gather;
+    bool tracking = true; // the gathers mark change flags (the step reads them back)
 
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
@@ -207,7 +208,8 @@ struct Impl final : LawBackend {
             throw std::runtime_error("lattice too large for one launch");
         }
         std::vector<std::uint64_t> k = args(dt);
-        void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz};
+        int track = tracking ? 1 : 0;
+        void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz, &track};
         check(fe_hip_launch(fn(w->name), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(w->block_y), 1, 0, argv),
             ("launch " + w->name).c_str());
         ++prof.launches;
@@ -938,6 +940,7 @@ struct Impl final : LawBackend {
         std::vector<Violation> none;
         std::vector<Judge> judges;
         int slot = 0;
+        tracking = false;
         substeps.assign(module.terms.size(), 1);
         for (std::size_t t = 0; t < module.terms.size(); ++t) {
             substeps[t] = run_program(module.terms[t], layout.terms[t], module.terms[t].name, dt, respect, false, none, judges, slot);
@@ -961,6 +964,7 @@ struct Impl final : LawBackend {
         // The change flags are read back unless the step is lazy and unjudged (then every field a program writes counts
         // as changed, and the flags are not cleared either).
         const bool read_flags = !lazy || contracts;
+        tracking = read_flags;
         if (read_flags) {
             check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, nf)), "change flags reset");
         }
