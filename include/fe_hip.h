@@ -57,6 +57,9 @@ void fe_hip_free(void* p);
 int fe_hip_module_load(const void* code, size_t bytes, fe_hip_module* out);
 int fe_hip_module_unload(fe_hip_module module);
 int fe_hip_module_function(fe_hip_module module, const char* name, fe_hip_function* out);
+/* What a loaded kernel holds a lane: registers (hipFuncGetAttribute NUM_REGS) and private (scratch) bytes
+ * (LOCAL_SIZE_BYTES). No counterpart in the reference: the backend uses it to choose among a term's gathers. */
+int fe_hip_function_attributes(fe_hip_function f, int* registers, int* private_bytes);
 int fe_hip_module_global(fe_hip_module module, const char* name, void** dptr, size_t* bytes);
 
 /* Device memory (Buffer, CudaApi.hpp:129-145) and page-locked host memory (Pinned, CudaApi.hpp:148-161). */

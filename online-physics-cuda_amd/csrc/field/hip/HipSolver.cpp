@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <sstream>
 #include <stdexcept>
@@ -67,6 +68,7 @@ constexpr unsigned kBlockX = 64;
 constexpr unsigned kBlockY = 4;
 constexpr int kReduceBlocks = 1024; // blocks of a reduction's first pass (its partials fold in one block)
 constexpr int kTotals = 1024;      // doubles of conserved totals judged at once
+constexpr int kMaxWindowRegisters = 168; // a lane's registers that still leave a window gather three waves a SIMD
 
 struct Impl final : LawBackend {
     Law* law = nullptr;
@@ -236,6 +238,9 @@ struct Impl final : LawBackend {
                 }
                 continue;
             }
+            if (heavy.count(w.name) != 0) {
+                continue;
+            }
             const long long tiles = 1LL * ((shape.nx + 64 * w.vx - 1) / (64 * w.vx)) * ((shape.ny + w.ty - 1) / w.ty) * tz;
             if (best == nullptr || w.vx > best->vx || (w.vx == best->vx && w.ty > best->ty && tiles >= 2048)) {
                 best = &w;
@@ -285,7 +290,26 @@ struct Impl final : LawBackend {
         const int lr = fe_hip_module_load(code, bytes, &mod);
         fe_hip_free(code);
         check(lr, "module load");
+        weigh_windows();
         allocate();
+    }
+
+    // Window gathers that would run badly: private (scratch) memory, or more registers a lane than leave two waves a
+    // SIMD (four cells a lane of a law with many fields). The automatic choice passes them over.
+    std::set<std::string> heavy;
+    void weigh_windows()
+    {
+        heavy.clear();
+        for (const auto* progs : {&layout.terms, &layout.learned}) {
+            for (const hip::ProgramCode& p : *progs) {
+                for (const hip::WindowKernel& w : p.windows) {
+                    int regs = 0, priv = 0;
+                    if (fe_hip_function_attributes(fn(w.name), &regs, &priv) == 0 && (priv > 0 || regs > kMaxWindowRegisters)) {
+                        heavy.insert(w.name);
+                    }
+                }
+            }
+        }
     }
 
     void allocate()

@@ -286,10 +286,28 @@ __global__ __launch_bounds__(256) void k_cols(const HOpt* __restrict__ opts, con
         return;
     }
     const long long nx = o.nx;
+    const int ny = o.ny;
     float* col = T1 + o.off + i;
     const HRow* cf = rows + o.row;
+    // The sweeps walk kChunk rows at a time: the chunk's loads are issued together (they do not depend on the
+    // recurrence), so a column waits for memory once a chunk instead of once a row. The arithmetic and its order are
+    // the row-by-row sweep's.
+    constexpr int kChunk = 8;
     float prev = 0.0f;
-    for (int j = 0; j < o.ny; ++j) {
+    int j = 0;
+    for (; j + kChunk <= ny; j += kChunk) {
+        float d[kChunk];
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+            d[q] = col[(j + q) * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+            prev = (d[q] - (j + q == 0 ? 0.0f : cf[j + q].ya * prev)) * cf[j + q].ym;
+            col[(j + q) * nx] = prev;
+        }
+    }
+    for (; j < ny; ++j) {
         const float dj = col[j * nx];
         prev = (dj - (j == 0 ? 0.0f : cf[j].ya * prev)) * cf[j].ym;
         col[j * nx] = prev;
@@ -297,7 +315,28 @@ __global__ __launch_bounds__(256) void k_cols(const HOpt* __restrict__ opts, con
     float next = 0.0f;
     float* d2 = D2 + o.off + i;
     float* u = U + o.off + i;
-    for (int j = o.ny - 1; j >= 0; --j) {
+    j = ny - 1;
+    for (; j - kChunk + 1 >= 0; j -= kChunk) {
+        float c[kChunk], a[kChunk], b[kChunk];
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+            c[q] = col[(j - q) * nx];
+            if (FINAL) {
+                a[q] = u[(j - q) * nx];
+                b[q] = d2[(j - q) * nx];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+            next = c[q] - cf[j - q].ycp * next;
+            if (FINAL) {
+                u[(j - q) * nx] = a[q] + (b[q] + next);
+            } else {
+                d2[(j - q) * nx] = next;
+            }
+        }
+    }
+    for (; j >= 0; --j) {
         next = col[j * nx] - cf[j].ycp * next;
         if (FINAL) {
             u[j * nx] = u[j * nx] + (d2[j * nx] + next);

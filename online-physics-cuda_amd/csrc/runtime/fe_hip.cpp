@@ -291,6 +291,19 @@ int fe_hip_module_function(fe_hip_module module, const char* name, fe_hip_functi
     return static_cast<int>(r);
 }
 
+int fe_hip_function_attributes(fe_hip_function f, int* registers, int* private_bytes)
+{
+    if (f == nullptr || registers == nullptr || private_bytes == nullptr) {
+        return static_cast<int>(hipErrorInvalidValue);
+    }
+    const hipFunction_t h = reinterpret_cast<hipFunction_t>(f);
+    hipError_t r = hipFuncGetAttribute(registers, HIP_FUNC_ATTRIBUTE_NUM_REGS, h);
+    if (r == hipSuccess) {
+        r = hipFuncGetAttribute(private_bytes, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, h);
+    }
+    return static_cast<int>(r);
+}
+
 int fe_hip_module_global(fe_hip_module module, const char* name, void** dptr, size_t* bytes)
 {
     hipDeviceptr_t p = nullptr;
