@@ -839,7 +839,9 @@ int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* 
         hipLaunchKernelGGL(k_backward, dim3(fe_lnet::kTrainBlocks), dim3(64 * kWavesTrain), 0, s, n->frags.p, n->params.p, u0, v0, n->E.p, L,
             n->scale, faces, n->partials.p);
         hipLaunchKernelGGL(k_sum, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, n->partials.p, waves, kParams, n->grad.p);
-        hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, n->loss_parts.p, cells, n->loss.p);
+        if (loss_out != nullptr) { // the loss's total only when it is asked for (one block: it is not free)
+            hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, n->loss_parts.p, cells, n->loss.p);
+        }
         hcheck(hipGetLastError(), "gradient");
         if (grad_out != nullptr) {
             hcheck(hipMemcpyAsync(grad_out, n->grad.p, sizeof(float) * kParams, hipMemcpyDeviceToHost, s), "download");
