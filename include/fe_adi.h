@@ -43,7 +43,8 @@ int fe_heston_prices(fe_heston* b, double* out);
 int fe_heston_lattice(fe_heston* b, int option, float* out);
 /* Lattice classes (options of one nx) and their launch shapes: lanes a row in the row sweeps (16, 32, 64: the
  * Thomas-against-PCR split, nx / lanes cells a lane in 2..16), waves a block there (1, 2, 4), threads a block in the
- * column sweeps (64, 128, 256). Setting a shape drops a captured graph. */
+ * column sweeps (64, 128, 256; -64: 64 threads keeping the forward sweep's results in LDS instead of HBM, for classes
+ * of at most 256 rows). Setting a shape drops a captured graph. */
 int fe_heston_classes(fe_heston* b);
 int fe_heston_get_shape(fe_heston* b, int cls, int* nx, int* lanes, int* row_waves, int* col_threads);
 int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col_threads);

@@ -61,7 +61,7 @@ def test_every_launch_shape_matches(device):
             k = shape["nx"] // lanes
             if k not in (2, 4, 8, 16):
                 continue
-            for row_waves, col_threads in ((1, 64), (2, 128), (4, 256)):
+            for row_waves, col_threads in ((1, 64), (2, 128), (4, 256), (2, -64)):
                 print(f"class {shape['nx']} lanes {lanes} waves {row_waves} cols {col_threads}", flush=True)
                 b.set_shape(cls, lanes, row_waves, col_threads)
                 b.reset()
@@ -112,3 +112,19 @@ def test_tuner_picks_valid_shapes_and_results_hold(device):
         assert r["lanes"] in (16, 32, 64) and r["row_waves"] in (1, 2, 4) and r["col_threads"] in (64, 128, 256)
     b.step(16)
     assert np.max(np.abs(b.prices() - ref_prices)) < PRICE_ATOL
+
+
+def test_column_sweeps_in_lds_give_the_same_bits(device):
+    import field_hip
+
+    shapes = ((128, 64), (256, 128), (512, 256))
+    nx, ny, opts = draw(9, 21, shapes)
+    runs = []
+    for cols in (64, -64):
+        b = field_hip.HestonBatch(nx, ny, opts, 12)
+        for cls, shape in enumerate(b.shapes()):
+            b.set_shape(cls, shape["lanes"], shape["row_waves"], cols)
+        b.reset()
+        b.step(12)
+        runs.append(np.concatenate([b.lattice(o) for o in range(len(nx))]))
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
