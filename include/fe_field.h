@@ -79,6 +79,13 @@ int fe_solver_step(fe_solver* s, double dt);
  * them. No counterpart in the reference (its callers loop over Solver::step): with lazy readback, pairs of steps that
  * need no contract check and no read-back run as one captured graph, replayed. */
 int fe_solver_run(fe_solver* s, double dt, int steps);
+/* The forward pass of a `half` law net on the matrix cores (fp16 operands, fp32 accumulation), for a batch of input
+ * rows already normalised: sizes[n_sizes] (inputs, hidden..., outputs), act in the order tanh, relu, silu, linear,
+ * gelu, weights in the .net layout (per layer W (out x in, row-major) then b), rows n_rows x inputs, out n_rows x
+ * outputs (times out_scale). Replaces the reference's tensor-core net kernels (src/field/cuda/CudaNet.cpp:189-277) as
+ * a testable entry point; laws reach the same kernel through the solver. Waits for the card. */
+int fe_net_half_forward(const int* sizes, int n_sizes, int act, float out_scale, const float* weights, const float* rows,
+    long long n_rows, float* out);
 int fe_solver_refresh_derived(fe_solver* s);               /* Solver::refresh_derived (Solver.hpp:148) */
 int fe_solver_set_lazy_readback(fe_solver* s, int on);     /* Solver::set_lazy_readback (Solver.hpp:138) */
 int fe_solver_check_contracts_every(fe_solver* s, int steps); /* Solver::check_contracts_every (Solver.hpp:171) */

@@ -82,6 +82,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_set_param": (c_int, [c_void_p, ctypes.c_char_p, c_double]),
         "fe_solver_step": (c_int, [c_void_p, c_double]),
         "fe_solver_run": (c_int, [c_void_p, c_double, c_int]),
+        "fe_net_half_forward": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, c_void_p, c_void_p, ctypes.c_longlong, c_void_p]),
         "fe_solver_refresh_derived": (c_int, [c_void_p]),
         "fe_solver_set_lazy_readback": (c_int, [c_void_p, c_int]),
         "fe_solver_check_contracts_every": (c_int, [c_void_p, c_int]),
@@ -600,6 +601,22 @@ class LearnedNet:
 
     def adam(self, lr: float, clip: float = 1e-2) -> None:
         self._check(lib().fe_lnet_adam(self.ptr, lr, clip), "adam")
+
+
+def net_half_forward(sizes, act: int, out_scale: float, weights, rows):
+    """A `half` law net's forward pass on the matrix cores (fp16 operands, fp32 accumulation): rows (n x inputs,
+    already normalised) -> n x outputs."""
+    import numpy as np
+
+    sizes = np.ascontiguousarray(sizes, np.int32)
+    w = np.ascontiguousarray(weights, np.float32)
+    x = np.ascontiguousarray(rows, np.float32).reshape(-1, int(sizes[0]))
+    out = np.empty((x.shape[0], int(sizes[-1])), np.float32)
+    rc = lib().fe_net_half_forward(sizes.ctypes.data, len(sizes), act, out_scale, w.ctypes.data, x.ctypes.data, x.shape[0],
+                                   out.ctypes.data)
+    if rc != 0:
+        raise FieldHipError(f"net_half_forward failed ({rc})")
+    return out
 
 
 def coarsen(fine, fnx, fny, count, factor, coarse) -> None:
