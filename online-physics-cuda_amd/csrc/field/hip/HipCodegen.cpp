@@ -310,6 +310,16 @@ private:
                 s.kind = StageCode::Kind::Tridiag;
                 s.axis = in.index;
                 lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
+            } else if ((in.op == Op::Mlp || in.op == Op::Flux || in.op == Op::LDiffuse) && in.shape == ir::Shape::Cell &&
+                law_.nets[static_cast<std::size_t>(in.index)].half) {
+                const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
+                s.kind = StageCode::Kind::NetHalf;
+                s.net = in.index;
+                s.net_op = in.op;
+                s.rows_per_cell = in.op == Op::Mlp ? 1 : 6;
+                s.in_pad = (net.inputs() + 15) / 16 * 16;
+                s.outs = net.outputs();
+                lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
             } else if (in.op == Op::Distance) {
                 s.kind = StageCode::Kind::Distance;
                 lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots;
@@ -401,6 +411,12 @@ private:
             return;
         }
         const auto stage = lay_.stages.find(id);
+        if (stage != lay_.stages.end() && stage->second.kind == StageCode::Kind::NetHalf) {
+            f << "template <class M> __device__ __forceinline__ float v" << n << "(const K& k, const M& m, int x, int y, int z)\n{\n"
+              << "    (void)m;\n    return AT(" << lay_.slot[static_cast<std::size_t>(id)] << ", x, y, z);\n}\n";
+            out_ << f.str();
+            return;
+        }
         const bool stored = stage != lay_.stages.end() && stage->second.kind == StageCode::Kind::Stored;
         const std::string head = "template <class M> __device__ __forceinline__ float " + std::string(stored ? "b" : "v") + n +
             "(const K& k, const M& m, int x, int y, int z)";
@@ -905,6 +921,138 @@ private:
         case StageCode::Kind::Poisson:
             poisson_kernels(s, in, slot);
             break;
+        case StageCode::Kind::NetHalf:
+            net_half_kernels(s, in, slot);
+            break;
+        }
+    }
+
+    // A `half` net's op in three kernels around the library's MFMA forward (see StageCode::Kind::NetHalf). The inputs,
+    // the limiter, the ldiffuse weights and the divergence are the fp32 path's (src/field/Law.cpp:673-738,1298-1443);
+    // only the net runs in fp16 with fp32 accumulation.
+    void net_half_kernels(const StageCode& s, const ir::Inst& in, int slot)
+    {
+        const Runtime walk(runtime_);
+        const NetDecl& net = law_.nets[static_cast<std::size_t>(in.index)];
+        std::vector<std::size_t> offsets;
+        (void)net_blob(law_, offsets);
+        const std::size_t off = offsets[static_cast<std::size_t>(in.index)];
+        const int nin = net.inputs();
+        const int rpc = s.rows_per_cell, pad = s.in_pad, outs = s.outs;
+        const std::string head = "    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (t >= count) return;\n"
+                                 "    const i64 i = c0 + t; const int x = (int)(i % k.nx); const int y = (int)((i / k.nx) % k.ny);"
+                                 " const int z = (int)(i / k.nxy);\n    const Direct m(k);\n";
+        std::ostringstream p;
+        p << "extern \"C\" __global__ void " << s.kernel << "_pack(K k, _Float16* rows, i64 c0, i64 count) {\n" << head
+          << "    const float* P = k.N + " << off << ";\n"
+          << "    auto put = [&](int r, const float* v) { _Float16* o = rows + (t * " << rpc << " + r) * " << pad << ";\n"
+          << "        for (int q = 0; q < " << pad << "; ++q) o[q] = q < " << nin << " ? (_Float16)((v[q] - P[q]) * P[" << nin
+          << " + q]) : (_Float16)0.0f; };\n";
+        std::ostringstream fk;
+        fk << "extern \"C\" __global__ void " << s.kernel << "_face(K k, const float* outs, float* faces, i64 c0, i64 count) {\n"
+           << head;
+        static const char* ax[3] = {"x", "y", "z"};
+        static const char* cnt[3] = {"k.nx", "k.ny", "k.nz"};
+        if (in.op == Op::Mlp) {
+            p << "    float v[" << nin << "];\n";
+            for (std::size_t q = 0; q < in.args.size(); ++q) {
+                p << "    v[" << q << "] = (float)" << d(in.args[q]) << ";\n";
+            }
+            p << "    put(0, v);\n";
+            fk << "    AT(" << slot << ", x, y, z) = outs[t * " << outs << "];\n";
+        } else if (in.op == Op::Flux) {
+            const std::size_t mm = in.args.size() - 2;
+            const double lo = at(in.args[mm]).number, hi = at(in.args[mm + 1]).number;
+            p << "    float xij[" << 2 * mm << "], xji[" << 2 * mm << "];\n";
+            fk << "    const float per = k.dt > 0.0 ? (float)(0.999 * k.h / (6.0 * k.dt)) : 0.0f;\n";
+            for (int a = 0; a < 3; ++a) {
+                const std::string j[3] = {a == 0 ? "x + 1" : "x", a == 1 ? "y + 1" : "y", a == 2 ? "z + 1" : "z"};
+                p << "    if (" << ax[a] << " + 1 < " << cnt[a] << ") {\n";
+                for (std::size_t q = 0; q < mm; ++q) {
+                    p << "        xij[" << q << "] = (float)" << d(in.args[q]) << "; xij[" << mm + q << "] = (float)"
+                      << d(in.args[q], j[0], j[1], j[2]) << ";\n"
+                      << "        xji[" << q << "] = xij[" << mm + q << "]; xji[" << mm + q << "] = xij[" << q << "];\n";
+                }
+                p << "    } else {\n        for (int q = 0; q < " << 2 * mm << "; ++q) xij[q] = xji[q] = 0.0f;\n    }\n"
+                  << "    put(" << 2 * a << ", xij); put(" << 2 * a + 1 << ", xji);\n";
+                fk << "    if (" << ax[a] << " + 1 < " << cnt[a] << ") {\n"
+                   << "        const float gij = outs[(t * 6 + " << 2 * a << ") * " << outs << "], gji = outs[(t * 6 + " << 2 * a + 1 << ") * "
+                   << outs << "];\n"
+                   << "        float r = 0.5f * (gij - gji);\n"
+                   << "        const float fi = (float)" << d(in.args[0]) << ", fj = (float)" << d(in.args[0], j[0], j[1], j[2]) << ";\n"
+                   << "        if (k.dt > 0.0) {\n            if (r > 0.0f) {\n";
+                if (lo > -1e29) {
+                    fk << "                r = fe_minf(r, fe_maxf(0.0f, (fi - (float)" << literal(lo) << ") * per));\n";
+                }
+                if (hi < 1e29) {
+                    fk << "                r = fe_minf(r, fe_maxf(0.0f, ((float)" << literal(hi) << " - fj) * per));\n";
+                }
+                fk << "            } else if (r < 0.0f) {\n";
+                if (lo > -1e29) {
+                    fk << "                r = fe_maxf(r, -fe_maxf(0.0f, (fj - (float)" << literal(lo) << ") * per));\n";
+                }
+                if (hi < 1e29) {
+                    fk << "                r = fe_maxf(r, -fe_maxf(0.0f, ((float)" << literal(hi) << " - fi) * per));\n";
+                }
+                fk << "            }\n        }\n        faces[3 * i + " << a << "] = r;\n    } else {\n        faces[3 * i + " << a
+                   << "] = 0.0f;\n    }\n";
+            }
+        } else {
+            // ldiffuse(NET, k, F): the face's window F[i-1] F[i] | F[j] F[j+1] (ends repeated), both ways round
+            const int ka = in.args[0], fa = in.args[1];
+            auto window = [&](std::ostringstream& o, int a) {
+                const int sx = a == 0, sy = a == 1, sz = a == 2;
+                const std::string c = ax[a];
+                o << "        const int xj = x + " << sx << ", yj = y + " << sy << ", zj = z + " << sz << ";\n"
+                  << "        const int xm = " << c << " > 0 ? x - " << sx << " : x, ym = " << c << " > 0 ? y - " << sy << " : y, zm = " << c
+                  << " > 0 ? z - " << sz << " : z;\n"
+                  << "        const int xp = " << c << " + 2 < " << cnt[a] << " ? xj + " << sx << " : xj, yp = " << c << " + 2 < " << cnt[a]
+                  << " ? yj + " << sy << " : yj, zp = " << c << " + 2 < " << cnt[a] << " ? zj + " << sz << " : zj;\n"
+                  << "        const float w[4] = {(float)" << d(fa, "xm", "ym", "zm") << ", (float)" << d(fa) << ", (float)"
+                  << d(fa, "xj", "yj", "zj") << ", (float)" << d(fa, "xp", "yp", "zp") << "};\n"
+                  << "        const double ka = " << d(ka) << ", kb = " << d(ka, "xj", "yj", "zj") << ";\n"
+                  << "        const float kf = (float)((ka + kb) > 0.0 ? 2.0 * ka * kb / (ka + kb) : 0.0);\n"
+                  << "        const float mirror[4] = {w[3], w[2], w[1], w[0]};\n";
+            };
+            p << "    auto inputs = [&](const float* w, float kf, float* v) {\n"
+              << "        const float centre = 0.5f * (w[1] + w[2]);\n        float spread = 1e-6f;\n"
+              << "        for (int a = 0; a < 4; ++a) spread = fe_maxf(spread, fabsf(w[a] - centre));\n"
+              << "        for (int a = 0; a < 4; ++a) v[a] = (w[a] - centre) / spread;\n"
+              << "        const float dtf = (float)k.dt, hf = (float)k.h;\n        v[4] = kf * dtf / (hf * hf);\n    };\n"
+              << "    float v1[5], v2[5];\n";
+            fk << "    auto raw = [&](const float* w, float kf, const float* y) -> float {\n"
+               << "        const float hf = (float)k.h;\n"
+               << "        const float mean = 0.25f * (y[0] + y[1] + y[2] + y[3]);\n"
+               << "        const float base[4] = {0.0f, -1.0f, 1.0f, 0.0f};\n        float acc = 0.0f;\n"
+               << "        for (int a = 0; a < 4; ++a) acc += (base[a] + (y[a] - mean)) * w[a];\n"
+               << "        return -kf / hf * acc;\n    };\n";
+            for (int a = 0; a < 3; ++a) {
+                p << "    if (" << ax[a] << " + 1 < " << cnt[a] << ") {\n";
+                window(p, a);
+                p << "        inputs(w, kf, v1); inputs(mirror, kf, v2);\n"
+                  << "    } else {\n        for (int q = 0; q < 5; ++q) v1[q] = v2[q] = 0.0f;\n    }\n"
+                  << "    put(" << 2 * a << ", v1); put(" << 2 * a + 1 << ", v2);\n";
+                fk << "    if (" << ax[a] << " + 1 < " << cnt[a] << ") {\n";
+                window(fk, a);
+                fk << "        const float ij = raw(w, kf, outs + (t * 6 + " << 2 * a << ") * " << outs << "), ji = raw(mirror, kf, outs + (t * 6 + "
+                   << 2 * a + 1 << ") * " << outs << ");\n"
+                   << "        faces[3 * i + " << a << "] = 0.5f * (ij - ji);\n    } else {\n        faces[3 * i + " << a << "] = 0.0f;\n    }\n";
+            }
+        }
+        p << "}\n";
+        fk << "}\n";
+        out_ << p.str() << fk.str();
+        if (in.op != Op::Mlp) {
+            // the cell: in through its -x, -y, -z faces, out through +x, +y, +z, in the CPU's order, over h
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k, const float* faces) { CELL_HEAD\n"
+                 << "    const i64 i = IDX(x, y, z);\n    float out = 0.0f;\n"
+                 << "    if (x > 0) out += faces[3 * (i - 1) + 0];\n"
+                 << "    if (x + 1 < k.nx) out -= faces[3 * i + 0];\n"
+                 << "    if (y > 0) out += faces[3 * (i - k.nx) + 1];\n"
+                 << "    if (y + 1 < k.ny) out -= faces[3 * i + 1];\n"
+                 << "    if (z > 0) out += faces[3 * (i - k.nxy) + 2];\n"
+                 << "    if (z + 1 < k.nz) out -= faces[3 * i + 2];\n"
+                 << "    AT(" << slot << ", x, y, z) = out / (float)k.h;\n}\n";
         }
     }
 

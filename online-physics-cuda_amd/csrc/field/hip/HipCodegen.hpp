@@ -34,7 +34,12 @@ struct StageCode {
     // Both keep four words of state (words .. words+3 of the solver's stage words, zero at the start): [0] changed,
     // [1] / [2] Distance: some cell inside / outside, Poisson with a scalar source: its double's bits; [3] solved
     // once. A solve begins by clearing [0] (Distance: [0..2]) with k_zero.
-    enum class Kind { Stored, Reduce, Tridiag, Distance, Poisson };
+    // NetHalf: a `half` net's op (mlp, flux, ldiffuse) on the matrix cores, a chunk of cells at a time: <kernel>_pack
+    // writes each evaluation's normalised inputs as an fp16 row (rows_per_cell a cell: 1 for mlp; flux and ldiffuse
+    // 6, the cell's +x, +y, +z faces both ways round), the library's MFMA forward turns rows into outputs, and
+    // <kernel>_face makes the op's value (mlp) or each face's flux (into a 3-a-cell face buffer); for flux and
+    // ldiffuse <kernel> then sums every cell's faces (the divergence).
+    enum class Kind { Stored, Reduce, Tridiag, Distance, Poisson, NetHalf };
     Kind kind = Kind::Stored;
     int inst = -1;
     std::string kernel;     // the stage's kernel (Reduce: its per-block pass; the final pass is k_reduce_final)
@@ -43,6 +48,11 @@ struct StageCode {
     int iterations = 0;     // Poisson: sweeps a solve
     int words = -1;         // Distance, Poisson: the first of its four state words
     bool scalar_source = false; // Poisson: the source is one number (its _mark kernel runs as one thread)
+    int net = -1;               // NetHalf: the net, its op, rows a cell, input row stride (halves), outputs a row
+    Op net_op = Op::Mlp;
+    int rows_per_cell = 0;
+    int in_pad = 0;
+    int outs = 0;
 };
 
 // A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of

@@ -4,6 +4,7 @@
 #include "fe_hip.h"
 #include "field/hip/HipCodegen.hpp"
 #include "field/ir/LawIR.hpp"
+#include "kernels/net_mfma.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -98,6 +99,11 @@ struct Impl final : LawBackend {
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
     std::vector<void*> now, nxt;
     Buffer P, U, A, S, L, C, N, W, records, firsts, totals;
+    // `half` nets on the matrix cores: their packed weights, and the row, output and face buffers their stages share
+    // (a chunk of kHalfChunk cells at a time).
+    static constexpr long long kHalfChunk = 1LL << 20;
+    std::map<int, std::unique_ptr<hip::HalfNetCard>> half_nets;
+    Buffer half_rows, half_outs, half_faces;
     std::vector<float> net_host; // the law's nets as the kernels read them (HipCodegen.cpp: net_blob)
     std::vector<double> uploaded_params;
     bool params_valid = false;
@@ -363,6 +369,24 @@ struct Impl final : LawBackend {
                 check(fe_hip_memset_d32_async(answer, 0u, cells), "poisson warm start reset");
             }
         }
+        std::size_t row_bytes = 0, out_bytes = 0;
+        bool faces = false;
+        half_nets.clear();
+        for (const auto& [id, st] : layout.stages) {
+            if (st.kind != hip::StageCode::Kind::NetHalf) {
+                continue;
+            }
+            if (half_nets.count(st.net) == 0) {
+                half_nets.emplace(st.net, hip::half_net_create(law->nets[static_cast<std::size_t>(st.net)]));
+            }
+            const std::size_t chunk = static_cast<std::size_t>(std::min<long long>(kHalfChunk, static_cast<long long>(cells)));
+            row_bytes = std::max(row_bytes, chunk * static_cast<std::size_t>(st.rows_per_cell * st.in_pad) * 2);
+            out_bytes = std::max(out_bytes, chunk * static_cast<std::size_t>(st.rows_per_cell * st.outs) * 4);
+            faces = faces || st.net_op != Op::Mlp;
+        }
+        half_rows.allocate(row_bytes);
+        half_outs.allocate(out_bytes);
+        half_faces.allocate(faces ? cells * 3 * 4 : 0);
         records.allocate(std::max<std::size_t>(1, nf) * 2 * 4);
         firsts.allocate(std::max<std::size_t>(1, nf) * 8);
         totals.allocate(static_cast<std::size_t>(kTotals) * 8);
@@ -500,6 +524,28 @@ struct Impl final : LawBackend {
             }
             launch_cells(s.kernel + "_copy", dt, {&w, &from});
             ++prof.launches;
+            break;
+        }
+        case hip::StageCode::Kind::NetHalf: {
+            const long long cells = static_cast<long long>(shape.cells());
+            const hip::HalfNetCard& net = *half_nets.at(s.net);
+            void* rows = half_rows.p;
+            float* outs = half_outs.as<float>();
+            float* faces = half_faces.as<float>();
+            std::vector<std::uint64_t> k = args(dt);
+            for (long long c0 = 0; c0 < cells; c0 += kHalfChunk) {
+                long long count = std::min(kHalfChunk, cells - c0);
+                const unsigned blocks = static_cast<unsigned>((count + 255) / 256);
+                void* pargs[] = {k.data(), &rows, &c0, &count};
+                check(fe_hip_launch(fn(s.kernel + "_pack"), blocks, 1, 1, 256, 1, 1, 0, pargs), "net pack launch");
+                check(hip::half_net_run(net, rows, count * s.rows_per_cell, outs), "half net launch");
+                void* fargs[] = {k.data(), &outs, &faces, &c0, &count};
+                check(fe_hip_launch(fn(s.kernel + "_face"), blocks, 1, 1, 256, 1, 1, 0, fargs), "net face launch");
+                prof.launches += 3;
+            }
+            if (s.net_op != Op::Mlp) {
+                launch_cells(s.kernel, dt, {&faces});
+            }
             break;
         }
         case hip::StageCode::Kind::Reduce: {

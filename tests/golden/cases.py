@@ -323,6 +323,16 @@ NET_CASES = [
          nets={"g": net_text("g", [4, 8, 1], "tanh", 35)}, atol=1e-6),
     dict(name="net_ldiffuse_silu", law=net_law("net w in 5 hidden 8 out 4 act silu\n", "  d A = ldiffuse(w, kk, A)\n"),
          nets={"w": net_text("w", [5, 8, 4], "silu", 36, 0.3)}, atol=1e-6),
+    # `half` nets: the CPU evaluates them in fp32 as any net; the card on its matrix cores in fp16. Held as the
+    # reference holds its tensor cores to its CPU (src/tests/nets_smoke.cpp:183): 0.03 of the largest change plus 1e-3.
+    dict(name="net_mlp_half", law=net_law("net q in 2 hidden 32 32 out 2 act tanh half\n",
+                                          "  d B = mlp(q, A, B)\n  d A = 0.5 * mlp(q, B, A)\n"),
+         nets={"q": net_text("q", [2, 32, 32, 2], "tanh", 41, 0.7, [0.5, 0.2], [2.0, 1.5])}, atol_moved=0.03),
+    dict(name="net_flux_half", law=net_law("net g in 4 hidden 16 out 1 act tanh half\n", "  d A = flux(g, A, B)\n", "0 1"),
+         nets={"g": net_text("g", [4, 16, 1], "tanh", 42, 1.3, [0.5, 0.3, 0.5, 0.3])}, dt=0.05, atol_moved=0.03),
+    dict(name="net_ldiffuse_half", law=net_law("net w in 5 hidden 16 out 4 act relu half\n",
+                                               "  d B = ldiffuse(w, kk, B)\n  d A = ldiffuse(w, 0.5 + B, A)\n"),
+         nets={"w": net_text("w", [5, 16, 4], "relu", 43, 0.3)}, atol_moved=0.03),
 ]
 
 CASES = [

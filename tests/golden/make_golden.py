@@ -73,6 +73,9 @@ def main() -> None:
         for key in ("nets", "atol"):
             if key in case:
                 meta[key] = case[key]
+        if "atol_moved" in case:
+            moved = max(float(np.max(np.abs(out["final"][f].astype(np.float64) - case["init"][f]))) for f in case["init"])
+            meta["atol"] = case["atol_moved"] * moved + 1e-3
         meta["reference_stdout"] = out["stdout"]
         meta["reference_stderr"] = out["stderr"]
         arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)

@@ -12,7 +12,7 @@ import golden_io
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["heat_20x14x9", "multi", "op_poisson", "op_distance_moving", "kitchen", "net_flux_relu", "adi_yz"]
+CASES = ["heat_20x14x9", "multi", "op_poisson", "op_distance_moving", "kitchen", "net_flux_relu", "net_ldiffuse_half", "adi_yz"]
 
 
 def make(field_hip, meta, init):
