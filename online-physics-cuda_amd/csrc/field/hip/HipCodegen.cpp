@@ -103,7 +103,10 @@ extern "C" __global__ void k_range(K k, int field, double lo, double hi, int cla
 
 // Conserved totals, pass 1: block b sums cells [b * chunk, (b + 1) * chunk), lane t every 256th from t; then the lanes
 // fold in a fixed tree. out[2b], out[2b+1]: the sum and the sum of magnitudes (each a compensated pair, rounded).
-extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 chunk, double* out)
+// With rec not null the same pass is also the field's range contract (k_range's test and clamp, on the value the
+// total took).
+extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 chunk, double* out, double lo, double hi, int clamp,
+    unsigned* rec, u64* first)
 {
     __shared__ double ss[256], sc[256], ms[256], mc[256];
     const int t = (int)threadIdx.x;
@@ -111,7 +114,21 @@ extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 ch
     i64 end = begin + chunk; if (end > k.cells) end = k.cells;
     Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
     const float* v = k.now[field];
-    for (i64 i = begin + t; i < end; i += 256) { const double d = (double)v[i]; add(s, d); add(m, fabs(d)); }
+    for (i64 i = begin + t; i < end; i += 256) {
+        const double d = (double)v[i]; add(s, d); add(m, fabs(d));
+        if (rec != nullptr) {
+            const bool fin = finite_d(d);
+            bool bad = !fin;
+            if (fin && (d < lo || d > hi)) {
+                if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
+                else bad = true;
+            }
+            if (bad) {
+                atomicAdd(&rec[fin ? 0 : 1], 1u);
+                atomicMin(first, (u64)i);
+            }
+        }
+    }
     ss[t] = s.s; sc[t] = s.c; ms[t] = m.s; mc[t] = m.c;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

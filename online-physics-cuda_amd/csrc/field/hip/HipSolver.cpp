@@ -426,6 +426,7 @@ struct Impl final : LawBackend {
             check(fe_hip_memcpy_htod_async(now[f], v.data(), cells * 4), "field upload");
             prof.upload_bytes += cells * 4;
             stale[f] = 0;
+            touched(static_cast<int>(f));
         }
         edits.all = false;
         edits.planes.assign(law->fields.size(), std::vector<char>(static_cast<std::size_t>(shape.nz), 0));
@@ -574,21 +575,74 @@ struct Impl final : LawBackend {
         }
     }
 
-    // The conserved total (sum, sum of magnitudes) of a field into totals[slot], [slot + 1].
-    void total(int field, int slot)
+    // Which fields may have changed since: a counter a field, bumped by every write to it (gathers, derives, uploads,
+    // device writes, restores, clamps, replays). A conserved total taken at a generation stands for the field until
+    // the next bump, so a judged step's "before" total is the last judged step's "after" one, copied.
+    std::vector<long long> generation;
+    struct Cached {
+        int slot;
+        long long gen;
+    };
+    std::map<int, Cached> cached_totals;
+
+    void touched(int f)
+    {
+        if (generation.size() <= static_cast<std::size_t>(f)) {
+            generation.resize(static_cast<std::size_t>(f) + 1, 0);
+        }
+        ++generation[static_cast<std::size_t>(f)];
+    }
+    long long gen_of(int f)
+    {
+        touched(f);
+        --generation[static_cast<std::size_t>(f)];
+        return generation[static_cast<std::size_t>(f)];
+    }
+    void note_total(int field, int slot)
+    {
+        for (auto it = cached_totals.begin(); it != cached_totals.end();) {
+            it = it->second.slot == slot && it->first != field ? cached_totals.erase(it) : std::next(it);
+        }
+        cached_totals[field] = Cached{slot, gen_of(field)};
+    }
+    // The total of a field into totals[slot], from the last one taken when the field has not changed since.
+    void total_before(int field, int slot)
+    {
+        const auto it = cached_totals.find(field);
+        if (it != cached_totals.end() && it->second.gen == gen_of(field)) {
+            if (it->second.slot != slot) {
+                check(fe_hip_memcpy_dtod_async(totals.as<double>() + slot, totals.as<double>() + it->second.slot, 16), "total copy");
+            }
+        } else {
+            total(field, slot);
+        }
+        note_total(field, slot);
+    }
+
+    // The conserved total (sum, sum of magnitudes) of a field into totals[slot], [slot + 1]; with `ranged` the same
+    // pass checks (and clamps) the field's range.
+    void total(int field, int slot, bool ranged = false)
     {
         const long long cells = static_cast<long long>(shape.cells());
         long long chunk = std::max(1LL, (cells + kReduceBlocks - 1) / kReduceBlocks);
         const int blocks = static_cast<int>((cells + chunk - 1) / chunk);
         std::vector<std::uint64_t> k = args(0.0);
         double* parts = L.as<double>();
-        void* argv1[] = {k.data(), &field, &chunk, &parts};
+        const FieldDecl& d = law->fields[static_cast<std::size_t>(field)];
+        double lo = d.lo, hi = d.hi;
+        int clamp = d.clamp ? 1 : 0;
+        unsigned* rec = ranged ? records.as<unsigned>() + 2 * field : nullptr;
+        unsigned long long* first = firsts.as<unsigned long long>() + field;
+        void* argv1[] = {k.data(), &field, &chunk, &parts, &lo, &hi, &clamp, &rec, &first};
         check(fe_hip_launch(fn("k_total"), static_cast<unsigned>(blocks), 1, 1, 256, 1, 1, 0, argv1), "total launch");
         double* dst = totals.as<double>() + slot;
         int count = blocks;
         void* argv2[] = {&parts, &count, &dst};
         check(fe_hip_launch(fn("k_total_final"), 1, 1, 1, 256, 1, 1, 0, argv2), "total final launch");
         prof.launches += 2;
+        if (ranged && clamp) {
+            touched(field);
+        }
     }
 
     void range(int field)
@@ -599,6 +653,9 @@ struct Impl final : LawBackend {
         unsigned* rec = records.as<unsigned>() + 2 * field;
         unsigned long long* first = firsts.as<unsigned long long>() + field;
         launch_cells("k_range", 0.0, {&field, &lo, &hi, &clamp, &rec, &first});
+        if (clamp) {
+            touched(field);
+        }
     }
 
     void reset_records()
@@ -662,7 +719,7 @@ struct Impl final : LawBackend {
         if (contracts) {
             for (const std::vector<int>& group : p.conserves) {
                 for (std::size_t q = 0; q < group.size(); ++q) {
-                    total(group[q], total_slot + 2 * static_cast<int>(q));
+                    total_before(group[q], total_slot + 2 * static_cast<int>(q));
                 }
                 total_slot += 4 * static_cast<int>(group.size());
             }
@@ -674,17 +731,33 @@ struct Impl final : LawBackend {
             launch_gather(code, sub_dt);
             swap(code.swaps);
         }
+        for (const int f : p.writes) {
+            touched(f);
+        }
         if (contracts) {
+            // a written field's range is checked in the pass that takes its "after" total
+            std::set<int> ranged;
             int slot = j.slot;
             for (const std::vector<int>& group : p.conserves) {
                 const int m = static_cast<int>(group.size());
                 for (int q = 0; q < m; ++q) {
-                    total(group[static_cast<std::size_t>(q)], slot + 2 * m + 2 * q);
+                    const int f = group[static_cast<std::size_t>(q)];
+                    const bool writes = std::find(p.writes.begin(), p.writes.end(), f) != p.writes.end() && ranged.count(f) == 0;
+                    total(f, slot + 2 * m + 2 * q, writes);
+                    note_total(f, slot + 2 * m + 2 * q);
+                    if (writes) {
+                        ranged.insert(f);
+                        if (law->fields[static_cast<std::size_t>(f)].clamp) {
+                            cached_totals.erase(f); // the clamp changed it after its total
+                        }
+                    }
                 }
                 slot += 4 * m;
             }
             for (const int f : p.writes) {
-                range(f);
+                if (ranged.count(f) == 0) {
+                    range(f);
+                }
             }
             judges.push_back(j);
         }
@@ -778,6 +851,7 @@ struct Impl final : LawBackend {
     {
         const std::size_t bytes = shape.cells() * 4;
         for (std::size_t i = 0; i < fields.size(); ++i) {
+            touched(fields[i]);
             check(fe_hip_memcpy_dtod_async(now[static_cast<std::size_t>(fields[i])], saved[i]->p, bytes), "restore");
         }
     }
@@ -794,6 +868,9 @@ struct Impl final : LawBackend {
             }
             launch_cells(code.gather, 0.0);
             swap(code.swaps);
+            for (const int f : module.derives[d].writes) {
+                touched(f);
+            }
         }
         if (contracts && !module.derives.empty()) {
             Judge j;
@@ -926,6 +1003,13 @@ struct Impl final : LawBackend {
             if (length > 0 && replayable(respect, learned, edits)) {
                 replay_pair(dt, respect, length, substeps);
                 step_count += length;
+                for (const auto* progs : {&module.terms, &module.derives}) {
+                    for (const ir::Program& p : *progs) {
+                        for (const int f : p.writes) {
+                            touched(f);
+                        }
+                    }
+                }
                 changed.assign(nf, 0);
                 for (const auto* progs : {&module.terms, &module.derives}) {
                     for (const ir::Program& p : *progs) {
@@ -1192,13 +1276,18 @@ struct Impl final : LawBackend {
     // Device pointers of a field's current cells (for callers that keep data on the card).
     [[nodiscard]] void* field_pointer(int f) const { return now[static_cast<std::size_t>(f)]; }
     // A caller wrote field f's cells on the card: the host copy is stale until fetched.
-    void device_written(int f) { stale[static_cast<std::size_t>(f)] = 1; }
+    void device_written(int f)
+    {
+        stale[static_cast<std::size_t>(f)] = 1;
+        touched(f);
+    }
 
     // The field's fixed-order compensated total and total magnitude, from the card (waits for it).
     void field_total(int f, double& sum, double& mag)
     {
         const std::lock_guard<std::mutex> lock(device_mutex());
         total(f, kTotals - 2);
+        note_total(f, kTotals - 2);
         double out[2] = {0.0, 0.0};
         check(fe_hip_memcpy_dtoh_async(out, totals.as<double>() + (kTotals - 2), sizeof(out)), "totals readback");
         check(fe_hip_stream_sync(), "sync");
@@ -1260,6 +1349,8 @@ std::string HipSolver::source(const Law& law, std::string& why)
 
 void* hip_backend_field_pointer(LawBackend& backend, int field)
 {
+    // whoever holds the address may write through it: the field's cached total no longer stands for it
+    static_cast<Impl&>(backend).touched(field);
     return static_cast<Impl&>(backend).field_pointer(field);
 }
 

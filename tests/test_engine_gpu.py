@@ -73,6 +73,45 @@ def test_contracts_report_a_leak_and_a_range_breach(device):
     assert ok.violations() == []
 
 
+def test_judged_steps_reuse_a_total_only_while_it_stands(device):
+    # Each judged step's "before" total is the last judged "after" one when nothing wrote the field since; an upload,
+    # a device write, another term's write or a clamp must make it be taken again (else a false leak is reported).
+    import field_hip
+
+    text = ("field T init 0 range -1000000 1000000\nfield S init 0 range 0 1 clamp\nparam k = 0.2\nparam leak = 0\n"
+            "term conduct\n  d T = diffuse(k, T) + leak\n  d S = diffuse(k, S) + 0.3\n  conserves T\n"
+            "term source\n  d T = 0.01 * S\n")
+    law = field_hip.Law(text)
+    solver = field_hip.Solver(law, (16, 8, 4), 1.0)
+    rng = np.random.default_rng(5)
+    solver.set_field("T", rng.random(16 * 8 * 4).astype(np.float32))
+    solver.set_field("S", rng.random(16 * 8 * 4).astype(np.float32))
+    solver.check_contracts_every(1)
+
+    def kinds():
+        return sorted(v["kind"] for v in solver.violations())
+
+    for _ in range(3):
+        solver.step(1.0)
+        assert kinds() == [], solver.violations()
+    solver.set_param("leak", 0.5)
+    solver.step(1.0)
+    assert kinds() == ["conservation"], solver.violations()
+    solver.set_param("leak", 0.0)
+    solver.step(1.0)
+    assert kinds() == []
+    solver.set_field("T", rng.random(16 * 8 * 4).astype(np.float32) * 7)
+    solver.step(1.0)
+    assert kinds() == [], solver.violations()
+    src = field_hip.DeviceArray.from_host(rng.random(16 * 8 * 4).astype(np.float32) * 3)
+    solver.set_field_device("T", src)
+    solver.step(1.0)
+    assert kinds() == [], solver.violations()
+    solver.run(1.0, 5)
+    solver.step(1.0)
+    assert kinds() == [], solver.violations()
+
+
 def test_substeps_follow_max_dt_and_can_be_ignored(device):
     import field_hip
 
