@@ -58,6 +58,25 @@ def test_generated_source_compiles_for_gfx950(name):
     assert len(blob) > 1000
 
 
+@pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="the reference's laws are only in the build container")
+@pytest.mark.parametrize("paths", reference_laws(), ids=lambda p: "+".join(os.path.basename(q) for q in p))
+def test_shipped_laws_generate_and_compile(tmp_path, paths):
+    # Every op of the shipped laws has a form on the card; the world law (world + magic, 43 terms) is also compiled
+    # (the others' sources are ordinary ops the golden laws compile; compiling each would take minutes here).
+    import field_hip
+
+    try:
+        law = field_hip.Law.from_files(*paths)
+    except field_hip.FieldHipError:
+        pytest.skip("the reference's front end refuses it too (checked by test_ir_json_is_the_references)")
+    src = law.hip_source()
+    assert "extern \"C\" __global__ void g_t0(K k" in src
+    if [os.path.basename(p) for p in paths] == ["world.law", "magic.law"]:
+        opts = ["-O3", "-std=c++17", "-ffp-contract=off", "-fgpu-flush-denormals-to-zero", "-Xclang", "-target-feature",
+                "-Xclang", "-packed-fp32-ops"]
+        assert len(field_hip.compile_hip(src, "law.hip", opts)) > 1000
+
+
 ERRORS = [
     ("field T init 0\nterm t\n  d T = diffuse(k, T)\n", "unknown name 'k'"),
     ("field T init 0\nterm t\n  d T = lap(T) +\n", "expression ends too early"),
