@@ -107,11 +107,17 @@ void* fe_solver_field_device(fe_solver* s, int field);
  * (checkpoints, adjoint sweeps); call it once the solver has uploaded its fields (a step or refresh_derived). No
  * counterpart in the reference, whose states travel through the host (tools/lawc/main.cpp:310-323). */
 int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
+/* Every cell of a field set to `value` on the card (no host copy; the host copy goes stale as after a device write).
+ * No counterpart in the reference: the reverse-mode driver clears its per-step parameter adjoints with it. */
+int fe_solver_fill_field(fe_solver* s, int field, float value);
 /* A field's total and total magnitude as the contracts compute them on the card: fixed-order compensated sums in
  * double (a block a fixed range of cells, the blocks folded in order), so two runs give the same bits; waits for the
  * card. The reference reports the totals its conservation judge reads (src/field/Solver.cpp:31-39,
  * src/field/cuda/CudaSolver.cpp:515-571). magnitude may be NULL. */
 int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude);
+/* The same total added on the card into device_acc[0] (sum) and device_acc[1] (magnitude), two doubles in device
+ * memory, queued without waiting: a caller summing a field's total over many steps reads the sum once. */
+int fe_solver_total_accumulate(fe_solver* s, int field, double* device_acc);
 
 #ifdef __cplusplus
 }

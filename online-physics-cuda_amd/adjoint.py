@@ -69,11 +69,10 @@ def vjp(law_text: str, shape, h: float, params: dict, init: dict, dt: float, ste
     adj.set_field(loss_field + "__adj", (2.0 * final).astype(np.float32))
     adj.refresh_derived()  # uploads the seed now: from here on only device copies write the adjoint solver's fields
     # A param's adjoint field takes one step's contribution at a time: its total (fixed-order, compensated, in double
-    # on the card) is added here and the field cleared. Left to accumulate in fp32 over the steps, the cells' sums
-    # would round at every step, and the lattice's total of them cancels heavily.
+    # on the card) is added to a double on the card, step after step, and the field cleared. Left to accumulate in
+    # fp32 over the steps, the cells' sums would round at every step, and the lattice's total of them cancels heavily.
     pnames = [p for p in params if p + "__adj" in adj_law.field_names()]
-    zeros = field_hip.DeviceArray.from_host(np.zeros(cells, np.float32)) if pnames else None
-    ptotal = {p: 0.0 for p in pnames}
+    pacc = {p: field_hip.DeviceArray.from_host(np.zeros(4, np.float32)) for p in pnames}  # two doubles, added on the card
     ev.begin()
     for seg in range(n_ckpt - 1, -1, -1):
         first = seg * every
@@ -89,10 +88,10 @@ def vjp(law_text: str, shape, h: float, params: dict, init: dict, dt: float, ste
                 adj.set_field_device(name, arr)
             adj.step(dt)
             for p in pnames:
-                ptotal[p] += adj.total(p + "__adj")[0]
-                adj.set_field_device(p + "__adj", zeros)
+                adj.total_accumulate(p + "__adj", pacc[p])
+                adj.fill_field(p + "__adj", 0.0)
     backward_ms = ev.end()
     grads = {n: adj.get_field(n + "__adj").astype(np.float64) for n in inputs}
-    pgrads = {p: float(v) for p, v in ptotal.items()}
+    pgrads = {p: float(pacc[p].download().view(np.float64)[0]) for p in pnames}
     return Result(loss, grads, pgrads, forward_ms, backward_ms,
                   {"checkpoints": n_ckpt, "every": every, "states_held": n_ckpt + every, "cells": cells, "steps": steps})

@@ -20,6 +20,7 @@ namespace fieldengine::field {
 void* hip_backend_field_pointer(LawBackend& backend, int field);
 void hip_backend_device_written(LawBackend& backend, int field);
 void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag);
+void hip_backend_total_accumulate(LawBackend& backend, int field, double* acc);
 }
 
 struct fe_solver {
@@ -295,6 +296,32 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
     hip_backend_device_written(*s->solver->backend(), field);
     s->solver->mark_device_written(field);
     return 0;
+}
+
+int fe_solver_fill_field(fe_solver* s, int field, float value)
+{
+    if (!valid_field(s, field) || s->solver->backend() == nullptr) {
+        return -1;
+    }
+    void* dst = hip_backend_field_pointer(*s->solver->backend(), field);
+    std::uint32_t bits = 0;
+    std::memcpy(&bits, &value, 4);
+    const int r = fe_hip_memset_d32_async(dst, bits, s->solver->fields_const_shape_cells());
+    if (r != 0) {
+        s->error = std::string("device fill: ") + fe_hip_error_string(r);
+        return r;
+    }
+    hip_backend_device_written(*s->solver->backend(), field);
+    s->solver->mark_device_written(field);
+    return 0;
+}
+
+int fe_solver_total_accumulate(fe_solver* s, int field, double* device_acc)
+{
+    if (!valid_field(s, field) || device_acc == nullptr || s->solver->backend() == nullptr) {
+        return -1;
+    }
+    return guarded(s, [&] { hip_backend_total_accumulate(*s->solver->backend(), field, device_acc); });
 }
 
 int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude)

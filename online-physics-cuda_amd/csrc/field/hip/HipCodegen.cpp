@@ -160,6 +160,12 @@ extern "C" __global__ __launch_bounds__(256) void k_total_final(const double* pa
     if (t == 0) { dst[0] = ss[0] + sc[0]; dst[1] = ms[0] + mc[0]; }
 }
 
+// acc[0..1] += src[0..1] in double (a field's total added on the card, in call order: no wait for the host).
+extern "C" __global__ void k_acc(const double* src, double* acc)
+{
+    if (threadIdx.x == 0) { acc[0] = acc[0] + src[0]; acc[1] = acc[1] + src[1]; }
+}
+
 // Clears n words (the change flags) with a kernel: a graph replays kernels, and fills recorded as memset nodes have
 // given wrong results on replay on this hardware.
 extern "C" __global__ void k_zero(unsigned* p, int n)

@@ -1283,6 +1283,17 @@ struct Impl final : LawBackend {
     }
 
     // The field's fixed-order compensated total and total magnitude, from the card (waits for it).
+    // The field's total added into acc[0] (sum) and acc[1] (magnitude) on the card, queued.
+    void field_total_accumulate(int f, double* acc)
+    {
+        const std::lock_guard<std::mutex> lock(device_mutex());
+        total(f, kTotals - 2);
+        note_total(f, kTotals - 2);
+        double* src = totals.as<double>() + (kTotals - 2);
+        void* argv[] = {&src, &acc};
+        check(fe_hip_launch(fn("k_acc"), 1, 1, 1, 64, 1, 1, 0, argv), "total accumulate");
+    }
+
     void field_total(int f, double& sum, double& mag)
     {
         const std::lock_guard<std::mutex> lock(device_mutex());
@@ -1357,6 +1368,11 @@ void* hip_backend_field_pointer(LawBackend& backend, int field)
 void hip_backend_device_written(LawBackend& backend, int field)
 {
     static_cast<Impl&>(backend).device_written(field);
+}
+
+void hip_backend_total_accumulate(LawBackend& backend, int field, double* acc)
+{
+    static_cast<Impl&>(backend).field_total_accumulate(field, acc);
 }
 
 void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag)

@@ -82,6 +82,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_set_param": (c_int, [c_void_p, ctypes.c_char_p, c_double]),
         "fe_solver_step": (c_int, [c_void_p, c_double]),
         "fe_solver_run": (c_int, [c_void_p, c_double, c_int]),
+        "fe_solver_fill_field": (c_int, [c_void_p, c_int, ctypes.c_float]),
+        "fe_solver_total_accumulate": (c_int, [c_void_p, c_int, c_void_p]),
         "fe_net_half_forward": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, c_void_p, c_void_p, ctypes.c_longlong, c_void_p]),
         "fe_solver_refresh_derived": (c_int, [c_void_p]),
         "fe_solver_set_lazy_readback": (c_int, [c_void_p, c_int]),
@@ -446,6 +448,13 @@ class Solver:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(lib().fe_solver_total(self.ptr, self._field(field), ctypes.byref(a), ctypes.byref(b)), "total")
         return a.value, b.value
+
+    def total_accumulate(self, field, acc) -> None:
+        """Adds the field's (sum, magnitude) into acc (a DeviceArray of at least 4 floats, read as two doubles)."""
+        self._check(lib().fe_solver_total_accumulate(self.ptr, self._field(field), acc.ptr), "total_accumulate")
+
+    def fill_field(self, field, value: float) -> None:
+        self._check(lib().fe_solver_fill_field(self.ptr, self._field(field), float(value)), "fill_field")
 
     def set_field_device(self, field, src) -> None:
         """Copies a whole field from device memory (a DeviceArray or a device pointer) into the solver on the card."""
