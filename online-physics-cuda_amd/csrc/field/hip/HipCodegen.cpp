@@ -1382,6 +1382,49 @@ private:
         }
     }
 
+    // A lane's VX cells through the cell function (the fast view m, then the exact view EX when m flagged a cell), then
+    // the stores and change tracking.
+    std::string cells_body(const std::string& cellfn, const std::string& EX, int VX, const std::vector<int>& targets, bool in_place) const
+    {
+        const int NT = static_cast<int>(targets.size());
+        std::ostringstream body;
+        body << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
+        body << "            m.bad = false; m.zc = z;\n";
+        for (int c = 0; c < VX; ++c) {
+            body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
+        }
+        body << "            if (m.bad) {\n                const " << EX << "<P2> mx(m);\n";
+        for (int c = 0; c < VX; ++c) {
+            body << "                " << cellfn << "(k, mx, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
+        }
+        body << "            }\n";
+        body << "            const i64 i = IDX(x, y, z);\n";
+        // Change tracking (TRACK) only where the step reads the flags back; an in-place store still skips unchanged
+        // cells.
+        for (int q = 0; q < NT; ++q) {
+            const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
+            body << "            { bool ch = false;\n";
+            for (int c = 0; c < VX; ++c) {
+                body << "              if (TRACK || " << (in_place ? "true" : "false") << ") { const bool d = __float_as_uint(o[" << c << "]["
+                     << q << "]) != __float_as_uint(n[" << c << "][" << q << "]); ch = ch || d;";
+                if (in_place) {
+                    body << " if (d) k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "];";
+                }
+                body << " }\n";
+            }
+            if (!in_place) {
+                if (VX == 4) {
+                    body << "              *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
+                         << "], n[2][" << q << "], n[3][" << q << "]);\n";
+                } else {
+                    body << "              k.nxt[" << t << "][i] = n[0][" << q << "];\n";
+                }
+            }
+            body << "              seen[" << q << "] = seen[" << q << "] || ch; }\n";
+        }
+        return body.str();
+    }
+
     void window_kernel(const std::string& cellfn, const std::string& name, const std::vector<int>& targets, const std::vector<int>& wf,
         int R, int VX, int TY, bool in_place)
     {
@@ -1521,41 +1564,7 @@ private:
             s << "    };\n";
         }
         // The cells of a lane: their new values and the stores.
-        std::ostringstream body;
-        body << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
-        body << "            m.bad = false; m.zc = z;\n";
-        for (int c = 0; c < VX; ++c) {
-            body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
-        }
-        body << "            if (m.bad) {\n                const WX_" << name << "<P2> mx(m);\n";
-        for (int c = 0; c < VX; ++c) {
-            body << "                " << cellfn << "(k, mx, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
-        }
-        body << "            }\n";
-        body << "            const i64 i = IDX(x, y, z);\n";
-        // Change tracking (TRACK) only where the step reads the flags back; an in-place store still skips unchanged
-        // cells.
-        for (int q = 0; q < NT; ++q) {
-            const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
-            body << "            { bool ch = false;\n";
-            for (int c = 0; c < VX; ++c) {
-                body << "              if (TRACK || " << (in_place ? "true" : "false") << ") { const bool d = __float_as_uint(o[" << c << "]["
-                     << q << "]) != __float_as_uint(n[" << c << "][" << q << "]); ch = ch || d;";
-                if (in_place) {
-                    body << " if (d) k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "];";
-                }
-                body << " }\n";
-            }
-            if (!in_place) {
-                if (VX == 4) {
-                    body << "              *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
-                         << "], n[2][" << q << "], n[3][" << q << "]);\n";
-                } else {
-                    body << "              k.nxt[" << t << "][i] = n[0][" << q << "];\n";
-                }
-            }
-            body << "              seen[" << q << "] = seen[" << q << "] || ch; }\n";
-        }
+        const std::string body = cells_body(cellfn, "WX_" + name, VX, targets, in_place);
         // A tile's change flags are marked once, after its last plane (a store a plane would be waited on with the loads).
         s << "    bool seen[" << std::max(1, NT) << "] = {};\n";
         s << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); keep(p, zp - z0 + " << R
@@ -1570,7 +1579,7 @@ private:
           << "        put(cur, z + " << R << ");\n"
           << "        keep(cur, " << 2 * R << ");\n"
           << "        __syncthreads();\n"
-          << "        if (live) {\n" << body.str() << "        }\n"
+          << "        if (live) {\n" << body << "        }\n"
           << "        for (int w = 0; w < " << NW << "; ++w) for (int j = 0; j < " << 2 * R << "; ++j) for (int c = 0; c < " << VX
           << "; ++c) m.own[w][j][c] = m.own[w][j + 1][c];\n"
           << "    };\n"

@@ -300,7 +300,7 @@ struct Impl final : LawBackend {
         allocate();
     }
 
-    // Window gathers that would run badly: private (scratch) memory, or more registers a lane than leave two waves a
+    // Window gathers that would run badly: private (scratch) memory, or more registers a lane than leave three waves a
     // SIMD (four cells a lane of a law with many fields). The automatic choice passes them over.
     std::set<std::string> heavy;
     void weigh_windows()
