@@ -110,6 +110,10 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
 /* Every cell of a field set to `value` on the card (no host copy; the host copy goes stale as after a device write).
  * No counterpart in the reference: the reverse-mode driver clears its per-step parameter adjoints with it. */
 int fe_solver_fill_field(fe_solver* s, int field, float value);
+/* A law net's weights replaced (count = the net's parameter count, the .net layout): what a trainer writes through
+ * Solver::law_for_training() (src/field/Solver.hpp:82-84); the next step takes them up (the backend hashes the
+ * weights every step, src/field/cuda/CudaSolver.cpp:282-311). -1: no such net or another count. */
+int fe_solver_set_net_weights(fe_solver* s, const char* net, const float* weights, size_t count);
 /* A field's total and total magnitude as the contracts compute them on the card: fixed-order compensated sums in
  * double (a block a fixed range of cells, the blocks folded in order), so two runs give the same bits; waits for the
  * card. The reference reports the totals its conservation judge reads (src/field/Solver.cpp:31-39,

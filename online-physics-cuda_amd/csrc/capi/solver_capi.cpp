@@ -298,6 +298,26 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
     return 0;
 }
 
+int fe_solver_set_net_weights(fe_solver* s, const char* net, const float* weights, size_t count)
+{
+    if (s == nullptr || net == nullptr || weights == nullptr) {
+        return -1;
+    }
+    Law& law = s->solver->law_for_training();
+    const int n = law.net_id(net);
+    if (n < 0) {
+        s->error = std::string("no net '") + net + "'";
+        return -1;
+    }
+    NetDecl& d = law.nets[static_cast<std::size_t>(n)];
+    if (count != d.parameter_count()) {
+        s->error = "net '" + d.name + "' has " + std::to_string(d.parameter_count()) + " weights, not " + std::to_string(count);
+        return -1;
+    }
+    d.weights.assign(weights, weights + count);
+    return 0;
+}
+
 int fe_solver_fill_field(fe_solver* s, int field, float value)
 {
     if (!valid_field(s, field) || s->solver->backend() == nullptr) {

@@ -360,6 +360,7 @@ struct Impl final : LawBackend {
         if (!net_host.empty()) {
             check(fe_hip_memcpy_htod_async(N.p, net_host.data(), net_host.size() * 4), "nets upload");
         }
+        nets_key = nets_hash(*law);
         // The Distance and Poisson stages' state words start at zero, and a Poisson stage's answer (its warm start) too.
         W.allocate(static_cast<std::size_t>(std::max(1, layout.stage_words)) * 4);
         check(fe_hip_memset_d32_async(W.p, 0u, static_cast<std::size_t>(std::max(1, layout.stage_words))), "stage words reset");
@@ -393,8 +394,55 @@ struct Impl final : LawBackend {
         stale.assign(nf, 0);
     }
 
+    // A trainer may write a law's net weights between steps (Solver::law_for_training, fe_solver_set_net_weights):
+    // their bits are hashed every step, as the reference's backend does (src/field/cuda/CudaSolver.cpp:282-311), and
+    // a change goes up again, in place (the same addresses, so captured graphs stay valid).
+    std::uint64_t nets_key = 0;
+    static std::uint64_t nets_hash(const Law& l)
+    {
+        std::uint64_t key = 1469598103934665603ull;
+        auto mix = [&](const std::vector<float>& v) {
+            for (const float w : v) {
+                std::uint32_t bits = 0;
+                std::memcpy(&bits, &w, 4);
+                key = (key ^ bits) * 1099511628211ull;
+            }
+        };
+        for (const NetDecl& net : l.nets) {
+            mix(net.weights);
+            mix(net.in_shift);
+            mix(net.in_scale);
+            mix({net.out_scale});
+        }
+        return key;
+    }
+    void upload_nets()
+    {
+        if (law->nets.empty()) {
+            return;
+        }
+        const std::uint64_t key = nets_hash(*law);
+        if (key == nets_key) {
+            return;
+        }
+        std::vector<std::size_t> offsets;
+        std::vector<float> blob = hip::net_blob(*law, offsets);
+        if (blob.size() != net_host.size()) {
+            throw std::runtime_error("a net changed its shape after the law was built");
+        }
+        net_host = std::move(blob);
+        check(fe_hip_memcpy_htod_async(N.p, net_host.data(), net_host.size() * 4), "nets upload");
+        for (auto& [n, card] : half_nets) {
+            hip::half_net_update(*card, law->nets[static_cast<std::size_t>(n)]);
+        }
+        check(fe_hip_stream_sync(), "sync"); // the host copies the uploads read live until here
+        nets_key = key;
+        drop_pairs(); // captured launches carry a half net's out_scale by value
+    }
+
     void upload_params()
     {
+        upload_nets();
         if (params_valid && uploaded_params == law->param_values) {
             return;
         }

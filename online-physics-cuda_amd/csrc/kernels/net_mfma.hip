@@ -103,24 +103,12 @@ HalfNetCard::~HalfNetCard()
     }
 }
 
-std::unique_ptr<HalfNetCard> half_net_create(const NetDecl& net)
+namespace {
+
+// fp16 weights (each layer W[out_pad][in_pad], zero padded) then fp32 biases (b[out_pad]); widths into d.
+void pack_half(const NetDecl& net, HalfNetDesc& d, std::vector<_Float16>& wh, std::vector<float>& bf, std::size_t* woff, std::size_t* boff)
 {
     const int layers = net.layers();
-    if (layers < 1 || layers > kHalfNetMaxLayers) {
-        throw std::runtime_error("net '" + net.name + "': " + std::to_string(layers) + " layers (the matrix-core path takes 1.." +
-            std::to_string(kHalfNetMaxLayers) + ")");
-    }
-    auto card = std::make_unique<HalfNetCard>();
-    HalfNetDesc& d = card->desc;
-    d.layers = layers;
-    d.in_pad = pad16(net.inputs());
-    d.outs = net.outputs();
-    d.act = static_cast<int>(net.act);
-    d.out_scale = net.out_scale;
-    // fp16 weights (each layer W[out_pad][in_pad], zero padded) then fp32 biases (b[out_pad]), each 16-byte aligned.
-    std::vector<_Float16> wh;
-    std::vector<float> bf;
-    std::size_t woff[kHalfNetMaxLayers], boff[kHalfNetMaxLayers];
     int K = d.in_pad;
     for (int l = 0; l < layers; ++l) {
         const int ni = net.sizes[static_cast<std::size_t>(l)], no = net.sizes[static_cast<std::size_t>(l) + 1];
@@ -142,6 +130,28 @@ std::unique_ptr<HalfNetCard> half_net_create(const NetDecl& net)
         }
         K = N;
     }
+}
+
+} // namespace
+
+std::unique_ptr<HalfNetCard> half_net_create(const NetDecl& net)
+{
+    const int layers = net.layers();
+    if (layers < 1 || layers > kHalfNetMaxLayers) {
+        throw std::runtime_error("net '" + net.name + "': " + std::to_string(layers) + " layers (the matrix-core path takes 1.." +
+            std::to_string(kHalfNetMaxLayers) + ")");
+    }
+    auto card = std::make_unique<HalfNetCard>();
+    HalfNetDesc& d = card->desc;
+    d.layers = layers;
+    d.in_pad = pad16(net.inputs());
+    d.outs = net.outputs();
+    d.act = static_cast<int>(net.act);
+    d.out_scale = net.out_scale;
+    std::vector<_Float16> wh;
+    std::vector<float> bf;
+    std::size_t woff[kHalfNetMaxLayers], boff[kHalfNetMaxLayers];
+    pack_half(net, d, wh, bf, woff, boff);
     const std::size_t wbytes = (wh.size() * 2 + 255) / 256 * 256;
     const std::size_t bytes = wbytes + bf.size() * 4;
     if (hipMalloc(&card->blob, bytes) != hipSuccess ||
@@ -154,6 +164,24 @@ std::unique_ptr<HalfNetCard> half_net_create(const NetDecl& net)
         d.b[l] = reinterpret_cast<const float*>(static_cast<const char*>(card->blob) + wbytes) + boff[l];
     }
     return card;
+}
+
+void half_net_update(HalfNetCard& card, const NetDecl& net)
+{
+    HalfNetDesc d = card.desc;
+    d.out_scale = net.out_scale;
+    std::vector<_Float16> wh;
+    std::vector<float> bf;
+    std::size_t woff[kHalfNetMaxLayers], boff[kHalfNetMaxLayers];
+    pack_half(net, d, wh, bf, woff, boff);
+    const std::size_t wbytes = (wh.size() * 2 + 255) / 256 * 256;
+    const hipStream_t s = static_cast<hipStream_t>(fe_hip_stream());
+    if (hipMemcpyAsync(card.blob, wh.data(), wh.size() * 2, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(static_cast<char*>(card.blob) + wbytes, bf.data(), bf.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        throw std::runtime_error("net '" + net.name + "': device copy of its half weights failed");
+    }
+    card.desc.out_scale = net.out_scale;
 }
 
 int half_net_run(const HalfNetCard& net, const void* rows, long long n_rows, float* out)

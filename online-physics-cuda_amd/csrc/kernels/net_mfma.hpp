@@ -35,6 +35,8 @@ struct HalfNetCard {
 // kHalfNetMaxWidth) or the device copy fails.
 std::unique_ptr<HalfNetCard> half_net_create(const NetDecl& net);
 int half_net_in_pad(const NetDecl& net);
+// New weights (same shape) into the card's copy, in place; synchronous.
+void half_net_update(HalfNetCard& card, const NetDecl& net);
 // out[row * outs + o] for n_rows rows of in_pad halves each, queued on the runtime's stream. 0 or a HIP status.
 int half_net_run(const HalfNetCard& net, const void* rows, long long n_rows, float* out);
 

@@ -83,6 +83,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_step": (c_int, [c_void_p, c_double]),
         "fe_solver_run": (c_int, [c_void_p, c_double, c_int]),
         "fe_solver_fill_field": (c_int, [c_void_p, c_int, ctypes.c_float]),
+        "fe_solver_set_net_weights": (c_int, [c_void_p, ctypes.c_char_p, c_void_p, ctypes.c_size_t]),
         "fe_solver_total_accumulate": (c_int, [c_void_p, c_int, c_void_p]),
         "fe_net_half_forward": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, c_void_p, c_void_p, ctypes.c_longlong, c_void_p]),
         "fe_solver_refresh_derived": (c_int, [c_void_p]),
@@ -452,6 +453,12 @@ class Solver:
     def total_accumulate(self, field, acc) -> None:
         """Adds the field's (sum, magnitude) into acc (a DeviceArray of at least 4 floats, read as two doubles)."""
         self._check(lib().fe_solver_total_accumulate(self.ptr, self._field(field), acc.ptr), "total_accumulate")
+
+    def set_net_weights(self, net: str, weights) -> None:
+        import numpy as np
+
+        w = np.ascontiguousarray(weights, np.float32)
+        self._check(lib().fe_solver_set_net_weights(self.ptr, net.encode(), w.ctypes.data, w.size), "set_net_weights")
 
     def fill_field(self, field, value: float) -> None:
         self._check(lib().fe_solver_fill_field(self.ptr, self._field(field), float(value)), "fill_field")

@@ -208,3 +208,60 @@ def test_conduction_with_infinities_and_nan_follows_the_oracle(device, monkeypat
     nan = np.isnan(expect)
     assert np.array_equal(np.isnan(got), nan)  # NaN payloads differ between the CPU and the card; which cells do not
     assert exact(got[~nan], expect[~nan])
+
+
+@pytest.mark.parametrize("name,net", [("net_flux_relu", "g"), ("net_ldiffuse_half", "w")])
+def test_new_net_weights_are_taken_up(device, name, net):
+    # Weights written after the law was built (what a trainer does) reach the card at the next step: a solver whose
+    # weights were replaced runs to the bits of one built from a .net file holding the new weights.
+    import field_hip
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from cases import net_text
+
+    meta, init, _ = golden_io.load(name)
+    old = meta["nets"][net]
+    sizes = [int(v) for v in old.split("sizes ")[1].split("\n")[0].split()]
+    act = old.split("act ")[1].split()[0]
+    new_text = net_text(net, sizes, act, 777, 0.9)
+    weights = np.array([float(v) for v in new_text.split("weights\n")[1].split()], np.float32)
+    shift_scale = {k: old.split(k + " ")[1].split("\n")[0] for k in ("in_shift", "in_scale", "out_scale")}
+    # the new file keeps the old normalisation (set_net_weights changes weights only)
+    lines = new_text.splitlines()
+    lines = [("%s %s" % (l.split()[0], shift_scale[l.split()[0]]) if l.split() and l.split()[0] in shift_scale else l) for l in lines]
+    new_meta = dict(meta, nets={net: "\n".join(lines) + "\n"})
+
+    def run(m, swap):
+        nx, ny, nz, h = m["grid"]
+        s = field_hip.Solver(golden_io.make_law(field_hip, m), (nx, ny, nz), h)
+        for f, v in init.items():
+            s.set_field(f, v)
+        s.refresh_derived()
+        s.set_lazy_readback(True)
+        s.check_contracts_every(1 << 30)
+        s.step(m["dt"])
+        if swap:
+            s.set_net_weights(net, weights)
+        s.run(m["dt"], 3)
+        return {f: s.get_field(f) for f in init}
+
+    changed, fresh, kept = run(meta, True), None, run(meta, False)
+    # the reference run: new weights from the start would differ in the first step, so compare from the same point
+    nx, ny, nz, h = meta["grid"]
+    s = field_hip.Solver(golden_io.make_law(field_hip, new_meta), (nx, ny, nz), h)
+    s0 = field_hip.Solver(golden_io.make_law(field_hip, meta), (nx, ny, nz), h)
+    for f, v in init.items():
+        s0.set_field(f, v)
+    s0.refresh_derived()
+    s0.step(meta["dt"])
+    for f in init:
+        s.set_field(f, s0.get_field(f))
+    s.refresh_derived()
+    s.set_lazy_readback(True)
+    s.check_contracts_every(1 << 30)
+    s.run(meta["dt"], 3)
+    fresh = {f: s.get_field(f) for f in init}
+    for f in init:
+        assert np.array_equal(changed[f].view(np.uint32), fresh[f].view(np.uint32)), f
+    assert any(not np.array_equal(changed[f], kept[f]) for f in init)
