@@ -101,6 +101,16 @@ extern "C" __global__ void k_range(K k, int field, double lo, double hi, int cla
     }
 }
 
+// A clamped field's finite strays into its range, on a step that checks no contract (src/field/Solver.cpp:471-476
+// clamps every step; the reference's CUDA backend launches its k_clamp after every term that writes the field).
+extern "C" __global__ void k_clamp(K k, int field, double lo, double hi)
+{
+    CELL_HEAD
+    const i64 i = IDX(x, y, z);
+    const double d = (double)k.now[field][i];
+    if (finite_d(d) && (d < lo || d > hi)) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
+}
+
 // Conserved totals, pass 1: block b sums cells [b * chunk, (b + 1) * chunk), lane t every 256th from t; then the lanes
 // fold in a fixed tree. out[2b], out[2b+1]: the sum and the sum of magnitudes (each a compensated pair, rounded).
 // With rec not null the same pass is also the field's range contract (k_range's test and clamp, on the value the
@@ -1354,6 +1364,9 @@ private:
         const int R = radius;
         const int NW = static_cast<int>(wf.size());
         const int RS = 2 * R + 2;
+        // gfx950 gives a workgroup up to 160 KB of LDS, but a window that spills (a law with many fields at reach 2)
+        // can reach its ring through flat addresses, and flat accesses past the first 64 KB of LDS fault: a
+        // 76 KB ring of such a kernel did. Windows stay within 64 KB.
         constexpr int kLdsBudget = 64 * 1024;
         struct Shape {
             int vx;
@@ -1382,9 +1395,13 @@ private:
         }
     }
 
-    // A lane's VX cells through the cell function (the fast view m, then the exact view EX when m flagged a cell), then
-    // the stores and change tracking.
-    std::string cells_body(const std::string& cellfn, const std::string& EX, int VX, const std::vector<int>& targets, bool in_place) const
+    // A lane's VX cells through the cell function: the window's fast forms, where any cell flags (a NaN sum: an
+    // infinity or NaN among its inputs) the lane's plane is kept in `redo` and taken again after the last plane,
+    // exactly, by the direct gather's view; then the stores and change tracking. A flagged lane's ping-pong store
+    // still happens (every store unconditional: one under a branch would make the compiler wait for loads in flight
+    // at the next wait) and the redo overwrites it; its in-place stores and change marks wait for the redo, which
+    // reads the cells as they were.
+    std::string cells_body(const std::string& cellfn, int VX, const std::vector<int>& targets, bool in_place) const
     {
         const int NT = static_cast<int>(targets.size());
         std::ostringstream body;
@@ -1393,35 +1410,61 @@ private:
         for (int c = 0; c < VX; ++c) {
             body << "            " << cellfn << "(k, m, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
-        body << "            if (m.bad) {\n                const " << EX << "<P2> mx(m);\n";
+        body << "            const bool fine = !m.bad;\n            if (!fine) redo |= 1ull << (z - z0);\n"
+             << store_body(VX, targets, in_place, "            ", "fine");
+        return body.str();
+    }
+
+    // After a tile's last plane: the lane's flagged planes again, in the exact forms of the direct gather.
+    std::string redo_body(const std::string& cellfn, int VX, const std::vector<int>& targets, bool in_place) const
+    {
+        const int NT = static_cast<int>(targets.size());
+        std::ostringstream body;
+        body << "    if (redo != 0) {\n        const Direct md(k);\n        do {\n"
+             << "            const int z = z0 + __builtin_ctzll(redo); redo &= redo - 1;\n"
+             << "            float o[" << VX << "][" << std::max(1, NT) << "], n[" << VX << "][" << std::max(1, NT) << "];\n";
         for (int c = 0; c < VX; ++c) {
-            body << "                " << cellfn << "(k, mx, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
+            body << "            " << cellfn << "(k, md, x + " << c << ", y, z, o[" << c << "], n[" << c << "]);\n";
         }
-        body << "            }\n";
-        body << "            const i64 i = IDX(x, y, z);\n";
-        // Change tracking (TRACK) only where the step reads the flags back; an in-place store still skips unchanged
-        // cells.
+        body << store_body(VX, targets, in_place, "            ", "true") << "        } while (redo != 0);\n    }\n";
+        return body.str();
+    }
+
+    // The stores of a lane's VX cells at plane z (o: the old values, n: the new) and their change tracking: TRACK
+    // only where the step reads the flags back; an in-place store still skips unchanged cells. `fine`: whether the
+    // values are final (in-place stores and change marks only then).
+    std::string store_body(int VX, const std::vector<int>& targets, bool in_place, const std::string& ind, const std::string& fine) const
+    {
+        const int NT = static_cast<int>(targets.size());
+        std::ostringstream body;
+        body << ind << "{ const i64 i = IDX(x, y, z);\n";
         for (int q = 0; q < NT; ++q) {
             const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
-            body << "            { bool ch = false;\n";
-            for (int c = 0; c < VX; ++c) {
-                body << "              if (TRACK || " << (in_place ? "true" : "false") << ") { const bool d = __float_as_uint(o[" << c << "]["
-                     << q << "]) != __float_as_uint(n[" << c << "][" << q << "]); ch = ch || d;";
-                if (in_place) {
-                    body << " if (d) k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "];";
+            body << ind << "{ bool ch = false;\n";
+            if (!in_place) {
+                // A branch the compiler keeps (the empty asm cannot be hoisted out of it): untracked steps skip the
+                // comparisons instead of computing them and masking the answer.
+                body << ind << "  if (TRACK) { asm volatile(\"\");";
+                for (int c = 0; c < VX; ++c) {
+                    body << " ch = ch || __float_as_uint(o[" << c << "][" << q << "]) != __float_as_uint(n[" << c << "][" << q << "]);";
                 }
                 body << " }\n";
             }
+            for (int c = 0; c < VX && in_place; ++c) {
+                body << ind << "  { const bool d = __float_as_uint(o[" << c << "][" << q << "]) != __float_as_uint(n[" << c << "][" << q
+                     << "]); ch = ch || d; if (d && " << fine << ") k.now[" << t << "][i + " << c << "] = n[" << c << "][" << q << "]; }\n";
+            }
             if (!in_place) {
                 if (VX == 4) {
-                    body << "              *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
+                    body << ind << "  *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
                          << "], n[2][" << q << "], n[3][" << q << "]);\n";
                 } else {
-                    body << "              k.nxt[" << t << "][i] = n[0][" << q << "];\n";
+                    body << ind << "  k.nxt[" << t << "][i] = n[0][" << q << "];\n";
                 }
             }
-            body << "              seen[" << q << "] = seen[" << q << "] || ch; }\n";
+            body << ind << "  seen[" << q << "] = seen[" << q << "] || (ch && " << fine << "); }\n";
         }
+        body << ind << "}\n";
         return body.str();
     }
 
@@ -1442,7 +1485,7 @@ private:
         // The ring at file scope: reads through it keep its 16-byte alignment, so a lane's four neighbouring cells merge
         // into one wide LDS read.
         s << "__shared__ __attribute__((aligned(16))) float " << LDS << "[" << NW * RF << "];\n";
-        // Read through the window in the fast forms (a flagged cell is taken again through WX, the exact forms).
+        // Read through the window in the fast forms (a flagged cell is taken again after the tile, in the exact forms).
         // A read at an offset from the lane's own cells that the compiler knows (a stencil's fixed neighbours, once
         // inlined) is served from registers when it lies in the lane's own column: the cells the lane loaded itself
         // for planes z-R..z+R (`own`), as the LDS ring holds them. Every other read goes through the ring.
@@ -1475,23 +1518,12 @@ private:
               << X0 << ")];\n";
         }
         s << "        else return k.now[F][IDX(x, y, z)];\n    }\n};\n";
-        // The exact view holds a copy of the window's state, not its address: an address taken would keep the
-        // window's registers (its own cells, its offsets) in memory.
-        s << "template <bool P2> struct WX_" << name << " {\n    static constexpr bool kClamps = false;\n    static constexpr bool kExact = true;\n"
-          << "    static constexpr bool kPow2 = P2;\n"
-          << "    " << W << "<P2> w; double u[" << std::max(1, lay_.uniforms) << "];\n"
-          << "    __device__ explicit WX_" << name << "(const " << W << "<P2>& src) : w(src) { for (int q = 0; q < "
-          << std::max(1, lay_.uniforms) << "; ++q) u[q] = src.u[q]; }\n"
-          << "    __device__ void flag(bool) const {}\n"
-          << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return w.template fld<F>(k, x, y, z); }\n"
-          << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const { return w.template fld_any<F>(k, x, y, z); }\n"
-          << "};\n";
         // The kernel's body, instantiated for "h and h^2 are powers of two" (their divisions are multiplications,
         // with no branch between cells) and for the general case.
         s << "template <bool P2> __device__ __forceinline__ void run_" << name
-          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK) {\n"
+          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK, const bool REMAP) {\n"
           << "    float* const lds = " << LDS << ";\n"
-          << "    const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z);\n"
+          << "    const int t = REMAP ? xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z) : (int)blockIdx.x;\n"
           // x0 passes through an empty asm: the compiler then knows nothing of its low bits, so a neighbour's x + c
           // stays an add (not an or), and its offset from the lane's own cell, x + c - x, folds to c.
           << "    int x0 = (t % tiles_x) * " << SPAN << "; asm volatile(\"\" : \"+s\"(x0)); const int y0 = ((t / tiles_x) % tiles_y) * "
@@ -1564,9 +1596,9 @@ private:
             s << "    };\n";
         }
         // The cells of a lane: their new values and the stores.
-        const std::string body = cells_body(cellfn, "WX_" + name, VX, targets, in_place);
+        const std::string body = cells_body(cellfn, VX, targets, in_place);
         // A tile's change flags are marked once, after its last plane (a store a plane would be waited on with the loads).
-        s << "    bool seen[" << std::max(1, NT) << "] = {};\n";
+        s << "    bool seen[" << std::max(1, NT) << "] = {};\n    unsigned long long redo = 0; // planes z0 + b whose cells a NaN sum flagged\n";
         s << "    for (int zp = z0 - " << R << "; zp < z0 + " << R << "; ++zp) { Pl p; load(p, zp); put(p, zp); keep(p, zp - z0 + " << R
           << "); }\n"
           << "    Pl A, B, C, D;\n"
@@ -1590,14 +1622,15 @@ private:
           << "        step(z, A, D); step(z + 1, B, A); step(z + 2, C, B); step(z + 3, D, C);\n"
           << "    }\n"
           << "    if (z < z1) { step(z, A, D); if (z + 1 < z1) { step(z + 1, B, A); if (z + 2 < z1) step(z + 2, C, B); } }\n";
+        s << redo_body(cellfn, VX, targets, in_place);
         for (int q = 0; q < NT; ++q) {
             s << "    if (TRACK) mark(k, " << targets[static_cast<std::size_t>(q)] << ", seen[" << q << "]);\n";
         }
         s << "}\n";
         s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
-          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int track) {\n"
-          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z, track != 0);\n"
-          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z, track != 0);\n}\n";
+          << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int mode) {\n"
+          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0);\n"
+          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0);\n}\n";
         out_ << s.str();
     }
 

@@ -56,8 +56,11 @@ struct StageCode {
 };
 
 // A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of
-// tiles_x * tiles_y * tiles_z blocks of 64 x block_y threads with (K, zchunk, tiles_x, tiles_y, tiles_z); a tile is
-// 64 * vx cells by ty rows by zchunk planes. vx = 4 needs nx a multiple of 4.
+// tiles_x * tiles_y * tiles_z blocks of 64 x block_y threads with (K, zchunk, tiles_x, tiles_y, tiles_z, mode: bit 0
+// track changes, bit 1 spread consecutive tiles over the XCDs in runs); a tile is
+// 64 * vx cells by ty rows by zchunk planes (at most kMaxZchunk: a lane keeps one bit a plane). vx = 4 needs nx a
+// multiple of 4.
+constexpr int kMaxZchunk = 64;
 struct WindowKernel {
     std::string name;
     int vx = 1;

@@ -207,6 +207,17 @@ term decay
 """
 
 
+CLAMP_LAW = """# a clamped field a term pushes past both ends of its range every step, and a clamped derived field
+field a init 0 range 0 1 clamp
+field b init 0 range -1000 1000
+field c init 0 range 0 2 clamp
+derive c = 3 * a + b
+
+term push
+  d a = 4 * (b - 0.5) + diffuse(0.05, a)
+"""
+
+
 def _uniform(n, seed, lo=0.0, hi=1.0):
     rng = np.random.default_rng(seed)
     return (lo + (hi - lo) * rng.random(int(n))).astype(np.float32)
@@ -358,6 +369,8 @@ CASES = [
     dict(name="multi", law=MULTI_LAW, grid=(18, 11, 7, 0.4), dt=0.25, steps=5, params={},
          init={"a": _uniform(18 * 11 * 7, 14, -1.0, 1.0), "b": _uniform(18 * 11 * 7, 15, 0.0, 1.9),
                "c": _uniform(18 * 11 * 7, 16, 0.0, 2.0)}),
+    dict(name="clamp_every_step", law=CLAMP_LAW, grid=(16, 9, 5, 0.5), dt=0.1, steps=6, params={},
+         init={"a": _uniform(16 * 9 * 5, 17), "b": _uniform(16 * 9 * 5, 18)}),
     dict(name="heston_cs_3", law=HESTON_CS_LAW, grid=(32, 16, 3, 6.0 / 32), dt=0.5 / 8, steps=8, params={},
          init=heston_state(32, 16, HESTON_PLANES)),
     dict(name="heston_hv_3", law=HESTON_HV_LAW, grid=(32, 16, 3, 6.0 / 32), dt=0.5 / 8, steps=8, params={},

@@ -55,6 +55,30 @@ def test_two_runs_and_lazy_readback_agree_to_the_bit(device, name):
     assert a.digest() == b.digest()
 
 
+@pytest.mark.parametrize("how", ["step", "lazy", "run"])
+def test_clamped_fields_are_clamped_on_steps_that_check_no_contract(device, how):
+    """The reference CPU clamps a clamped field after every term that writes it and after the derives, every step;
+    here contracts are checked once in 1000 steps, so only the clamp kernel can keep the fields in range."""
+    import field_hip
+
+    meta, init, final = golden_io.load("clamp_every_step")
+    nx, ny, nz, h = meta["grid"]
+    solver = field_hip.Solver(golden_io.make_law(field_hip, meta), (nx, ny, nz), h)
+    for name, values in init.items():
+        solver.set_field(name, values)
+    solver.refresh_derived()
+    solver.set_lazy_readback(how != "step")
+    solver.check_contracts_every(1000)
+    solver.step(meta["dt"])  # step 0 checks contracts; the others do not
+    if how == "run":
+        solver.run(meta["dt"], meta["steps"] - 1)
+    else:
+        for _ in range(meta["steps"] - 1):
+            solver.step(meta["dt"])
+    for field, expected in final.items():
+        assert exact(solver.get_field(field), expected), field
+
+
 def test_contracts_report_a_leak_and_a_range_breach(device):
     import field_hip
 

@@ -56,6 +56,11 @@ def test_generated_source_compiles_for_gfx950(name):
             "-Xclang", "-packed-fp32-ops"]
     blob = field_hip.compile_hip(src, "law.hip", opts)
     assert len(blob) > 1000
+    # every window's LDS ring within 64 KB (a spilling window may reach it through flat addresses, which fault past
+    # the first 64 KB)
+    import re
+    rings = [int(n) * 4 for n in re.findall(r"__shared__ __attribute__\(\(aligned\(16\)\)\) float L_\w+\[(\d+)\]", src)]
+    assert all(b <= 64 * 1024 for b in rings), rings
 
 
 @pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="the reference's laws are only in the build container")
