@@ -18,6 +18,7 @@ namespace {
 // lane a fixed stride of it, the lanes in a fixed tree; the partials fold in one block, in order. No float atomics.
 const char* kPrelude = R"HIP(
 typedef long long i64;
+typedef float f4v __attribute__((ext_vector_type(4)));
 typedef unsigned long long u64;
 #define IDX(x, y, z) ((i64)(z) * k.nxy + (i64)(y) * k.nx + (i64)(x))
 #define AT(slot, x, y, z) k.A[(i64)(slot) * k.cells + IDX(x, y, z)]
@@ -1456,8 +1457,11 @@ private:
             }
             if (!in_place) {
                 if (VX == 4) {
-                    body << ind << "  *reinterpret_cast<float4*>(k.nxt[" << t << "] + i) = make_float4(n[0][" << q << "], n[1][" << q
-                         << "], n[2][" << q << "], n[3][" << q << "]);\n";
+                    // STREAM: a streaming (nontemporal) store, for lattices past the last-level cache (the empty asm
+                    // around it keeps the compiler from hoisting or sinking the two stores into one ordinary store)
+                    body << ind << "  { const f4v nv = {n[0][" << q << "], n[1][" << q << "], n[2][" << q << "], n[3][" << q << "]};\n"
+                         << ind << "    if (STREAM) { asm volatile(\"\"); __builtin_nontemporal_store(nv, reinterpret_cast<f4v*>(k.nxt[" << t << "] + i)); asm volatile(\"\"); }\n"
+                         << ind << "    else *reinterpret_cast<f4v*>(k.nxt[" << t << "] + i) = nv; }\n";
                 } else {
                     body << ind << "  k.nxt[" << t << "][i] = n[0][" << q << "];\n";
                 }
@@ -1521,7 +1525,7 @@ private:
         // The kernel's body, instantiated for "h and h^2 are powers of two" (their divisions are multiplications,
         // with no branch between cells) and for the general case.
         s << "template <bool P2> __device__ __forceinline__ void run_" << name
-          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK, const bool REMAP) {\n"
+          << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK, const bool REMAP, const bool STREAM) {\n"
           << "    float* const lds = " << LDS << ";\n"
           << "    const int t = REMAP ? xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z) : (int)blockIdx.x;\n"
           // x0 passes through an empty asm: the compiler then knows nothing of its low bits, so a neighbour's x + c
@@ -1629,8 +1633,8 @@ private:
         s << "}\n";
         s << "extern \"C\" __global__ __launch_bounds__(" << 64 * (TY + 2 * R) << ") void " << name
           << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int mode) {\n"
-          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0);\n"
-          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0);\n}\n";
+          << "    if ((k.flags & 3) == 3) run_" << name << "<true>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0, (mode & 4) != 0);\n"
+          << "    else run_" << name << "<false>(k, zchunk, tiles_x, tiles_y, tiles_z, (mode & 1) != 0, (mode & 2) != 0, (mode & 4) != 0);\n}\n";
         out_ << s.str();
     }
 
