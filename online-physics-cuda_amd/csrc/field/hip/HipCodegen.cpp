@@ -1349,7 +1349,7 @@ private:
 
     // Window gathers: tiles of cells marching in z, the fields read off-cell (`wf`, reach <= R) from an LDS ring of
     // planes. The ring holds planes z-R..z+R and the one being filled (RS >= 2R + 2 slots, a slot a plane, slot of
-    // plane p = (p + RS) % RS); each plane of each field is loaded once by its tile, rows and halo columns at clamped
+    // plane p = (p + RS) % RS, in unsigned arithmetic: p >= -R, so p + RS > 0); each plane of each field is loaded once by its tile, rows and halo columns at clamped
     // coordinates, so every coordinate within R of a cell holds the clamped cell's value (what a clamped read gets),
     // three planes ahead of use through a four-slot register ring (unrolled: register copies of loads in flight would
     // make the compiler wait for them). One barrier a plane: the slot being filled is one no lane reads that plane.
@@ -1511,14 +1511,14 @@ private:
               << "#pragma unroll\n                    for (int c = 0; c < " << VX << "; ++c) v = (dz + " << R << " == j && dx == c) ? own[" << w
               << "][j][c] : v;\n"
               << "                return v;\n            }\n"
-              << "            return " << LDS << "[" << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R
+              << "            return " << LDS << "[" << w * RF << " + ((unsigned)(z + " << RS << ") % " << RS << "u) * " << TP << " + (y - y0 + " << R
               << ") * " << TW << " + (x - x0 + " << X0 << ")];\n        }\n";
         }
         s << "        else return k.now[F][IDX(x, y, z)];\n    }\n"
           << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const {\n";
         for (int w = 0; w < NW; ++w) {
             s << "        " << (w ? "else " : "") << "if constexpr (F == " << wf[static_cast<std::size_t>(w)] << ") return " << LDS << "["
-              << w * RF << " + ((z + " << RS << ") % " << RS << ") * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + "
+              << w * RF << " + ((unsigned)(z + " << RS << ") % " << RS << "u) * " << TP << " + (y - y0 + " << R << ") * " << TW << " + (x - x0 + "
               << X0 << ")];\n";
         }
         s << "        else return k.now[F][IDX(x, y, z)];\n    }\n};\n";
@@ -1556,7 +1556,7 @@ private:
                 s << "        p.v[" << w << "] = k.now[" << f << "][o + xs]; p.h[" << w << "] = k.now[" << f << "][o + hx];\n";
             }
             s << "    };\n"
-              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((zp + " << RS << ") % " << RS << ") * " << TP << ";\n";
+              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((unsigned)(zp + " << RS << ") % " << RS << "u) * " << TP << ";\n";
             for (int w = 0; w < NW; ++w) {
                 s << "        sl[" << w * RF << " + " << R << " + lane] = p.v[" << w << "]; if (hslot >= 0) sl[" << w * RF << " + hslot] = p.h["
                   << w << "];\n";
@@ -1585,7 +1585,7 @@ private:
                   << "        for (int j = 0; j < " << R << "; ++j) p.h[" << w << "][j] = k.now[" << f << "][o + hx[j]];\n";
             }
             s << "    };\n"
-              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((zp + " << RS << ") % " << RS << ") * " << TP << ";\n";
+              << "    auto put = [&](const Pl& p, int zp) { float* sl = row + ((unsigned)(zp + " << RS << ") % " << RS << "u) * " << TP << ";\n";
             for (int w = 0; w < NW; ++w) {
                 s << "        { float4 v = p.v[" << w << "]; if (dead) v = make_float4(v.w, v.w, v.w, v.w);\n"
                   << "          *reinterpret_cast<float4*>(sl + " << w * RF << " + 4 + 4 * lane) = v;\n"
