@@ -44,7 +44,9 @@ int fe_heston_lattice(fe_heston* b, int option, float* out);
 /* Lattice classes (options of one nx) and their launch shapes: lanes a row in the row sweeps (16, 32, 64: the
  * Thomas-against-PCR split, nx / lanes cells a lane in 2..16), waves a block there (1, 2, 4), threads a block in the
  * column sweeps (64, 128, 256; -64: 64 threads keeping the forward sweep's results in LDS instead of HBM, for classes
- * of at most 256 rows). Setting a shape drops a captured graph. */
+ * of at most 256 rows; -8, -16: the column split over waves, that many rows a thread in registers and the chunks'
+ * equations solved through LDS, 64 columns by ceil(rows / 8 or 16) waves a block, for classes of at most 16 such
+ * chunks). Setting a shape drops a captured graph. */
 int fe_heston_classes(fe_heston* b);
 int fe_heston_get_shape(fe_heston* b, int cls, int* nx, int* lanes, int* row_waves, int* col_threads);
 int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col_threads);

@@ -53,6 +53,7 @@ struct HRow {
     float lx, ux, ly, uy; // A1, A2 coefficients of the row
     float rv;             // rho * var of the row
     float ya, ycp, ym;    // (I - th dt A2) factorised: lower coefficient, c', 1 / pivot (edges folded)
+    float yb, yc;         // (I - th dt A2) itself: the diagonal (edges folded) and the upper coefficient (0 on the last row)
 };
 
 enum Mode { kFirst = 0, kCsSecond = 1, kHvSecond = 2 };
@@ -418,6 +419,144 @@ __global__ __launch_bounds__(64) void k_cols_lds(const HOpt* __restrict__ opts, 
     }
 }
 
+// Passes 2 and 4 partitioned: a block takes 64 columns of one option, P = blockDim.y waves, wave p the rows
+// [p M, (p + 1) M) of every column in registers (rows past the option's ny are x = 0 equations, which couple to
+// nothing: the last real row's upper coefficient is folded). Each thread reduces its chunk as the row solver does (a
+// downward and an upward sweep express every unknown in the chunk's last unknown and the one before the chunk); the P
+// equations of a column in those unknowns are solved down the column by one thread (Thomas through LDS; measured
+// faster than parallel cyclic reduction over the waves, whose extra barriers and divisions cost more than the idle
+// waves), and each thread substitutes back. HBM traffic is the pass's input and output once: 8 bytes a cell (16 with the update of U),
+// against 16 (24) for k_cols. The arithmetic differs from the Thomas sweep's, so its bits do too (both fp32).
+template <int M, bool FINAL>
+__global__ __launch_bounds__(1024) void k_cols_part(const HOpt* __restrict__ opts, const HRow* __restrict__ rows,
+    const Task* __restrict__ tasks, float* __restrict__ U, float* __restrict__ D2, const float* __restrict__ T1)
+{
+    static_assert(M >= 2, "a chunk holds two rows at least");
+    extern __shared__ float red[]; // [8][P][64]: first-row terms P0 Q0 R0, the chunk equations al be ga de, u
+    const Task t = tasks[blockIdx.x];
+    const HOpt o = opts[t.opt];
+    // blockDim.x is the wave: p is the same across a wave, so the row coefficients are scalar loads
+    const int c = (int)threadIdx.x, p = __builtin_amdgcn_readfirstlane((int)threadIdx.y), P = (int)blockDim.y;
+    const int i = t.first + c;
+    const bool on = i < o.nx;
+    const int ic = on ? i : o.nx - 1;
+    const int ny = o.ny;
+    const long long nx = o.nx;
+    const HRow* cf = rows + o.row;
+    const int j0 = p * M;
+    const int stride = P * 64;
+    float* sP = red;
+    float* sQ = red + stride;
+    float* sR = red + 2 * stride;
+    float* sAl = red + 3 * stride;
+    float* sBe = red + 4 * stride;
+    float* sGa = red + 5 * stride;
+    float* sDe = red + 6 * stride;
+    float* sU = red + 7 * stride;
+    const int me = p * 64 + c;
+    // every load unconditional (clamped rows and column), padding selected afterwards
+    float x[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = min(j0 + q, ny - 1);
+        x[q] = T1[o.off + j * nx + ic];
+    }
+    float C[M], A[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = j0 + q;
+        const bool real = j < ny;
+        const HRow r = cf[min(j, ny - 1)];
+        const float a = real && j > 0 ? r.ya : 0.0f;
+        const float b = real ? r.yb : 1.0f;
+        const float cc = real ? r.yc : 0.0f;
+        const float d = real ? x[q] : 0.0f;
+        if (q == 0) {
+            const float m = 1.0f / b;
+            C[0] = cc * m;
+            A[0] = a * m;
+            x[0] = d * m;
+        } else {
+            const float m = 1.0f / (b - a * C[q - 1]);
+            C[q] = cc * m;
+            A[q] = -a * A[q - 1] * m;
+            x[q] = (d - a * x[q - 1]) * m;
+        }
+    }
+    // upward: x_q = x[q] - C[q] x_e - A[q] x_(s-1) for q < M - 1 (x[] then holds P, C holds Q, A holds R)
+    const float ce = C[M - 1], ae = A[M - 1], de = x[M - 1];
+#pragma unroll
+    for (int q = M - 3; q >= 0; --q) {
+        x[q] = x[q] - C[q] * x[q + 1];
+        A[q] = A[q] - C[q] * A[q + 1];
+        C[q] = -C[q] * C[q + 1];
+    }
+    sP[me] = x[0];
+    sQ[me] = C[0];
+    sR[me] = A[0];
+    __syncthreads();
+    {
+        const bool tail = p == P - 1;
+        const float p1 = tail ? 0.0f : sP[me + 64], q1 = tail ? 0.0f : sQ[me + 64], r1 = tail ? 0.0f : sR[me + 64];
+        sAl[me] = ae;
+        sBe[me] = 1.0f - ce * r1;
+        sGa[me] = -ce * q1;
+        sDe[me] = de - ce * p1;
+    }
+    __syncthreads();
+    if (p == 0) {
+        // Thomas down the column's P chunk equations (al u_(k-1) + be u_k + ga u_(k+1) = de), c' and d' kept in place
+        float cp = 0.0f, dp = 0.0f;
+        for (int k = 0; k < P; ++k) {
+            const int s = k * 64 + c;
+            const float al = k > 0 ? sAl[s] : 0.0f;
+            const float m = 1.0f / (sBe[s] - al * cp);
+            cp = sGa[s] * m;
+            dp = (sDe[s] - al * dp) * m;
+            sGa[s] = cp;
+            sDe[s] = dp;
+        }
+        float u = 0.0f;
+        for (int k = P - 1; k >= 0; --k) {
+            const int s = k * 64 + c;
+            u = sDe[s] - sGa[s] * u;
+            sU[s] = u;
+        }
+    }
+    __syncthreads();
+    const float u = sU[me];
+    const float ul = p > 0 ? sU[me - 64] : 0.0f;
+    x[M - 1] = u;
+#pragma unroll
+    for (int q = 0; q < M - 1; ++q) {
+        x[q] = x[q] - C[q] * u - A[q] * ul;
+    }
+    if (FINAL) {
+        // (no loads moved above the substitution: with them in flight beside x, C and A a 1024-thread block spills)
+        __builtin_amdgcn_sched_barrier(0);
+        float a[M], b[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const long long at = o.off + min(j0 + q, ny - 1) * nx + ic;
+            a[q] = U[at];
+            b[q] = D2[at];
+        }
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            if (on && j0 + q < ny) {
+                U[o.off + (j0 + q) * nx + i] = a[q] + (b[q] + x[q]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            if (on && j0 + q < ny) {
+                D2[o.off + (j0 + q) * nx + i] = x[q];
+            }
+        }
+    }
+}
+
 // The cell-averaged put payoff (in double, rounded once).
 __global__ void k_payoff(const HOpt* __restrict__ opts, const double* __restrict__ strikes, int n, float* __restrict__ U)
 {
@@ -536,6 +675,7 @@ struct fe_heston {
         int row_waves = 4;  // waves a block in the row passes
         int col_threads = 64; // threads a block in the column passes
         bool col_lds = false;  // the column passes keep d' in LDS (k_cols_lds; 64 threads a block, ny_max <= 256)
+        int col_part = 0;      // M > 0: the column passes partitioned, M rows a thread (k_cols_part; 64 x P threads)
         Dev<Task> row_tasks, col_tasks;
         int n_row_tasks = 0, n_col_tasks = 0;
     };
@@ -586,6 +726,27 @@ struct fe_heston {
 
     static unsigned lds_bytes(const Class& c) { return static_cast<unsigned>(c.ny_max) * 64u * 4u; }
     static bool lds_fits(const Class& c) { return c.ny_max >= 1 && c.ny_max <= 256; }
+    // The partitioned column passes: P = ceil(ny_max / M) waves a block (at most 16: 1024 threads).
+    static int part_waves(const Class& c, int m) { return (c.ny_max + m - 1) / m; }
+    static bool part_fits(const Class& c, int m) { return (m == 8 || m == 16) && c.ny_max >= 1 && part_waves(c, m) <= 16; }
+    void launch_cols_part(const Class& c, bool final_pass, hipStream_t s)
+    {
+        const int pw = part_waves(c, c.col_part);
+        const dim3 block(64, static_cast<unsigned>(pw));
+        const unsigned shm = 8u * static_cast<unsigned>(pw) * 64u * 4u;
+        const dim3 grid(static_cast<unsigned>(c.n_col_tasks));
+        if (c.col_part == 8) {
+            if (final_pass) {
+                hipLaunchKernelGGL((k_cols_part<8, true>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+            } else {
+                hipLaunchKernelGGL((k_cols_part<8, false>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+            }
+        } else if (final_pass) {
+            hipLaunchKernelGGL((k_cols_part<16, true>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+        } else {
+            hipLaunchKernelGGL((k_cols_part<16, false>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+        }
+    }
 
     void queue_step() { queue_step_of(-1); }
 
@@ -606,6 +767,8 @@ struct fe_heston {
                     const float* src = pass == 2 && scheme == 0 ? D2.p : U.p;
                     hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(c.n_row_tasks)), dim3(64 * c.row_waves), 0, s, opts.p, rows.p,
                         c.row_tasks.p, src, D2.p, T1.p);
+                } else if (c.col_part > 0) {
+                    launch_cols_part(c, pass == 3, s);
                 } else if (pass == 1) {
                     if (c.col_lds) {
                         hipLaunchKernelGGL(k_cols_lds<false>, dim3(static_cast<unsigned>(c.n_col_tasks)), dim3(64), lds_bytes(c), s, opts.p,
@@ -680,31 +843,48 @@ struct fe_heston {
             c.row_waves = bw;
             best = 1e30f;
             bool blds = false;
-            for (const int threads : {64, 128, 256, -64}) { // -64: 64 threads, d' in LDS
-                if (threads < 0 && !lds_fits(c)) {
+            int bpart = 0;
+            // -64: 64 threads, d' in LDS; -8, -16: partitioned, 8 or 16 rows a thread
+            for (const int threads : {64, 128, 256, -64, -8, -16}) {
+                if ((threads == -64 && !lds_fits(c)) || ((threads == -8 || threads == -16) && !part_fits(c, -threads))) {
                     continue;
                 }
-                c.col_threads = threads < 0 ? -threads : threads;
-                c.col_lds = threads < 0;
+                set_cols(c, threads);
                 build_tasks(c);
                 const float ms = time_class(static_cast<int>(ci), reps);
                 if (ms < best) {
                     best = ms;
                     bc = c.col_threads;
                     blds = c.col_lds;
+                    bpart = c.col_part;
                 }
             }
             c.col_threads = bc;
             c.col_lds = blds;
+            c.col_part = bpart;
             build_tasks(c);
             log << (ci ? "," : "") << "{\"nx\":" << c.nx << ",\"options\":" << c.members.size() << ",\"lanes\":" << c.lanes
                 << ",\"row_waves\":" << c.row_waves << ",\"col_threads\":" << c.col_threads << ",\"col_lds\":" << (c.col_lds ? "true" : "false")
+                << ",\"col_rows_a_thread\":" << c.col_part
                 << ",\"ms_a_step\":" << best / reps << "}";
         }
         log << "]";
         reset();
         hcheck(hipStreamSynchronize(stream()), "sync");
         return log.str();
+    }
+
+    // The column passes' launch shape from its code: 64, 128 or 256 threads a column each; -64 d' in LDS; -8 or -16
+    // partitioned with that many rows a thread.
+    static void set_cols(Class& c, int code)
+    {
+        c.col_lds = code == -64;
+        c.col_part = code == -8 || code == -16 ? -code : 0;
+        c.col_threads = code > 0 ? code : 64;
+    }
+    static int cols_code(const Class& c)
+    {
+        return c.col_part > 0 ? -c.col_part : (c.col_lds ? -64 : c.col_threads);
     }
 
     void reset()
@@ -818,6 +998,8 @@ fe_heston* fe_heston_create(int n, const int* nx, const int* ny, const double* o
                 hr.ya = static_cast<float>(a);
                 hr.ycp = static_cast<float>(cp);
                 hr.ym = static_cast<float>(1.0 / piv);
+                hr.yb = static_cast<float>(bd);
+                hr.yc = static_cast<float>(j + 1 == Y ? 0.0 : c);
                 cprev = cp;
                 rows.push_back(hr);
             }
@@ -1024,7 +1206,7 @@ int fe_heston_get_shape(fe_heston* b, int cls, int* nx, int* lanes, int* row_wav
     *nx = c.nx;
     *lanes = c.lanes;
     *row_waves = c.row_waves;
-    *col_threads = c.col_lds ? -c.col_threads : c.col_threads;
+    *col_threads = fe_heston::cols_code(c);
     return 0;
 }
 
@@ -1037,7 +1219,8 @@ int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col
     const int k = lanes > 0 ? c.nx / lanes : 0;
     if (!(lanes == 16 || lanes == 32 || lanes == 64) || c.nx % lanes != 0 || !(k == 2 || k == 4 || k == 8 || k == 16) ||
         !(row_waves == 1 || row_waves == 2 || row_waves == 4) ||
-        !(col_threads == 64 || col_threads == 128 || col_threads == 256 || (col_threads == -64 && fe_heston::lds_fits(c)))) {
+        !(col_threads == 64 || col_threads == 128 || col_threads == 256 || (col_threads == -64 && fe_heston::lds_fits(c)) ||
+            ((col_threads == -8 || col_threads == -16) && fe_heston::part_fits(c, -col_threads)))) {
         return fail(b, "shape out of range");
     }
     try {
@@ -1048,8 +1231,7 @@ int fe_heston_set_shape(fe_heston* b, int cls, int lanes, int row_waves, int col
         hcheck(hipStreamSynchronize(b->stream()), "sync");
         c.lanes = lanes;
         c.row_waves = row_waves;
-        c.col_threads = col_threads < 0 ? -col_threads : col_threads;
-        c.col_lds = col_threads < 0;
+        fe_heston::set_cols(c, col_threads);
         b->build_tasks(c);
         return 0;
     } catch (const std::exception& e) {

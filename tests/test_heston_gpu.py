@@ -61,7 +61,10 @@ def test_every_launch_shape_matches(device):
             k = shape["nx"] // lanes
             if k not in (2, 4, 8, 16):
                 continue
-            for row_waves, col_threads in ((1, 64), (2, 128), (4, 256), (2, -64)):
+            ny_max = int(ny[nx == shape["nx"]].max())
+            # -64: d' in LDS; -8, -16: partitioned columns, that many rows a thread (at most 16 waves a block)
+            cols = [(1, 64), (2, 128), (4, 256), (2, -64), (4, -16)] + ([(1, -8)] if ny_max <= 128 else [])
+            for row_waves, col_threads in cols:
                 print(f"class {shape['nx']} lanes {lanes} waves {row_waves} cols {col_threads}", flush=True)
                 b.set_shape(cls, lanes, row_waves, col_threads)
                 b.reset()
@@ -71,6 +74,34 @@ def test_every_launch_shape_matches(device):
                 for o in np.flatnonzero(nx == shape["nx"]):
                     ref = V[offsets[o]:offsets[o] + nx[o] * ny[o]]
                     assert np.max(np.abs(b.lattice(o) - ref)) / np.max(np.abs(ref)) < LATTICE_RTOL
+
+
+@pytest.mark.parametrize("scheme,theta", [(0, 0.5), (1, 0.5 + np.sqrt(3) / 6)], ids=["cs", "hv"])
+def test_partitioned_columns_on_uneven_heights(device, scheme, theta):
+    """Column passes split over waves (rows a thread in registers, the chunk equations through LDS): heights that are
+    no multiple of the chunk, options of one class with different heights, a class needing 16 waves."""
+    import field_hip
+
+    rng = np.random.default_rng(31)
+    nx = np.array([128, 128, 128, 256, 256, 64, 64], np.int32)
+    ny = np.array([21, 100, 64, 256, 131, 5, 37], np.int32)
+    _, _, opts = draw(len(nx), 32)
+    opts[:, 6] = rng.uniform(0.25, 1.0, len(nx))
+    print("lattices", list(zip(nx, ny)), flush=True)
+    ref_prices, V, offsets = heston.batch64(nx, ny, opts, 16, scheme, theta)
+    b = field_hip.HestonBatch(nx, ny, opts, 16, scheme, theta)
+    for rows_a_thread in (8, 16):
+        for cls, shape in enumerate(b.shapes()):
+            ny_max = int(ny[nx == shape["nx"]].max())
+            if -(-ny_max // rows_a_thread) <= 16:
+                b.set_shape(cls, shape["lanes"], shape["row_waves"], -rows_a_thread)
+        print(f"rows a thread {rows_a_thread}: {b.shapes()}", flush=True)
+        b.reset()
+        b.step(16)
+        assert np.max(np.abs(b.prices() - ref_prices)) < PRICE_ATOL
+        for o in range(len(nx)):
+            ref = V[offsets[o]:offsets[o] + nx[o] * ny[o]]
+            assert np.max(np.abs(b.lattice(o) - ref)) / np.max(np.abs(ref)) < LATTICE_RTOL, (o, nx[o], ny[o])
 
 
 def test_captured_graph_equals_queued_steps_to_the_bit(device):
