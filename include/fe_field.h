@@ -107,6 +107,12 @@ void* fe_solver_field_device(fe_solver* s, int field);
  * (checkpoints, adjoint sweeps); call it once the solver has uploaded its fields (a step or refresh_derived). No
  * counterpart in the reference, whose states travel through the host (tools/lawc/main.cpp:310-323). */
 int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
+/* The window moved by (dx, dy, dz) cells (Solver::shift_window, src/field/Solver.hpp:40-42): every field's host cells
+ * slide (what is now at c was at c + d) and the arriving boxes (n_boxes of x0 y0 z0 x1 y1 z1, half-open, inside the
+ * lattice) take `arriving`: for each field in order, each box in order, its cells x fastest. Before the next step the
+ * card slides its own copy and receives only those boxes (the reference's CUDA backend moves a ring's origin,
+ * src/field/cuda/CudaSolver.cpp:895-920). -1: bad arguments or a box outside the lattice. */
+int fe_solver_shift_window(fe_solver* s, int dx, int dy, int dz, const int* boxes, int n_boxes, const float* arriving);
 /* Every cell of a field set to `value` on the card (no host copy; the host copy goes stale as after a device write).
  * No counterpart in the reference: the reverse-mode driver clears its per-step parameter adjoints with it. */
 int fe_solver_fill_field(fe_solver* s, int field, float value);

@@ -318,6 +318,54 @@ int fe_solver_set_net_weights(fe_solver* s, const char* net, const float* weight
     return 0;
 }
 
+int fe_solver_shift_window(fe_solver* s, int dx, int dy, int dz, const int* boxes, int n_boxes, const float* arriving)
+{
+    if (s == nullptr || !s->solver || n_boxes < 0 || (n_boxes > 0 && (boxes == nullptr || arriving == nullptr))) {
+        return -1;
+    }
+    const fieldengine::field::GridShape g = s->solver->fields_const().shape();
+    std::vector<fieldengine::field::CellBox> arriving_boxes;
+    for (int b = 0; b < n_boxes; ++b) {
+        const int* q = boxes + 6 * b;
+        const fieldengine::field::CellBox box{q[0], q[1], q[2], q[3], q[4], q[5]};
+        if (box.x0 < 0 || box.y0 < 0 || box.z0 < 0 || box.x1 > g.nx || box.y1 > g.ny || box.z1 > g.nz || box.x0 > box.x1 ||
+            box.y0 > box.y1 || box.z0 > box.z1) {
+            s->error = "shift_window: a box outside the lattice";
+            return -1;
+        }
+        arriving_boxes.push_back(box);
+    }
+    return guarded(s, [&] {
+        fieldengine::field::FieldSet& f = s->solver->shift_window(dx, dy, dz, arriving_boxes);
+        const std::size_t nx = static_cast<std::size_t>(g.nx), nxy = nx * static_cast<std::size_t>(g.ny);
+        const float* in = arriving;
+        for (int field = 0; field < static_cast<int>(f.count()); ++field) {
+            std::vector<float>& v = f.values(field);
+            // what is now at c was at c + d (cells whose source is outside keep their value until a box fills them)
+            const std::vector<float> old = v;
+            for (int z = 0; z < g.nz; ++z) {
+                for (int y = 0; y < g.ny; ++y) {
+                    for (int x = 0; x < g.nx; ++x) {
+                        const int sx = x + dx, sy = y + dy, sz = z + dz;
+                        if (sx >= 0 && sx < g.nx && sy >= 0 && sy < g.ny && sz >= 0 && sz < g.nz) {
+                            v[z * nxy + y * nx + x] = old[sz * nxy + sy * nx + sx];
+                        }
+                    }
+                }
+            }
+            for (const fieldengine::field::CellBox& b : arriving_boxes) {
+                for (int z = b.z0; z < b.z1; ++z) {
+                    for (int y = b.y0; y < b.y1; ++y) {
+                        for (int x = b.x0; x < b.x1; ++x) {
+                            v[z * nxy + y * nx + x] = *in++;
+                        }
+                    }
+                }
+            }
+        }
+    });
+}
+
 int fe_solver_fill_field(fe_solver* s, int field, float value)
 {
     if (!valid_field(s, field) || s->solver->backend() == nullptr) {

@@ -26,6 +26,9 @@ public:
     [[nodiscard]] const std::vector<float>& values(int field); // fetches the one field when lazy
     float& edit_cell(int field, std::size_t cell);
     void mark_field_edited(int field);
+    // The window moved by (dx, dy, dz) cells (src/field/Solver.hpp:40-42): the caller slides the host cells (what is now
+    // at c was at c + d) and fills `arriving`; the backend slides its own copy on the card and uploads only those boxes.
+    [[nodiscard]] FieldSet& shift_window(int dx, int dy, int dz, const std::vector<CellBox>& arriving);
 
     bool set_param(std::string_view name, double value);
     [[nodiscard]] std::optional<double> param(std::string_view name) const;

@@ -112,6 +112,24 @@ extern "C" __global__ void k_clamp(K k, int field, double lo, double hi)
     if (finite_d(d) && (d < lo || d > hi)) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
 }
 
+// The window moved by d (src/field/Backend.hpp:33-40): dst gets what src held at c + d, where that is inside the lattice
+// (the rest arrives in boxes from the host).
+extern "C" __global__ void k_slide(K k, const float* src, float* dst, int dx, int dy, int dz)
+{
+    CELL_HEAD
+    const int sx = x + dx, sy = y + dy, sz = z + dz;
+    if (sx >= 0 && sx < k.nx && sy >= 0 && sy < k.ny && sz >= 0 && sz < k.nz) dst[IDX(x, y, z)] = src[IDX(sx, sy, sz)];
+}
+
+// A box of arriving cells, packed x fastest on the host, into field `field`: [x0, x0 + bx) x [y0, y0 + by) x [z0, z0 + bz).
+extern "C" __global__ void k_scatter(K k, int field, const float* packed, int x0, int y0, int z0, int bx, int by, int bz)
+{
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (i64)bx * by * bz) return;
+    const int x = x0 + (int)(i % bx), y = y0 + (int)((i / bx) % by), z = z0 + (int)(i / ((i64)bx * by));
+    k.now[field][IDX(x, y, z)] = packed[i];
+}
+
 // Conserved totals, pass 1: block b sums cells [b * chunk, (b + 1) * chunk), lane t every 256th from t; then the lanes
 // fold in a fixed tree. out[2b], out[2b+1]: the sum and the sum of magnitudes (each a compensated pair, rounded).
 // With rec not null the same pass is also the field's range contract (k_range's test and clamp, on the value the

@@ -105,6 +105,7 @@ struct Impl final : LawBackend {
     static constexpr long long kHalfChunk = 1LL << 20;
     std::map<int, std::unique_ptr<hip::HalfNetCard>> half_nets;
     Buffer half_rows, half_outs, half_faces;
+    Buffer spare, box_stage; // a moved window: the lattice a field slides into, the packed arriving box
     std::vector<float> net_host; // the law's nets as the kernels read them (HipCodegen.cpp: net_blob)
     std::vector<double> uploaded_params;
     bool params_valid = false;
@@ -473,9 +474,75 @@ struct Impl final : LawBackend {
         params_valid = true;
     }
 
+    // The window moved (edits.shift, edits.boxes; src/field/cuda/CudaSolver.cpp:895-920 keeps a ring and moves its
+    // origin): here every field's cells slide on the card into a spare lattice and back (the kernels keep plain
+    // row-major indexing), then only the arriving boxes go up, packed, and scatter into place. Captured step graphs are
+    // dropped. The stages' caches (a poisson warm start, the last set or source they compare with) stay where they are,
+    // as the CPU's caches in the law do: a moved set or source differs from the one held, and is solved again.
+    void slide(FieldSet& fields, const FieldEdits& edits)
+    {
+        const std::size_t cells = shape.cells();
+        if (edits.shift[0] != 0 || edits.shift[1] != 0 || edits.shift[2] != 0) {
+            if (spare.p == nullptr) {
+                spare.allocate(cells * 4);
+            }
+            for (std::size_t f = 0; f < law->fields.size(); ++f) {
+                void* src = now[f];
+                void* dst = spare.p;
+                int dx = edits.shift[0], dy = edits.shift[1], dz = edits.shift[2];
+                launch_cells("k_slide", 0.0, {&src, &dst, &dx, &dy, &dz});
+                check(fe_hip_memcpy_dtod_async(now[f], spare.p, cells * 4), "slide");
+            }
+        }
+        for (const CellBox& b0 : edits.boxes) {
+            const CellBox b{std::max(0, b0.x0), std::max(0, b0.y0), std::max(0, b0.z0), std::min(shape.nx, b0.x1),
+                std::min(shape.ny, b0.y1), std::min(shape.nz, b0.z1)};
+            int bx = b.x1 - b.x0, by = b.y1 - b.y0, bz = b.z1 - b.z0;
+            if (bx <= 0 || by <= 0 || bz <= 0) {
+                continue;
+            }
+            const std::size_t n = static_cast<std::size_t>(bx) * by * bz;
+            if (box_stage.bytes < n * 4) {
+                box_stage.allocate(n * 4);
+            }
+            std::vector<float> packed(n);
+            for (std::size_t f = 0; f < law->fields.size(); ++f) {
+                const std::vector<float>& v = fields.values(static_cast<int>(f));
+                std::size_t q = 0;
+                for (int z = b.z0; z < b.z1; ++z) {
+                    for (int y = b.y0; y < b.y1; ++y) {
+                        const std::size_t row = (static_cast<std::size_t>(z) * shape.ny + y) * shape.nx;
+                        for (int x = b.x0; x < b.x1; ++x) {
+                            const float c = v[row + static_cast<std::size_t>(x)];
+                            packed[q++] = c != 0.0f && std::fabs(c) < 1.17549435e-38f ? std::copysign(0.0f, c) : c;
+                        }
+                    }
+                }
+                check(fe_hip_memcpy_htod_async(box_stage.p, packed.data(), n * 4), "box upload");
+                // the host buffer is reused for the next field: wait for the copy
+                check(fe_hip_stream_sync(), "sync");
+                int field = static_cast<int>(f), x0 = b.x0, y0 = b.y0, z0 = b.z0;
+                void* src = box_stage.p;
+                std::vector<std::uint64_t> k = args(0.0);
+                void* argv[] = {k.data(), &field, &src, &x0, &y0, &z0, &bx, &by, &bz};
+                const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+                check(fe_hip_launch(fn("k_scatter"), blocks, 1, 1, 256, 1, 1, 0, argv), "launch k_scatter");
+                ++prof.launches;
+                prof.upload_bytes += n * 4;
+            }
+        }
+        for (std::size_t f = 0; f < law->fields.size(); ++f) {
+            touched(static_cast<int>(f));
+        }
+        drop_pairs();
+    }
+
     void upload(FieldSet& fields, FieldEdits& edits)
     {
         const std::size_t cells = shape.cells();
+        if (!edits.all && edits.shifted()) {
+            slide(fields, edits);
+        }
         for (std::size_t f = 0; f < law->fields.size(); ++f) {
             bool dirty = edits.all;
             if (!dirty && f < edits.planes.size()) {

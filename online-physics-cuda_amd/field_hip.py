@@ -82,6 +82,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_set_param": (c_int, [c_void_p, ctypes.c_char_p, c_double]),
         "fe_solver_step": (c_int, [c_void_p, c_double]),
         "fe_solver_run": (c_int, [c_void_p, c_double, c_int]),
+        "fe_solver_shift_window": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
         "fe_solver_fill_field": (c_int, [c_void_p, c_int, ctypes.c_float]),
         "fe_solver_set_net_weights": (c_int, [c_void_p, ctypes.c_char_p, c_void_p, ctypes.c_size_t]),
         "fe_solver_total_accumulate": (c_int, [c_void_p, c_int, c_void_p]),
@@ -459,6 +460,21 @@ class Solver:
 
         w = np.ascontiguousarray(weights, np.float32)
         self._check(lib().fe_solver_set_net_weights(self.ptr, net.encode(), w.ctypes.data, w.size), "set_net_weights")
+
+    def shift_window(self, dx: int, dy: int, dz: int, boxes, arriving) -> None:
+        """The window moved by (dx, dy, dz): every field's cells slide (what is now at c was at c + d) and the arriving
+        boxes ((x0, y0, z0, x1, y1, z1), half-open) take `arriving`, one array of each field's box cells (x fastest) in
+        field order then box order. The card slides its copy and receives only the boxes."""
+        import numpy as np
+
+        b = np.ascontiguousarray(np.asarray(boxes, np.int32).reshape(-1, 6))
+        a = np.ascontiguousarray(np.asarray(arriving, np.float32).ravel())
+        cells = int(sum((q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]) for q in b))
+        nf = len(self.law.field_names())
+        if a.size != cells * nf:
+            raise FieldHipError(f"shift_window: {a.size} arriving values for {cells} box cells of {nf} fields")
+        self._check(lib().fe_solver_shift_window(self.ptr, dx, dy, dz, b.ctypes.data if len(b) else None, len(b),
+                                                 a.ctypes.data if a.size else None), "shift_window")
 
     def fill_field(self, field, value: float) -> None:
         self._check(lib().fe_solver_fill_field(self.ptr, self._field(field), float(value)), "fill_field")
