@@ -99,7 +99,20 @@ struct Impl final : LawBackend {
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
     std::vector<void*> now, nxt;
-    Buffer P, U, A, S, L, C, N, W, records, firsts, totals;
+    Buffer P, U, A, S, L, C, N, W;
+    // The contract records, one allocation read back in one copy: range counts (two u32 a field), the first bad
+    // cell (a u64 a field), then the conserved totals (kTotals doubles); `judged` the pinned host copy.
+    struct View {
+        void* p = nullptr;
+        template <typename T>
+        [[nodiscard]] T* as() const
+        {
+            return static_cast<T*>(p);
+        }
+    };
+    Buffer contract;
+    View records, firsts, totals;
+    void* judged = nullptr;
     // `half` nets on the matrix cores: their packed weights, and the row, output and face buffers their stages share
     // (a chunk of kHalfChunk cells at a time).
     static constexpr long long kHalfChunk = 1LL << 20;
@@ -114,6 +127,9 @@ struct Impl final : LawBackend {
     ~Impl() override
     {
         drop_pairs();
+        if (judged != nullptr) {
+            fe_hip_host_free(judged);
+        }
         for (fe_hip_event e : events) {
             fe_hip_event_destroy(e);
         }
@@ -409,9 +425,16 @@ struct Impl final : LawBackend {
         half_rows.allocate(row_bytes);
         half_outs.allocate(out_bytes);
         half_faces.allocate(faces ? cells * 3 * 4 : 0);
-        records.allocate(std::max<std::size_t>(1, nf) * 2 * 4);
-        firsts.allocate(std::max<std::size_t>(1, nf) * 8);
-        totals.allocate(static_cast<std::size_t>(kTotals) * 8);
+        const std::size_t rec_bytes = std::max<std::size_t>(1, nf) * 8, first_bytes = std::max<std::size_t>(1, nf) * 8;
+        contract.allocate(rec_bytes + first_bytes + static_cast<std::size_t>(kTotals) * 8);
+        records.p = contract.p;
+        firsts.p = static_cast<char*>(contract.p) + rec_bytes;
+        totals.p = static_cast<char*>(contract.p) + rec_bytes + first_bytes;
+        if (judged != nullptr) {
+            fe_hip_host_free(judged);
+            judged = nullptr;
+        }
+        check(fe_hip_host_alloc(&judged, contract.bytes), "pinned allocation");
         stale.assign(nf, 0);
     }
 
@@ -928,14 +951,22 @@ struct Impl final : LawBackend {
         if (judges.empty()) {
             return;
         }
-        const std::size_t nf = law->fields.size();
-        std::vector<unsigned> rec(std::max<std::size_t>(1, nf) * 2);
-        std::vector<unsigned long long> first(std::max<std::size_t>(1, nf));
-        std::vector<double> tot(kTotals);
-        check(fe_hip_memcpy_dtoh_async(rec.data(), records.p, rec.size() * 4), "records readback");
-        check(fe_hip_memcpy_dtoh_async(first.data(), firsts.p, first.size() * 8), "records readback");
-        check(fe_hip_memcpy_dtoh_async(tot.data(), totals.p, tot.size() * 8), "totals readback");
+        // One copy into pinned memory: the records and the totals slots these judges use.
+        int used = 0;
+        for (const Judge& j : judges) {
+            int end = j.slot;
+            for (const std::vector<int>& g : j.groups) {
+                end += 4 * static_cast<int>(g.size());
+            }
+            used = std::max(used, end);
+        }
+        const std::size_t head = static_cast<std::size_t>(static_cast<char*>(totals.p) - static_cast<char*>(contract.p));
+        check(fe_hip_memcpy_dtoh_async(judged, contract.p, head + static_cast<std::size_t>(used) * 8), "records readback");
         check(fe_hip_stream_sync(), "sync");
+        const unsigned* rec = static_cast<const unsigned*>(judged);
+        const unsigned long long* first =
+            reinterpret_cast<const unsigned long long*>(static_cast<const char*>(judged) + (static_cast<char*>(firsts.p) - static_cast<char*>(contract.p)));
+        const double* tot = reinterpret_cast<const double*>(static_cast<const char*>(judged) + head);
         for (const Judge& j : judges) {
             int slot = j.slot;
             for (std::size_t g = 0; g < j.groups.size(); ++g) {
