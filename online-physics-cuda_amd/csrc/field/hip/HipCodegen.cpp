@@ -887,7 +887,17 @@ private:
         case Op::Add: return "(" + a(0) + " + " + a(1) + ")";
         case Op::Sub: return "(" + a(0) + " - " + a(1) + ")";
         case Op::Mul: return "([](double p, double q) { return p == 0.0 || q == 0.0 ? 0.0 : p * q; })(" + a(0) + ", " + a(1) + ")";
-        case Op::Div: return "([](double p, double q) { return q == 0.0 ? 0.0 : p / q; })(" + a(0) + ", " + a(1) + ")";
+        case Op::Div: {
+            // A division by h or h * h (the lattice spacing): the product with the reciprocal where that is a power of
+            // two (the quotient's bits, as div_h and div_hh give the stencils), no division on the card.
+            const ir::Inst& q = at(in.args[1]);
+            const bool by_h = q.op == Op::Cell;
+            const bool by_hh = q.op == Op::Mul && at(q.args[0]).op == Op::Cell && at(q.args[1]).op == Op::Cell;
+            if (by_h || by_hh) {
+                return std::string("(k.h == 0.0 ? 0.0 : ") + (by_h ? "div_h" : "div_hh") + "<M>(k, " + a(0) + "))";
+            }
+            return "([](double p, double q) { return q == 0.0 ? 0.0 : p / q; })(" + a(0) + ", " + a(1) + ")";
+        }
         case Op::Pow: return "pow(" + a(0) + ", " + a(1) + ")";
         case Op::Min: return "fe_min(" + a(0) + ", " + a(1) + ")";
         case Op::Max: return "fe_max(" + a(0) + ", " + a(1) + ")";
