@@ -119,15 +119,27 @@ struct Lattice {
     float h;
 };
 
+// (lattice, x, y) of cell index i of count lattices of nx x ny, in 32-bit arithmetic (a 64-bit division is a long
+// software sequence on the card; the host keeps 2 * count * nx * ny below 2^31).
+__device__ __forceinline__ void cell_of(long long i, int nx, int ny, long long& lat, int& x, int& y)
+{
+    const unsigned q = (unsigned)i, cells = (unsigned)nx * (unsigned)ny;
+    const unsigned l = q / cells, c = q - l * cells;
+    const unsigned yy = c / (unsigned)nx;
+    lat = (long long)l;
+    y = (int)yy;
+    x = (int)(c - yy * (unsigned)nx);
+}
+
 // Input k of evaluation row `row` (a face evaluation: (lattice, cell, direction) * 2 + side) at feature in (< 18).
 __device__ __forceinline__ float patch_value(const float* __restrict__ u, const float* __restrict__ v, const Lattice& L, long long fe,
     int side, int in)
 {
     const long long cells = (long long)L.nx * L.ny;
     const int dir = (int)(fe & 1);
-    const long long cell = (fe >> 1) % cells;
-    const long long lat = (fe >> 1) / cells;
-    int x = (int)(cell % L.nx), y = (int)(cell / L.nx);
+    long long lat;
+    int x, y;
+    cell_of(fe >> 1, L.nx, L.ny, lat, x, y);
     const int field = in / 9, a = (in % 9) / 3 - 1, b = in % 3 - 1;
     // the anchor: the cell itself (side 0) or the one across the face, read mirrored (a -> -a)
     int da = a, db = b;
@@ -210,7 +222,6 @@ __global__ __launch_bounds__(256) void k_faces(const bf16x8* __restrict__ frags,
     const int n = l & 15, g = l >> 4;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long waves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const long long cells = (long long)L.nx * L.ny;
     for (long long t = wave; t * 8 < faces; t += waves) {
         Acts s;
         float o0, o1;
@@ -219,8 +230,9 @@ __global__ __launch_bounds__(256) void k_faces(const bf16x8* __restrict__ frags,
         const long long fe = t * 8 + (n >> 1);
         if (g == 0 && (n & 1) == 0 && fe < faces) {
             const int dir = (int)(fe & 1);
-            const long long cell = (fe >> 1) % cells;
-            const int x = (int)(cell % L.nx), y = (int)(cell / L.nx);
+            long long lat;
+            int x, y;
+            cell_of(fe >> 1, L.nx, L.ny, lat, x, y);
             const bool open = dir == 0 ? x + 1 < L.nx : y + 1 < L.ny;
             F[2 * fe] = open ? 0.5f * scale * (o0 - m0) : 0.0f;
             F[2 * fe + 1] = open ? 0.5f * scale * (o1 - m1) : 0.0f;
@@ -279,8 +291,9 @@ __global__ void k_step(const float* __restrict__ u, const float* __restrict__ v,
     if (i >= cells * L.count) {
         return;
     }
-    const long long lat = i / cells, c = i % cells;
-    const int x = (int)(c % L.nx), y = (int)(c / L.nx);
+    long long lat;
+    int x, y;
+    cell_of(i, L.nx, L.ny, lat, x, y);
     float ru, rv;
     plain(u, v, L, R, lat, x, y, ru, rv);
     if (learned) {
@@ -314,8 +327,9 @@ __global__ void k_fallback(const float* __restrict__ u, const float* __restrict_
     if (i >= cells * L.count) {
         return;
     }
-    const long long lat = i / cells, c = i % cells;
-    const int x = (int)(c % L.nx), y = (int)(c / L.nx);
+    long long lat;
+    int x, y;
+    cell_of(i, L.nx, L.ny, lat, x, y);
     float ru, rv;
     plain(u, v, L, R, lat, x, y, ru, rv);
     u2[i] = u[i] + dt * ru;
@@ -342,8 +356,9 @@ __global__ void k_residual(const float* __restrict__ u0, const float* __restrict
     if (i >= n) {
         return;
     }
-    const long long lat = i / cells, c = i % cells;
-    const int x = (int)(c % L.nx), y = (int)(c / L.nx);
+    long long lat;
+    int x, y;
+    cell_of(i, L.nx, L.ny, lat, x, y);
     float ru, rv, cu, cv;
     plain(u0, v0, L, R, lat, x, y, ru, rv);
     correction(F, L, lat, x, y, cu, cv);
@@ -358,11 +373,11 @@ __global__ void k_residual(const float* __restrict__ u0, const float* __restrict
 // Training, 2: dL/dg of every evaluation row from E (through the faces' divergence and the antisymmetric pair).
 __device__ __forceinline__ void row_grad(const float* __restrict__ E, const Lattice& L, float scale, long long fe, int side, float& d0, float& d1)
 {
-    const long long cells = (long long)L.nx * L.ny;
     const int dir = (int)(fe & 1);
     const long long c = fe >> 1;
-    const long long cell = c % cells;
-    const int x = (int)(cell % L.nx), y = (int)(cell / L.nx);
+    long long lat;
+    int x, y;
+    cell_of(c, L.nx, L.ny, lat, x, y);
     const bool open = dir == 0 ? x + 1 < L.nx : y + 1 < L.ny;
     if (!open) {
         d0 = d1 = 0.0f;
@@ -612,8 +627,9 @@ __global__ void k_coarsen(const float* __restrict__ fine, int fnx, int fny, int 
     if (i >= cells * count) {
         return;
     }
-    const long long lat = i / cells, c = i % cells;
-    const int x = (int)(c % cnx), y = (int)(c / cnx);
+    long long lat;
+    int x, y;
+    cell_of(i, cnx, cny, lat, x, y);
     const float* src = fine + lat * (long long)fnx * fny;
     double s = 0.0;
     for (int b = 0; b < f; ++b) {
@@ -684,7 +700,13 @@ int fail(fe_lnet* n, const std::string& what)
     return -1;
 }
 
-Lattice lattice(int nx, int ny, int count, double h) { return Lattice{nx, ny, count, static_cast<float>(h)}; }
+Lattice lattice(int nx, int ny, int count, double h)
+{
+    if (2.0 * nx * ny * count >= 2147483648.0) {
+        throw std::runtime_error("batch too large: 2 * count * nx * ny must stay below 2^31 (32-bit cell indexing)");
+    }
+    return Lattice{nx, ny, count, static_cast<float>(h)};
+}
 
 } // namespace
 
