@@ -1406,7 +1406,7 @@ private:
             const int tw = vx == 4 ? 264 : 64 + 2 * R;
             return NW * RS * (ty + 2 * R) * tw * 4;
         };
-        for (const Shape sh : {Shape{1, R == 1 ? 6 : 4}, Shape{4, 2}, Shape{4, 6}}) {
+        for (const Shape sh : {Shape{1, R == 1 ? 6 : 4}, Shape{4, 2}, Shape{4, 4}, Shape{4, 6}}) {
             if (NW > 0 && lds_bytes(sh.vx, sh.ty) <= kLdsBudget) {
                 shapes.push_back(sh);
             }

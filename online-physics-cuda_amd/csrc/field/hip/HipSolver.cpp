@@ -254,9 +254,10 @@ struct Impl final : LawBackend {
         return 8.0 * static_cast<double>(shape.nx) * shape.ny * shape.nz * layout.fields <= 256.0 * 1024 * 1024;
     }
 
-    // The window gather for this lattice: four cells a lane when rows are whole float4s, with the taller tile when
-    // that still leaves at least 2048 tiles (eight per CU) to spread; FIELD_HIP_TILE (e.g. "4x2", "1x6") picks one
-    // by its cells a lane and rows. None: the direct gather.
+    // The window gather for this lattice: four cells a lane when rows are whole float4s, with the tallest tile that
+    // still leaves at least 512 tiles (two per CU) to spread (measured on MI355X, conduction at z-chunks of 32: 256^3
+    // 29.1 us with 256x4 tiles against 31.2 with 256x2 and 39.9 with 256x6; 512^3 219 us with 256x6 against 251 and
+    // 253); FIELD_HIP_TILE (e.g. "4x2", "1x6") picks one by its cells a lane and rows. None: the direct gather.
     const hip::WindowKernel* window_for(const hip::ProgramCode& code) const
     {
         if (!code.window || code.windows.empty() || force_direct) {
@@ -279,7 +280,7 @@ struct Impl final : LawBackend {
                 continue;
             }
             const long long tiles = 1LL * ((shape.nx + 64 * w.vx - 1) / (64 * w.vx)) * ((shape.ny + w.ty - 1) / w.ty) * tz;
-            if (best == nullptr || w.vx > best->vx || (w.vx == best->vx && w.ty > best->ty && tiles >= 2048)) {
+            if (best == nullptr || w.vx > best->vx || (w.vx == best->vx && w.ty > best->ty && tiles >= 512)) {
                 best = &w;
             }
         }

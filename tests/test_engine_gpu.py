@@ -160,7 +160,8 @@ def test_profile_reports_card_time(device):
 # Every gather the generator emits for a law, picked through the launch-shape overrides: the direct kernel (a thread a
 # cell), the window kernel with a cell a lane (64 x 6 tiles) and with four cells a lane (256 x 2 and 256 x 6 tiles).
 GATHERS = {"direct": {"FIELD_HIP_DIRECT": "1"}, "w1x6": {"FIELD_HIP_TILE": "1x6"}, "w4x2": {"FIELD_HIP_TILE": "4x2"},
-           "w4x6": {"FIELD_HIP_TILE": "4x6"}, "w4x2z5": {"FIELD_HIP_TILE": "4x2", "FIELD_HIP_ZCHUNK": "5"},
+           "w4x4": {"FIELD_HIP_TILE": "4x4"}, "w4x6": {"FIELD_HIP_TILE": "4x6"},
+           "w4x2z5": {"FIELD_HIP_TILE": "4x2", "FIELD_HIP_ZCHUNK": "5"},
            # launch order and streaming stores (what lattices past the last-level cache get)
            "w4x6s": {"FIELD_HIP_TILE": "4x6", "FIELD_HIP_REMAP": "0", "FIELD_HIP_STREAM": "1"}}
 
