@@ -52,3 +52,16 @@ def test_no_device_here_is_reported_in_words():
     if info.ok:
         pytest.skip("a device is present")
     assert info.why and b"device" in info.why
+
+
+def test_field_backend_cpu_keeps_the_law_off_the_card(monkeypatch):
+    """FIELD_BACKEND=cpu is refused by name before any device is touched (src/field/Solver.cpp:173-200 reads it the
+    same way); this build has no CPU integrator to fall back on."""
+    import field_hip
+
+    monkeypatch.setenv("FIELD_BACKEND", "cpu")
+    law = field_hip.Law("field T init 0\nterm t\n  d T = diffuse(0.1, T)\n")
+    why = ctypes.create_string_buffer(4096)
+    ptr = field_hip.lib().fe_solver_create(law.ptr, 4, 4, 4, 1.0, why, len(why))
+    assert not ptr
+    assert b"FIELD_BACKEND=cpu" in why.value
