@@ -9,6 +9,7 @@
 #include "field/Backend.hpp"
 #include "field/Law.hpp"
 
+#include <cstdint>
 #include <memory>
 #include <optional>
 #include <string>
@@ -37,6 +38,13 @@ public:
 
     bool use_gpu(std::string& why);
     [[nodiscard]] bool on_gpu() const { return gpu_ != nullptr; }
+    [[nodiscard]] std::string gpu_name() const { return gpu_ ? gpu_->adapter() : std::string{}; }
+    [[nodiscard]] const char* backend_api() const { return gpu_ ? gpu_->api() : "CPU"; }
+    [[nodiscard]] int field(std::string_view name) const { return law_.field_id(name); }
+    [[nodiscard]] bool learned() const { return learned_; }
+    // A trainer's note that it rewrote net weights (the backend also notices by hashing them every step).
+    void nets_changed() { ++nets_version_; }
+    [[nodiscard]] std::uint64_t nets_version() const { return nets_version_; }
     [[nodiscard]] LawBackend* backend() const { return gpu_.get(); }
     // A field's cells were written on the card by the caller: the host copy is fetched before it is next read.
     void mark_device_written(int field) { stale_[static_cast<std::size_t>(field)] = 1; }
@@ -68,6 +76,7 @@ private:
     bool lazy_ = false;
     bool learned_ = false;
     bool respect_max_dt_ = true;
+    std::uint64_t nets_version_ = 0;
     std::vector<Violation> violations_;
     std::vector<int> substeps_;
     std::vector<char> changed_;
