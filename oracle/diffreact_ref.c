@@ -1,5 +1,5 @@
-/* ORACLE (test infrastructure only): the plain 2-D diffusion-reaction law in FP64 on the CPU, the judge of the coarse
- * lattices' error. Only tests/ and bench.py call it.
+/* ORACLE (test infrastructure only): the plain 2-D diffusion-reaction law in FP64 on the CPU, the run the coarse
+ * lattices' error is measured against. Only tests/ and bench.py call it.
  *
  *   du/dt = Du lap u + u - u^3 - k - v,   dv/dt = Dv lap v + u - v
  *

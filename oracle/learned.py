@@ -3,7 +3,7 @@
 Restates what online-physics-cuda_amd/csrc/kernels/learned_flux.hip computes, with bf16 rounding (round to nearest
 even) at the same points as the card (inputs, weights, hidden activations and their gradients as they enter a matrix
 product), so the card and this differ only by fp32 accumulation. Also: the plain law (5-point Laplacian, zero-flux
-edges) stepped in FP64 (the judge of the learned coarse lattice's error), and block averages.
+edges) stepped in FP64 (what the learned coarse lattice's error is measured against), and block averages.
 
 Only tests/, the smoke check and bench.py call this.
 """
