@@ -687,7 +687,7 @@ struct fe_lnet {
     std::string error;
     hipStream_t stream() const { return static_cast<hipStream_t>(fe_hip_stream()); }
     static constexpr int kFaceBlocks = 1024; // face kernels: waves in flight (a grid-stride loop over 16-row tiles)
-    static constexpr int kTrainBlocks = 512;
+    static constexpr int kTrainBlocks = 1024; // one wave a SIMD: the training pass holds 512 registers a lane
 };
 
 namespace {
