@@ -143,18 +143,31 @@ extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 ch
     i64 end = begin + chunk; if (end > k.cells) end = k.cells;
     Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
     const float* v = k.now[field];
-    for (i64 i = begin + t; i < end; i += 256) {
-        const double d = (double)v[i]; add(s, d); add(m, fabs(d));
-        if (rec != nullptr) {
-            const bool fin = finite_d(d);
-            bool bad = !fin;
-            if (fin && (d < lo || d > hi)) {
-                if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
-                else bad = true;
-            }
-            if (bad) {
-                atomicAdd(&rec[fin ? 0 : 1], 1u);
-                atomicMin(first, (u64)i);
+    // Eight cells' loads (clamped into the range, unconditional) ahead of their compensated sums, which keep their
+    // order: the pass waits on memory once in eight cells, not every cell.
+    for (i64 i0 = begin + t; i0 < end; i0 += 8 * 256) {
+        float vv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const i64 i = i0 + q * 256;
+            vv[q] = v[i < end ? i : end - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const i64 i = i0 + q * 256;
+            if (i >= end) break;
+            const double d = (double)vv[q]; add(s, d); add(m, fabs(d));
+            if (rec != nullptr) {
+                const bool fin = finite_d(d);
+                bool bad = !fin;
+                if (fin && (d < lo || d > hi)) {
+                    if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
+                    else bad = true;
+                }
+                if (bad) {
+                    atomicAdd(&rec[fin ? 0 : 1], 1u);
+                    atomicMin(first, (u64)i);
+                }
             }
         }
     }
