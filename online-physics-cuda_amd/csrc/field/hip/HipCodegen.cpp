@@ -24,6 +24,18 @@ typedef unsigned long long u64;
 #define AT(slot, x, y, z) k.A[(i64)(slot) * k.cells + IDX(x, y, z)]
 #define CELL_HEAD const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x); const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y); \
     const int z = (int)blockIdx.z; if (x >= k.nx || y >= k.ny) return;
+// A cell's coordinates from its index: 32-bit divisions while the lattice has fewer than 2^31 cells (a 64-bit division
+// is a long software sequence on the card).
+__device__ __forceinline__ void cell_xyz(const K& k, i64 i, int& x, int& y, int& z)
+{
+    if (k.cells < 2147483648LL) {
+        const unsigned u = (unsigned)i, nx = (unsigned)k.nx, ny = (unsigned)k.ny;
+        const unsigned r = u / nx;
+        x = (int)(u - r * nx); y = (int)(r % ny); z = (int)(r / ny);
+    } else {
+        x = (int)(i % k.nx); y = (int)((i / k.nx) % k.ny); z = (int)(i / k.nxy);
+    }
+}
 // Clamps written as max(0, min(v, n - 1)): one v_med3_i32.
 __device__ __forceinline__ int ex(const K& k, int v) { return max(0, min(v, k.nx - 1)); }
 __device__ __forceinline__ int ey(const K& k, int v) { return max(0, min(v, k.ny - 1)); }
@@ -948,7 +960,7 @@ private:
             }
             out_ << "    Sum s = {0.0, 0.0}; double a = first; double b = first;\n"
                  << "    for (i64 i = begin + t; i < end; i += 256) {\n"
-                 << "        const int x = (int)(i % k.nx); const int y = (int)((i / k.nx) % k.ny); const int z = (int)(i / k.nxy);\n";
+                 << "        int x, y, z; cell_xyz(k, i, x, y, z);\n";
             if (scalar(arg)) {
                 out_ << "        const double q = s" << arg << "(k, m);\n";
             } else {
@@ -1015,8 +1027,7 @@ private:
         const int nin = net.inputs();
         const int rpc = s.rows_per_cell, pad = s.in_pad, outs = s.outs;
         const std::string head = "    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (t >= count) return;\n"
-                                 "    const i64 i = c0 + t; const int x = (int)(i % k.nx); const int y = (int)((i / k.nx) % k.ny);"
-                                 " const int z = (int)(i / k.nxy);\n    const Direct m(k);\n";
+                                 "    const i64 i = c0 + t; int x, y, z; cell_xyz(k, i, x, y, z);\n    const Direct m(k);\n";
         std::ostringstream p;
         p << "extern \"C\" __global__ void " << s.kernel << "_pack(K k, _Float16* rows, i64 c0, i64 count) {\n" << head
           << "    const float* P = k.N + " << off << ";\n"
