@@ -237,6 +237,41 @@ def test_conduction_with_infinities_and_nan_follows_the_oracle(device, monkeypat
     assert exact(got[~nan], expect[~nan])
 
 
+IN_PLACE = """field a init 0 range -1e30 1e30
+field b init 0 range -1e30 1e30
+term t
+  d b = diffuse(0.1, a) - 0.5 * b + shift(a, 1, 0, -1)
+"""
+
+
+@pytest.mark.parametrize("gather", sorted(g for g in GATHERS if g != "direct"))
+def test_flagged_planes_of_an_in_place_term_are_redone_exactly(device, monkeypatch, gather):
+    """A term writing b in place (b read only at its own cell) and a read around it through the window: infinities
+    and NaN in a flag lanes, which the window takes again after the tile in the exact forms while its in-place
+    stores wait. The direct gather (exact forms everywhere) gives the bits."""
+    import field_hip
+
+    shape = (40, 9, 11)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(17)
+    a, b = rng.random(n, dtype=np.float32), rng.random(n, dtype=np.float32)
+    for i, v in ((4 * 360 + 4 * 40 + 7, np.inf), (0, -np.inf), (7 * 360 + 8 * 40 + 39, np.nan), (10 * 360 + 5, np.inf)):
+        a[i] = v
+    runs = {}
+    for g in ("direct", gather):
+        use_gather(monkeypatch, g)
+        solver = field_hip.Solver(field_hip.Law(IN_PLACE), shape, 0.25)
+        solver.set_field("a", a)
+        solver.set_field("b", b)
+        for _ in range(3):
+            solver.step(0.01)
+        runs[g] = solver.get_field("b")
+    want, got = runs["direct"], runs[gather]
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert exact(got[~nan], want[~nan])
+
+
 @pytest.mark.parametrize("name,net", [("net_flux_relu", "g"), ("net_ldiffuse_half", "w")])
 def test_new_net_weights_are_taken_up(device, name, net):
     # Weights written after the law was built (what a trainer does) reach the card at the next step: a solver whose
