@@ -65,17 +65,26 @@ __device__ __host__ __forceinline__ unsigned short to_bf16_bits(float f)
 __device__ __forceinline__ __bf16 bf(float f) { return __builtin_bit_cast(__bf16, to_bf16_bits(f)); }
 __device__ __forceinline__ float unbf(__bf16 b) { return __builtin_bit_cast(float, static_cast<unsigned int>(__builtin_bit_cast(unsigned short, b)) << 16); }
 
+// tanh u = 1 - 2 / (1 + e^(2u)) through the hardware exponential and reciprocal (a few ulp; the activations it feeds
+// are rounded to bf16, 8 bits): the library tanhf is a long polynomial sequence, and the net's forward and backward
+// passes were bound by it, not by the matrix cores. Saturates to +-1 (e^(2u) overflowing to inf, or to 0).
+__device__ __forceinline__ float fast_tanh(float u)
+{
+    const float e = __expf(2.0f * u);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
 __device__ __forceinline__ float gelu(float z)
 {
     const float k0 = 0.7978845608028654f; // sqrt(2 / pi)
-    const float t = tanhf(k0 * (z + 0.044715f * z * z * z));
+    const float t = fast_tanh(k0 * (z + 0.044715f * z * z * z));
     return 0.5f * z * (1.0f + t);
 }
 
 __device__ __forceinline__ float gelu_grad(float z)
 {
     const float k0 = 0.7978845608028654f;
-    const float t = tanhf(k0 * (z + 0.044715f * z * z * z));
+    const float t = fast_tanh(k0 * (z + 0.044715f * z * z * z));
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k0 * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
