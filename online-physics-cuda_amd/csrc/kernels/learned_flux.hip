@@ -62,7 +62,8 @@ __device__ __host__ __forceinline__ unsigned short to_bf16_bits(float f)
     return static_cast<unsigned short>(u >> 16);
 }
 
-__device__ __forceinline__ __bf16 bf(float f) { return __builtin_bit_cast(__bf16, to_bf16_bits(f)); }
+// Round to nearest even: the hardware conversion (v_cvt_pk_bf16_f32 on gfx950), what to_bf16_bits computes by bits.
+__device__ __forceinline__ __bf16 bf(float f) { return static_cast<__bf16>(f); }
 __device__ __forceinline__ float unbf(__bf16 b) { return __builtin_bit_cast(float, static_cast<unsigned int>(__builtin_bit_cast(unsigned short, b)) << 16); }
 
 // tanh u = 1 - 2 / (1 + e^(2u)) through the hardware exponential and reciprocal (a few ulp; the activations it feeds
