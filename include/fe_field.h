@@ -48,6 +48,9 @@ int fe_law_term_count(const fe_law* law);
  * (ir::describe, src/field/ir/LawIR.hpp:544); free with fe_string_free. */
 char* fe_law_ir_json(const fe_law* law);
 char* fe_law_describe(const fe_law* law);
+/* The law written back as law text that compiles to the same IR (shared subtrees as lets). No counterpart in the
+ * reference (its lawc prints IR, not law text); the adjoint generator emits its laws through it. */
+char* fe_law_text(const fe_law* law);
 void fe_string_free(char* s);
 /* The generated gfx950 translation unit of a law (CudaSolver::source, src/field/cuda/CudaSolver.hpp:30; lawc --cuda,
  * tools/lawc/main.cpp:221-224); NULL with the reason in `why` when the law uses an op the backend does not run. */

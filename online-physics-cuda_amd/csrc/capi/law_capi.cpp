@@ -2,11 +2,13 @@
 #include "fe_field.h"
 
 #include "field/Law.hpp"
+#include "field/LawText.hpp"
 #include "field/capi_internal.hpp"
 #include "field/ir/LawIR.hpp"
 
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -99,6 +101,18 @@ char* fe_law_ir_json(const fe_law* law)
 char* fe_law_describe(const fe_law* law)
 {
     return law == nullptr ? nullptr : dup(ir::describe(law->module, law->law));
+}
+
+char* fe_law_text(const fe_law* law)
+{
+    if (law == nullptr) {
+        return nullptr;
+    }
+    try {
+        return dup(law_text(law->law));
+    } catch (const std::exception&) {
+        return nullptr; // a law holding something law text cannot state (a creature's state, a non-finite number)
+    }
 }
 
 void fe_string_free(char* s)

@@ -16,13 +16,6 @@
 
 using namespace fieldengine::field;
 
-namespace fieldengine::field {
-void* hip_backend_field_pointer(LawBackend& backend, int field);
-void hip_backend_device_written(LawBackend& backend, int field);
-void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag);
-void hip_backend_total_accumulate(LawBackend& backend, int field, double* acc);
-}
-
 struct fe_solver {
     std::unique_ptr<Solver> solver;
     std::string error;
@@ -235,6 +228,7 @@ char* fe_solver_profile_json(fe_solver* s)
         return nullptr;
     }
     const StepProfile& p = s->solver->profile();
+    const HipBackend* backend = s->solver->backend();
     std::ostringstream out;
     out.precision(9);
     out << "{\"backend\":";
@@ -247,7 +241,8 @@ char* fe_solver_profile_json(fe_solver* s)
         out << (t ? "," : "") << "{\"name\":";
         json_string(out, tp.name);
         out << ",\"kernel\":";
-        json_string(out, tp.kernel);
+        const std::vector<std::string>* kernels = backend != nullptr ? &backend->gather_kernels() : nullptr;
+        json_string(out, kernels != nullptr && t < kernels->size() ? (*kernels)[t] : std::string());
         out << ",\"ms\":" << tp.ms << ",\"launches\":" << tp.launches << ",\"substeps\":" << tp.substeps
             << ",\"learned\":" << (tp.learned ? "true" : "false") << ",\"fell_back\":" << (tp.fell_back ? "true" : "false") << '}';
     }
@@ -278,7 +273,7 @@ void* fe_solver_field_device(fe_solver* s, int field)
     if (!valid_field(s, field) || s->solver->backend() == nullptr) {
         return nullptr;
     }
-    return hip_backend_field_pointer(*s->solver->backend(), field);
+    return s->solver->backend()->device_field(field);
 }
 
 int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
@@ -286,14 +281,14 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
     if (!valid_field(s, field) || src == nullptr || s->solver->backend() == nullptr) {
         return -1;
     }
-    void* dst = hip_backend_field_pointer(*s->solver->backend(), field);
+    void* dst = s->solver->backend()->device_field(field);
     const std::size_t bytes = s->solver->fields_const_shape_cells() * sizeof(float);
     const int r = fe_hip_memcpy_dtod_async(dst, src, bytes);
     if (r != 0) {
         s->error = std::string("device copy: ") + fe_hip_error_string(r);
         return r;
     }
-    hip_backend_device_written(*s->solver->backend(), field);
+    s->solver->backend()->device_written(field);
     s->solver->mark_device_written(field);
     return 0;
 }
@@ -371,7 +366,7 @@ int fe_solver_fill_field(fe_solver* s, int field, float value)
     if (!valid_field(s, field) || s->solver->backend() == nullptr) {
         return -1;
     }
-    void* dst = hip_backend_field_pointer(*s->solver->backend(), field);
+    void* dst = s->solver->backend()->device_field(field);
     std::uint32_t bits = 0;
     std::memcpy(&bits, &value, 4);
     const int r = fe_hip_memset_d32_async(dst, bits, s->solver->fields_const_shape_cells());
@@ -379,7 +374,7 @@ int fe_solver_fill_field(fe_solver* s, int field, float value)
         s->error = std::string("device fill: ") + fe_hip_error_string(r);
         return r;
     }
-    hip_backend_device_written(*s->solver->backend(), field);
+    s->solver->backend()->device_written(field);
     s->solver->mark_device_written(field);
     return 0;
 }
@@ -389,7 +384,7 @@ int fe_solver_total_accumulate(fe_solver* s, int field, double* device_acc)
     if (!valid_field(s, field) || device_acc == nullptr || s->solver->backend() == nullptr) {
         return -1;
     }
-    return guarded(s, [&] { hip_backend_total_accumulate(*s->solver->backend(), field, device_acc); });
+    return guarded(s, [&] { s->solver->backend()->field_total_accumulate(field, device_acc); });
 }
 
 int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude)
@@ -399,7 +394,7 @@ int fe_solver_total(fe_solver* s, int field, double* sum, double* magnitude)
     }
     return guarded(s, [&] {
         double m = 0.0;
-        hip_backend_total(*s->solver->backend(), field, *sum, m);
+        s->solver->backend()->field_total(field, *sum, m);
         if (magnitude != nullptr) {
             *magnitude = m;
         }

@@ -8,6 +8,7 @@
 
 #include "field/Backend.hpp"
 #include "field/Law.hpp"
+#include "field/hip/HipSolver.hpp"
 
 #include <cstdint>
 #include <memory>
@@ -45,7 +46,7 @@ public:
     // A trainer's note that it rewrote net weights (the backend also notices by hashing them every step).
     void nets_changed() { ++nets_version_; }
     [[nodiscard]] std::uint64_t nets_version() const { return nets_version_; }
-    [[nodiscard]] LawBackend* backend() const { return gpu_.get(); }
+    [[nodiscard]] HipBackend* backend() const { return gpu_.get(); }
     // A field's cells were written on the card by the caller: the host copy is fetched before it is next read.
     void mark_device_written(int field) { stale_[static_cast<std::size_t>(field)] = 1; }
     [[nodiscard]] std::size_t fields_const_shape_cells() const { return fields_.shape().cells(); }
@@ -80,7 +81,7 @@ private:
     std::vector<Violation> violations_;
     std::vector<int> substeps_;
     std::vector<char> changed_;
-    std::shared_ptr<LawBackend> gpu_;
+    std::shared_ptr<HipBackend> gpu_;
     StepProfile empty_profile_;
 };
 

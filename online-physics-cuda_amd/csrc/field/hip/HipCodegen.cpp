@@ -788,14 +788,20 @@ private:
             if (last) {
                 f << "        h" << l + 1 << "[o] = acc;\n";
             } else {
-                switch (net.act) {
-                case Activation::Tanh: f << "        h" << l + 1 << "[o] = tanhf(acc);\n"; break;
-                case Activation::Relu: f << "        h" << l + 1 << "[o] = acc > 0.0f ? acc : 0.0f;\n"; break;
-                case Activation::Silu: f << "        h" << l + 1 << "[o] = acc / (1.0f + expf(-acc));\n"; break;
-                case Activation::Linear: f << "        h" << l + 1 << "[o] = acc;\n"; break;
-                case Activation::Gelu:
-                    f << "        h" << l + 1 << "[o] = 0.5f * acc * (1.0f + tanhf(0.7978845608028654f * (acc + 0.044715f * acc * acc * acc)));\n";
-                    break;
+                // By the activation's name: the reference's Activation has no gelu, and this generator compiles
+                // against either header set.
+                const std::string act = to_string(net.act);
+                f << "        h" << l + 1 << "[o] = ";
+                if (act == "tanh") {
+                    f << "tanhf(acc);\n";
+                } else if (act == "relu") {
+                    f << "acc > 0.0f ? acc : 0.0f;\n";
+                } else if (act == "silu") {
+                    f << "acc / (1.0f + expf(-acc));\n";
+                } else if (act == "gelu") { // the tanh form
+                    f << "0.5f * acc * (1.0f + tanhf(0.7978845608028654f * (acc + 0.044715f * acc * acc * acc)));\n";
+                } else {
+                    f << "acc;\n";
                 }
             }
             f << "    }\n";

@@ -71,7 +71,7 @@ constexpr int kReduceBlocks = 1024; // blocks of a reduction's first pass (its p
 constexpr int kTotals = 1024;      // doubles of conserved totals judged at once
 constexpr int kMaxWindowRegisters = 168; // a lane's registers that still leave a window gather three waves a SIMD
 
-struct Impl final : LawBackend {
+struct Impl final : HipBackend {
     Law* law = nullptr;
     GridShape shape;
     ir::Module module;
@@ -94,6 +94,7 @@ struct Impl final : LawBackend {
     int tile_vx = 0, tile_ty = 0, zchunk_override = 0, remap_override = -1, stream_override = -1;
     bool force_direct = false;
     std::string last_gather;
+    std::vector<std::string> gathers; // per term: the gather kernel its last substep launched
     bool tracking = true; // the gathers mark change flags (the step reads them back)
 
     // Device state.
@@ -1172,6 +1173,8 @@ struct Impl final : LawBackend {
         return true;
     }
 
+    [[nodiscard]] const std::vector<std::string>& gather_kernels() const override { return gathers; }
+
     void run(double dt, int steps, bool respect, bool learned, Law& l, FieldSet& fields, FieldEdits& edits,
         std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps) override
     {
@@ -1365,7 +1368,8 @@ struct Impl final : LawBackend {
                     violations, judges, total_slot);
             }
             check(fe_hip_event_record(event(ev + 1)), "event");
-            tp.kernel = last_gather;
+            gathers.resize(module.terms.size());
+            gathers[t] = last_gather;
             last_gather.clear();
             tp.substeps = substeps[t];
             tp.launches = prof.launches - launches0;
@@ -1466,8 +1470,13 @@ struct Impl final : LawBackend {
 
     // Device pointers of a field's current cells (for callers that keep data on the card).
     [[nodiscard]] void* field_pointer(int f) const { return now[static_cast<std::size_t>(f)]; }
+    void* device_field(int f) override
+    {
+        touched(f); // whoever holds the address may write through it: a cached total no longer stands for it
+        return field_pointer(f);
+    }
     // A caller wrote field f's cells on the card: the host copy is stale until fetched.
-    void device_written(int f)
+    void device_written(int f) override
     {
         stale[static_cast<std::size_t>(f)] = 1;
         touched(f);
@@ -1475,7 +1484,7 @@ struct Impl final : LawBackend {
 
     // The field's fixed-order compensated total and total magnitude, from the card (waits for it).
     // The field's total added into acc[0] (sum) and acc[1] (magnitude) on the card, queued.
-    void field_total_accumulate(int f, double* acc)
+    void field_total_accumulate(int f, double* acc) override
     {
         const std::lock_guard<std::mutex> lock(device_mutex());
         total(f, kTotals - 2);
@@ -1485,7 +1494,7 @@ struct Impl final : LawBackend {
         check(fe_hip_launch(fn("k_acc"), 1, 1, 1, 64, 1, 1, 0, argv), "total accumulate");
     }
 
-    void field_total(int f, double& sum, double& mag)
+    void field_total(int f, double& sum, double& mag) override
     {
         const std::lock_guard<std::mutex> lock(device_mutex());
         total(f, kTotals - 2);
@@ -1499,17 +1508,6 @@ struct Impl final : LawBackend {
 };
 
 } // namespace
-
-const char* to_string(Violation::Kind kind)
-{
-    switch (kind) {
-    case Violation::Kind::Range: return "range";
-    case Violation::Kind::NotFinite: return "not-finite";
-    case Violation::Kind::Conservation: return "conservation";
-    case Violation::Kind::SubstepCap: return "substep-cap";
-    }
-    return "?";
-}
 
 std::shared_ptr<LawBackend> HipSolver::create(Law& law, const GridShape& shape, std::string& why)
 {
@@ -1543,32 +1541,16 @@ std::string HipSolver::unavailable()
     return fe_hip_unavailable();
 }
 
+std::string HipSolver::source(const Law& law)
+{
+    std::string why;
+    return source(law, why);
+}
+
 std::string HipSolver::source(const Law& law, std::string& why)
 {
     hip::LawLayout layout;
     return hip::generate(ir::lower(law), law, layout, why);
-}
-
-void* hip_backend_field_pointer(LawBackend& backend, int field)
-{
-    // whoever holds the address may write through it: the field's cached total no longer stands for it
-    static_cast<Impl&>(backend).touched(field);
-    return static_cast<Impl&>(backend).field_pointer(field);
-}
-
-void hip_backend_device_written(LawBackend& backend, int field)
-{
-    static_cast<Impl&>(backend).device_written(field);
-}
-
-void hip_backend_total_accumulate(LawBackend& backend, int field, double* acc)
-{
-    static_cast<Impl&>(backend).field_total_accumulate(field, acc);
-}
-
-void hip_backend_total(LawBackend& backend, int field, double& sum, double& mag)
-{
-    static_cast<Impl&>(backend).field_total(field, sum, mag);
 }
 
 } // namespace fieldengine::field

@@ -1,67 +1,603 @@
-// Lowering a law's trees to one SSA module. See LawIR.hpp.
+// Lowering a law to its IR, the stage plan, and the IR's JSON and listing. See LawIR.hpp.
+//
+// What must come out is fixed by the IR's JSON, which tools compare across compilers byte for byte: the instruction
+// numbering (arguments first, left to right, program by program: each term's equations then its max_dt, its learned
+// scheme likewise, then the derives), the folding rules, the shapes and read radii, and the cost model that decides
+// which values are stored as stages (src/field/ir/LawIR.cpp:69-526 states them). How it is computed here:
+//
+//   - every per-op fact (name, shape rule, read radius, points read, global, net, foldable) is one row of a table;
+//   - lowering walks each expression tree with an explicit stack (no recursion depth limit on long laws) and interns
+//     instructions in a hash table keyed by their fields, so a value met twice anywhere in the law is one instruction;
+//   - rewrites (folding, identities) are a rule function applied before interning;
+//   - what a program needs (its stages, the fields it reads, how far off its cell it reads each field) is computed by
+//     sweeping the instruction list once from the highest id down, which visits every instruction after all of its
+//     users because arguments always have lower ids: reachability and the longest read distance become one pass each
+//     instead of a walk of every path.
 #include "field/ir/LawIR.hpp"
 
 #include <algorithm>
+#include <array>
+#include <bit>
 #include <cmath>
 #include <cstdio>
-#include <functional>
-#include <map>
+#include <limits>
 #include <set>
 #include <sstream>
+#include <unordered_map>
 
 namespace fieldengine::field::ir {
 
+namespace {
+
+enum class ShapeRule : unsigned char { Scalar, Cell, Args };
+
+struct OpFacts {
+    Op op;
+    const char* name;
+    ShapeRule shape;
+    int radius;       // cells off its own cell it reads its arguments at (-1: its shift offsets say)
+    int points;       // points of each argument one of its points reads
+    bool global;      // needs its argument over the whole lattice first
+    bool net;         // runs a network
+    bool foldable;    // plain arithmetic: all-constant arguments make a constant
+};
+
+constexpr int kOpCount = static_cast<int>(Op::GzT) + 1;
+
+constexpr std::array<OpFacts, kOpCount> kFacts{{
+    {Op::Const, "const", ShapeRule::Scalar, 0, 1, false, false, false},
+    {Op::Param, "param", ShapeRule::Scalar, 0, 1, false, false, false},
+    {Op::Cell, "h", ShapeRule::Scalar, 0, 1, false, false, false},
+    {Op::Rim, "rim", ShapeRule::Cell, 0, 1, false, false, false},
+    {Op::Field, "field", ShapeRule::Cell, 0, 1, false, false, false},
+    {Op::Neg, "neg", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Add, "add", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Sub, "sub", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Mul, "mul", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Div, "div", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Pow, "pow", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Min, "min", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Max, "max", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Abs, "abs", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Sqrt, "sqrt", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Exp, "exp", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Log, "log", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Step, "step", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Pos, "pos", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Clamp, "clamp", ShapeRule::Args, 0, 1, false, false, true},
+    {Op::Lap, "lap", ShapeRule::Cell, 1, 7, false, false, false},
+    {Op::Gx, "gx", ShapeRule::Cell, 1, 2, false, false, false},
+    {Op::Gy, "gy", ShapeRule::Cell, 1, 2, false, false, false},
+    {Op::Gz, "gz", ShapeRule::Cell, 1, 2, false, false, false},
+    {Op::Diffuse, "diffuse", ShapeRule::Cell, 1, 7, false, false, false},
+    {Op::Adv, "adv", ShapeRule::Cell, 1, 7, false, false, false},
+    {Op::Distance, "distance", ShapeRule::Cell, 0, 1, true, false, false},
+    {Op::Poisson, "poisson", ShapeRule::Cell, 0, 1, true, false, false},
+    {Op::MaxOf, "maxof", ShapeRule::Scalar, 0, 1, true, false, false},
+    {Op::MinOf, "minof", ShapeRule::Scalar, 0, 1, true, false, false},
+    {Op::SumOf, "sumof", ShapeRule::Scalar, 0, 1, true, false, false},
+    {Op::Fall, "fall", ShapeRule::Cell, 1, 3, false, false, false},
+    {Op::Spread, "spread", ShapeRule::Cell, 1, 5, false, false, false},
+    {Op::Topple, "topple", ShapeRule::Cell, 1, 13, false, false, false},
+    {Op::Shift, "shift", ShapeRule::Args, -1, 1, false, false, false},
+    {Op::Local, "local", ShapeRule::Scalar, 0, 1, false, false, false},
+    {Op::Mlp, "mlp", ShapeRule::Args, 0, 1, false, true, false},
+    {Op::Flux, "flux", ShapeRule::Cell, 1, 1, false, true, false},
+    {Op::LDiffuse, "ldiffuse", ShapeRule::Cell, 2, 1, false, true, false},
+    {Op::Dt, "dt", ShapeRule::Scalar, 0, 1, false, false, false},
+    {Op::Tridiag, "tridiag", ShapeRule::Cell, 0, 1, true, false, false},
+    {Op::ShiftT, "shift_t", ShapeRule::Cell, -1, 1, false, false, false},
+    {Op::GxT, "gx_t", ShapeRule::Cell, 1, 1, false, false, false},
+    {Op::GyT, "gy_t", ShapeRule::Cell, 1, 1, false, false, false},
+    {Op::GzT, "gz_t", ShapeRule::Cell, 1, 1, false, false, false},
+}};
+
+constexpr bool table_in_op_order()
+{
+    for (int i = 0; i < kOpCount; ++i) {
+        if (static_cast<int>(kFacts[static_cast<std::size_t>(i)].op) != i) {
+            return false;
+        }
+    }
+    return true;
+}
+static_assert(table_in_op_order(), "one row per op, in the op's order");
+
+const OpFacts& facts(Op op) { return kFacts[static_cast<std::size_t>(op)]; }
+
+// The value the reference CPU's scalar path gives an arithmetic op of numbers (in double).
+double fold(Op op, const std::vector<double>& v)
+{
+    const double a = v[0];
+    const double b = v.size() > 1 ? v[1] : 0.0;
+    switch (op) {
+    case Op::Neg: return -a;
+    case Op::Abs: return std::fabs(a);
+    case Op::Sqrt: return std::sqrt(0.0 < a ? a : 0.0);
+    case Op::Exp: return std::exp(80.0 < a ? 80.0 : a);
+    case Op::Log: return std::log(a < 1e-30 ? 1e-30 : a);
+    case Op::Step: return a > 0.0 ? 1.0 : 0.0;
+    case Op::Pos: return a < 0.0 ? 0.0 : a;
+    case Op::Add: return a + b;
+    case Op::Sub: return a - b;
+    case Op::Mul: return a * b;
+    case Op::Div: return b == 0.0 ? 0.0 : a / b;
+    case Op::Pow: return std::pow(a, b);
+    case Op::Min: return b < a ? b : a;
+    case Op::Max: return a < b ? b : a;
+    case Op::Clamp: {
+        const double top = b < v[2] ? v[2] : b;
+        return a < b ? b : (top < a ? top : a);
+    }
+    default: return 0.0;
+    }
+}
+
+// An instruction's identity: two instructions with equal keys are one value.
+struct Key {
+    int op;
+    std::uint64_t number;
+    int index;
+    int iterations;
+    std::array<int, 3> offset;
+    std::vector<int> args;
+    bool operator==(const Key&) const = default;
+};
+
+struct KeyHash {
+    std::size_t operator()(const Key& k) const
+    {
+        std::uint64_t h = 0x9e3779b97f4a7c15ull;
+        const auto mix = [&](std::uint64_t v) { h = (h ^ v) * 0x100000001b3ull; h ^= h >> 29; };
+        mix(static_cast<std::uint64_t>(k.op));
+        mix(k.number);
+        mix(static_cast<std::uint64_t>(static_cast<std::uint32_t>(k.index)));
+        mix(static_cast<std::uint64_t>(k.iterations));
+        for (const int o : k.offset) {
+            mix(static_cast<std::uint64_t>(static_cast<std::uint32_t>(o)));
+        }
+        for (const int a : k.args) {
+            mix(static_cast<std::uint64_t>(a));
+        }
+        return static_cast<std::size_t>(h);
+    }
+};
+
+// A constant's bits as its identity; every NaN of one sign is one value.
+std::uint64_t number_bits(double v)
+{
+    if (std::isnan(v)) {
+        return std::signbit(v) ? 0xfff8000000000000ull : 0x7ff8000000000000ull;
+    }
+    return std::bit_cast<std::uint64_t>(v);
+}
+
+class Builder {
+public:
+    explicit Builder(Module& m) : m_(m) {}
+
+    // The instruction computing `root`'s value: its arguments' first, left to right, depth first.
+    int lower(const Node& root)
+    {
+        struct Frame {
+            const Node* node;
+            std::size_t next = 0;
+            std::vector<int> args;
+        };
+        std::vector<Frame> stack;
+        stack.push_back(Frame{&root});
+        int last = -1;
+        while (!stack.empty()) {
+            Frame& f = stack.back();
+            if (f.next == 0) {
+                if (const auto seen = done_.find(f.node); seen != done_.end()) {
+                    last = seen->second;
+                    stack.pop_back();
+                    if (!stack.empty()) {
+                        stack.back().args.push_back(last);
+                    }
+                    continue;
+                }
+            }
+            // A shift's offsets are part of the instruction, not arguments.
+            const std::size_t operands = is_shift(f.node->op) ? 1 : f.node->args.size();
+            if (f.next < operands) {
+                const Node* child = f.node->args[f.next++].get();
+                stack.push_back(Frame{child});
+                continue;
+            }
+            last = finish(*f.node, std::move(f.args));
+            done_.emplace(f.node, last);
+            stack.pop_back();
+            if (!stack.empty()) {
+                stack.back().args.push_back(last);
+            }
+        }
+        return last;
+    }
+
+private:
+    static bool is_shift(Op op) { return op == Op::Shift || op == Op::ShiftT; }
+
+    [[nodiscard]] const Inst& inst(int id) const { return m_.insts[static_cast<std::size_t>(id)]; }
+    [[nodiscard]] bool number_is(int id, double v) const { return inst(id).op == Op::Const && inst(id).number == v; }
+    [[nodiscard]] bool scalar(int id) const { return inst(id).shape == Shape::Scalar; }
+
+    int finish(const Node& n, std::vector<int> args)
+    {
+        Inst in;
+        in.op = n.op;
+        in.number = n.number;
+        in.index = n.index;
+        in.iterations = n.iterations;
+        in.args = std::move(args);
+        if (is_shift(n.op)) {
+            for (int a = 0; a < 3; ++a) {
+                in.offset[a] = static_cast<int>(n.args[static_cast<std::size_t>(a) + 1]->number);
+            }
+        }
+        const int replaced = rewrite(in);
+        if (replaced >= 0) {
+            ++m_.folded;
+            return replaced;
+        }
+        return intern(std::move(in));
+    }
+
+    int number(double v)
+    {
+        Inst c;
+        c.op = Op::Const;
+        c.number = v;
+        return intern(std::move(c));
+    }
+
+    // The instruction `in` reduces to without being computed, or -1 when it stands.
+    int rewrite(const Inst& in)
+    {
+        const auto& a = in.args;
+        if (facts(in.op).foldable &&
+            std::all_of(a.begin(), a.end(), [&](int id) { return inst(id).op == Op::Const; })) {
+            std::vector<double> v;
+            for (const int id : a) {
+                v.push_back(inst(id).number);
+            }
+            return number(fold(in.op, v));
+        }
+        switch (in.op) {
+        case Op::Mul: // 0 * anything is 0 without looking at the anything (as the CPU computes it); 1 * x is x
+            if (number_is(a[0], 0.0) || number_is(a[1], 0.0)) {
+                return number(0.0);
+            }
+            return number_is(a[0], 1.0) ? a[1] : number_is(a[1], 1.0) ? a[0] : -1;
+        case Op::Add:
+            return number_is(a[0], 0.0) ? a[1] : number_is(a[1], 0.0) ? a[0] : -1;
+        case Op::Sub:
+            return number_is(a[1], 0.0) ? a[0] : -1;
+        case Op::Div:
+            if (number_is(a[1], 1.0)) {
+                return a[0];
+            }
+            return number_is(a[0], 0.0) ? number(0.0) : -1;
+        case Op::Neg:
+            return inst(a[0]).op == Op::Neg ? inst(a[0]).args[0] : -1;
+        case Op::Shift:
+            return scalar(a[0]) || (in.offset[0] == 0 && in.offset[1] == 0 && in.offset[2] == 0) ? a[0] : -1;
+        // What reads a field that is one number everywhere sees no difference, so no gradient and no flux.
+        case Op::Lap: case Op::Gx: case Op::Gy: case Op::Gz: case Op::Adv: case Op::Fall: case Op::Spread: case Op::Topple:
+            return scalar(a[0]) ? number(0.0) : -1;
+        case Op::Diffuse: case Op::LDiffuse:
+            return scalar(a[1]) ? number(0.0) : -1;
+        default:
+            return -1;
+        }
+    }
+
+    int intern(Inst in)
+    {
+        const OpFacts& f = facts(in.op);
+        switch (f.shape) {
+        case ShapeRule::Scalar: in.shape = Shape::Scalar; break;
+        case ShapeRule::Cell: in.shape = Shape::Cell; break;
+        case ShapeRule::Args:
+            in.shape = std::any_of(in.args.begin(), in.args.end(), [&](int id) { return !scalar(id); }) ? Shape::Cell
+                                                                                                       : Shape::Scalar;
+            break;
+        }
+        in.radius = f.radius >= 0
+            ? f.radius
+            : std::max({std::abs(in.offset[0]), std::abs(in.offset[1]), std::abs(in.offset[2])});
+        in.global = f.global;
+        in.net = f.net;
+        Key key{static_cast<int>(in.op), number_bits(in.number), in.index, in.iterations,
+            {in.offset[0], in.offset[1], in.offset[2]}, in.args};
+        const int next = static_cast<int>(m_.insts.size());
+        const auto [it, fresh] = ids_.try_emplace(std::move(key), next);
+        if (!fresh) {
+            ++m_.shared;
+            return it->second;
+        }
+        m_.insts.push_back(std::move(in));
+        return next;
+    }
+
+    Module& m_;
+    std::unordered_map<const Node*, int> done_;
+    std::unordered_map<Key, int, KeyHash> ids_;
+};
+
+// Decides which values are stored as stages and what a point of every instruction costs with everything else
+// inlined, in one pass over the instructions in id order (a value's readers come after it). A cell value read by a
+// stencil is stored once instead of inlined when one point of the stencil would cost more than kInlineBudget through
+// it; whatever a tridiagonal solve reads (a field aside) is stored, since its lines are walked one cell at a time.
+void plan(Module& m)
+{
+    const std::size_t n = m.insts.size();
+    m.materialized.assign(n, 0);
+    for (std::size_t id = 0; id < n; ++id) {
+        Inst& in = m.insts[id];
+        const long long per = facts(in.op).points;
+        long long cost = 1;
+        for (const int a : in.args) {
+            Inst& src = m.insts[static_cast<std::size_t>(a)];
+            const bool stored = m.is_stage(a);
+            long long unit = stored ? 1 : src.cost;
+            const bool storable = !stored && src.shape == Shape::Cell && src.op != Op::Field;
+            if (storable && (in.op == Op::Tridiag || (in.radius > 0 && unit * per > kInlineBudget))) {
+                m.materialized[static_cast<std::size_t>(a)] = 1;
+                unit = 1;
+            }
+            cost += unit * per;
+            ++src.uses;
+        }
+        in.cost = static_cast<int>(std::min<long long>(cost, 1 << 20));
+    }
+}
+
+// Marks every instruction reachable from `roots` (through all arguments), sweeping ids downwards.
+std::vector<char> reachable(const Module& m, const std::vector<int>& roots)
+{
+    std::vector<char> mark(m.insts.size(), 0);
+    for (const int r : roots) {
+        mark[static_cast<std::size_t>(r)] = 1;
+    }
+    for (std::size_t id = m.insts.size(); id-- > 0;) {
+        if (mark[id]) {
+            for (const int a : m.insts[id].args) {
+                mark[static_cast<std::size_t>(a)] = 1;
+            }
+        }
+    }
+    return mark;
+}
+
+// The instructions of a program that run over the lattice before its gather: stored cell values and global ops.
+std::vector<int> stages_of(const Module& m, const std::vector<char>& mark)
+{
+    std::vector<int> out;
+    for (std::size_t id = 0; id < mark.size(); ++id) {
+        const Inst& in = m.insts[id];
+        if (mark[id] && ((m.is_stage(static_cast<int>(id)) && in.shape == Shape::Cell) || in.global)) {
+            out.push_back(static_cast<int>(id));
+        }
+    }
+    return out;
+}
+
+// For every field, the farthest (in cells, along any path of inlined ops) the gather of `roots` reads it from its own
+// cell; -1 where it does not read it. A stage is read where it was stored (it saw the state as it was), so a path
+// stops at one unless it is a root itself; nets and global ops read what they read before the gather.
+std::vector<int> read_distance(const Module& m, const std::vector<int>& roots, std::size_t field_count)
+{
+    constexpr int kNone = std::numeric_limits<int>::min();
+    std::vector<int> entered(m.insts.size(), kNone); // the farthest offset a path enters the instruction at
+    std::vector<char> root(m.insts.size(), 0);
+    for (const int r : roots) {
+        root[static_cast<std::size_t>(r)] = 1;
+    }
+    std::vector<int> out(field_count, -1);
+    for (std::size_t id = m.insts.size(); id-- > 0;) {
+        const Inst& in = m.insts[id];
+        int at = root[id] ? std::max(entered[id], 0) : (m.is_stage(static_cast<int>(id)) ? kNone : entered[id]);
+        if (root[id] && m.is_stage(static_cast<int>(id))) {
+            at = 0; // a stored root is entered at its own cell only
+        }
+        if (at == kNone) {
+            continue;
+        }
+        if (in.op == Op::Field) {
+            int& d = out[static_cast<std::size_t>(in.index)];
+            d = std::max(d, at);
+            continue;
+        }
+        if (in.net || in.global) {
+            continue;
+        }
+        for (const int a : in.args) {
+            int& e = entered[static_cast<std::size_t>(a)];
+            e = std::max(e, at + in.radius);
+        }
+    }
+    return out;
+}
+
+Program program(Builder& b, const std::string& name, const std::vector<field::Equation>& eqs, const Node* max_dt)
+{
+    Program p;
+    p.name = name;
+    std::set<int> writes;
+    for (const field::Equation& eq : eqs) {
+        p.equations.push_back(Equation{eq.target, b.lower(*eq.rhs)});
+        writes.insert(eq.target);
+    }
+    if (max_dt != nullptr) {
+        p.max_dt = b.lower(*max_dt);
+    }
+    p.writes.assign(writes.begin(), writes.end());
+    return p;
+}
+
+void complete(const Module& m, Program& p, std::size_t field_count)
+{
+    std::vector<int> roots;
+    for (const Equation& eq : p.equations) {
+        roots.push_back(eq.value);
+    }
+    const std::vector<char> mark = reachable(m, roots);
+    p.stages = stages_of(m, mark);
+    if (p.max_dt >= 0) {
+        p.limit_stages = stages_of(m, reachable(m, {p.max_dt}));
+    }
+    p.reads.clear();
+    std::set<int> reads;
+    for (std::size_t id = 0; id < mark.size(); ++id) {
+        if (mark[id] && m.insts[id].op == Op::Field) {
+            reads.insert(m.insts[id].index);
+        }
+    }
+    p.reads.assign(reads.begin(), reads.end());
+    // In place only when no field it writes is read off its own cell: another cell's thread may already have
+    // stored what this one would read.
+    const std::vector<int> distance = read_distance(m, roots, field_count);
+    p.in_place = std::none_of(p.writes.begin(), p.writes.end(),
+        [&](int f) { return f >= 0 && static_cast<std::size_t>(f) < distance.size() && distance[static_cast<std::size_t>(f)] > 0; });
+}
+
+// ---- JSON ---------------------------------------------------------------------------------------------------------
+
+class JsonText {
+public:
+    std::string str() const { return out_.str(); }
+
+    JsonText& open(char c)
+    {
+        comma();
+        out_ << c;
+        first_.push_back(true);
+        return *this;
+    }
+    JsonText& close(char c)
+    {
+        first_.pop_back();
+        out_ << c;
+        return *this;
+    }
+    // The next element of an array that lists one element a line.
+    JsonText& line()
+    {
+        comma();
+        out_ << '\n';
+        pending_ = true; // the element follows without another separator
+        return *this;
+    }
+    JsonText& key(const char* k, bool on_new_line = false)
+    {
+        comma();
+        if (on_new_line) {
+            out_ << '\n';
+        }
+        out_ << '"' << k << "\":";
+        pending_ = true; // the value follows without a separator
+        return *this;
+    }
+    JsonText& str(const std::string& s)
+    {
+        comma();
+        out_ << '"';
+        for (const char c : s) {
+            if (c == '"' || c == '\\') {
+                out_ << '\\' << c;
+            } else {
+                out_ << (static_cast<unsigned char>(c) < 0x20 ? ' ' : c);
+            }
+        }
+        out_ << '"';
+        return *this;
+    }
+    JsonText& num(double v)
+    {
+        comma();
+        if (std::isnan(v)) {
+            out_ << '0';
+        } else if (std::isinf(v)) {
+            out_ << (v > 0 ? "1e300" : "-1e300");
+        } else {
+            char text[40];
+            std::snprintf(text, sizeof text, "%.17g", v);
+            out_ << text;
+        }
+        return *this;
+    }
+    JsonText& integer(long long v)
+    {
+        comma();
+        out_ << v;
+        return *this;
+    }
+    JsonText& boolean(bool v)
+    {
+        comma();
+        out_ << (v ? "true" : "false");
+        return *this;
+    }
+    JsonText& ints(const std::vector<int>& v)
+    {
+        open('[');
+        for (const int x : v) {
+            integer(x);
+        }
+        return close(']');
+    }
+    JsonText& literal(const char* text)
+    {
+        comma();
+        out_ << text;
+        return *this;
+    }
+
+private:
+    // A separator before a value: none after an opening bracket or a key.
+    void comma()
+    {
+        if (pending_) {
+            pending_ = false;
+            return;
+        }
+        if (!first_.empty()) {
+            if (!first_.back()) {
+                out_ << ',';
+            }
+            first_.back() = false;
+        }
+        pending_ = false;
+    }
+
+    std::ostringstream out_;
+    std::vector<bool> first_;
+    bool pending_ = false;
+};
+
+void program_json(JsonText& j, const Program& p)
+{
+    j.open('{').key("name").str(p.name).key("equations").open('[');
+    for (const Equation& eq : p.equations) {
+        j.open('[').integer(eq.target).integer(eq.value).close(']');
+    }
+    j.close(']');
+    j.key("stages").ints(p.stages).key("limit_stages").ints(p.limit_stages).key("in_place").boolean(p.in_place);
+    j.key("max_dt").integer(p.max_dt).key("reads").ints(p.reads).key("writes").ints(p.writes).key("conserves").open('[');
+    for (const std::vector<int>& g : p.conserves) {
+        j.ints(g);
+    }
+    j.close(']').key("accumulate").boolean(p.accumulate).close('}');
+}
+
+} // namespace
+
 const char* op_name(Op op)
 {
-    // The names the reference's JSON and listing use (src/field/ir/LawIR.cpp:13-61).
-    switch (op) {
-    case Op::Const: return "const";
-    case Op::Param: return "param";
-    case Op::Cell: return "h";
-    case Op::Rim: return "rim";
-    case Op::Field: return "field";
-    case Op::Neg: return "neg";
-    case Op::Add: return "add";
-    case Op::Sub: return "sub";
-    case Op::Mul: return "mul";
-    case Op::Div: return "div";
-    case Op::Pow: return "pow";
-    case Op::Min: return "min";
-    case Op::Max: return "max";
-    case Op::Abs: return "abs";
-    case Op::Sqrt: return "sqrt";
-    case Op::Exp: return "exp";
-    case Op::Log: return "log";
-    case Op::Step: return "step";
-    case Op::Pos: return "pos";
-    case Op::Clamp: return "clamp";
-    case Op::Lap: return "lap";
-    case Op::Gx: return "gx";
-    case Op::Gy: return "gy";
-    case Op::Gz: return "gz";
-    case Op::Diffuse: return "diffuse";
-    case Op::Adv: return "adv";
-    case Op::Distance: return "distance";
-    case Op::Poisson: return "poisson";
-    case Op::MaxOf: return "maxof";
-    case Op::MinOf: return "minof";
-    case Op::SumOf: return "sumof";
-    case Op::Fall: return "fall";
-    case Op::Spread: return "spread";
-    case Op::Topple: return "topple";
-    case Op::Shift: return "shift";
-    case Op::Local: return "local";
-    case Op::Mlp: return "mlp";
-    case Op::Flux: return "flux";
-    case Op::LDiffuse: return "ldiffuse";
-    case Op::Dt: return "dt";
-    case Op::Tridiag: return "tridiag";
-    case Op::ShiftT: return "shift_t";
-    case Op::GxT: return "gx_t";
-    case Op::GyT: return "gy_t";
-    case Op::GzT: return "gz_t";
-    }
-    return "?";
+    const int i = static_cast<int>(op);
+    return i >= 0 && i < kOpCount ? kFacts[static_cast<std::size_t>(i)].name : "?";
 }
 
 bool Module::is_stage(int id) const
@@ -70,561 +606,130 @@ bool Module::is_stage(int id) const
     return in.global || in.net || materialized[static_cast<std::size_t>(id)] != 0;
 }
 
-namespace {
-
-bool elementwise(Op op)
-{
-    switch (op) {
-    case Op::Neg: case Op::Add: case Op::Sub: case Op::Mul: case Op::Div: case Op::Pow: case Op::Min: case Op::Max:
-    case Op::Abs: case Op::Sqrt: case Op::Exp: case Op::Log: case Op::Step: case Op::Pos: case Op::Clamp:
-        return true;
-    default:
-        return false;
-    }
-}
-
-// The CPU's scalar path in double (src/field/Law.cpp:757-857): what a node of numbers evaluates to.
-double fold(Op op, const std::vector<double>& a)
-{
-    switch (op) {
-    case Op::Neg: return -a[0];
-    case Op::Abs: return std::abs(a[0]);
-    case Op::Sqrt: return std::sqrt(0.0 < a[0] ? a[0] : 0.0); // std::max(0.0, a): a NaN becomes 0
-    case Op::Exp: return std::exp(80.0 < a[0] ? 80.0 : a[0]);
-    case Op::Log: return std::log(a[0] < 1e-30 ? 1e-30 : a[0]);
-    case Op::Step: return a[0] > 0.0 ? 1.0 : 0.0;
-    case Op::Pos: return a[0] < 0.0 ? 0.0 : a[0];
-    case Op::Add: return a[0] + a[1];
-    case Op::Sub: return a[0] - a[1];
-    case Op::Mul: return a[0] == 0.0 || a[1] == 0.0 ? 0.0 : a[0] * a[1];
-    case Op::Div: return a[1] == 0.0 ? 0.0 : a[0] / a[1];
-    case Op::Pow: return std::pow(a[0], a[1]);
-    case Op::Min: return a[1] < a[0] ? a[1] : a[0];
-    case Op::Max: return a[0] < a[1] ? a[1] : a[0];
-    case Op::Clamp: {
-        const double hi = a[1] < a[2] ? a[2] : a[1];
-        return a[0] < a[1] ? a[1] : (hi < a[0] ? hi : a[0]);
-    }
-    default: return 0.0;
-    }
-}
-
-// How many points of each argument one point of the op reads (the cost model's multiplier).
-int points(Op op)
-{
-    switch (op) {
-    case Op::Lap: case Op::Diffuse: case Op::Adv: return 7;
-    case Op::Gx: case Op::Gy: case Op::Gz: return 2;
-    case Op::Fall: return 3;
-    case Op::Spread: return 5;
-    case Op::Topple: return 13;
-    default: return 1;
-    }
-}
-
-class Lowering {
-public:
-    explicit Lowering(Module& m) : m_(m) {}
-
-    int lower(const Node& n)
-    {
-        if (const auto it = memo_.find(&n); it != memo_.end()) {
-            return it->second;
-        }
-        Inst in;
-        in.op = n.op;
-        in.number = n.number;
-        in.index = n.index;
-        in.iterations = n.iterations;
-        if (n.op == Op::Shift || n.op == Op::ShiftT) {
-            in.args.push_back(lower(*n.args[0]));
-            for (int a = 0; a < 3; ++a) {
-                in.offset[a] = static_cast<int>(n.args[static_cast<std::size_t>(a) + 1]->number);
-            }
-        } else {
-            for (const NodePtr& a : n.args) {
-                in.args.push_back(lower(*a));
-            }
-        }
-        const int id = simplify(std::move(in));
-        memo_.emplace(&n, id);
-        return id;
-    }
-
-private:
-    [[nodiscard]] const Inst& at(int id) const { return m_.insts[static_cast<std::size_t>(id)]; }
-    [[nodiscard]] bool is(int id, double v) const { return at(id).op == Op::Const && at(id).number == v; }
-
-    int constant(double v)
-    {
-        Inst c;
-        c.op = Op::Const;
-        c.number = v;
-        return add(std::move(c));
-    }
-
-    int folded(int id)
-    {
-        ++m_.folded;
-        return id;
-    }
-
-    int simplify(Inst in)
-    {
-        if (elementwise(in.op) && std::all_of(in.args.begin(), in.args.end(), [&](int a) { return at(a).op == Op::Const; })) {
-            std::vector<double> v;
-            for (const int a : in.args) {
-                v.push_back(at(a).number);
-            }
-            return folded(constant(fold(in.op, v)));
-        }
-        switch (in.op) {
-        case Op::Mul:
-            if (is(in.args[0], 0.0) || is(in.args[1], 0.0)) {
-                return folded(constant(0.0)); // the CPU never looks at the other factor of a zero
-            }
-            if (is(in.args[0], 1.0)) {
-                return folded(in.args[1]);
-            }
-            if (is(in.args[1], 1.0)) {
-                return folded(in.args[0]);
-            }
-            break;
-        case Op::Add:
-            if (is(in.args[0], 0.0)) {
-                return folded(in.args[1]);
-            }
-            if (is(in.args[1], 0.0)) {
-                return folded(in.args[0]);
-            }
-            break;
-        case Op::Sub:
-            if (is(in.args[1], 0.0)) {
-                return folded(in.args[0]);
-            }
-            break;
-        case Op::Div:
-            if (is(in.args[1], 1.0)) {
-                return folded(in.args[0]);
-            }
-            if (is(in.args[0], 0.0)) {
-                return folded(constant(0.0));
-            }
-            break;
-        case Op::Neg:
-            if (at(in.args[0]).op == Op::Neg) {
-                return folded(at(in.args[0]).args[0]);
-            }
-            break;
-        case Op::Shift:
-            if (at(in.args[0]).shape == Shape::Scalar || (in.offset[0] == 0 && in.offset[1] == 0 && in.offset[2] == 0)) {
-                return folded(in.args[0]);
-            }
-            break;
-        case Op::Lap: case Op::Gx: case Op::Gy: case Op::Gz:
-            if (at(in.args[0]).shape == Shape::Scalar) {
-                return folded(constant(0.0));
-            }
-            break;
-        case Op::Diffuse: case Op::LDiffuse:
-            if (at(in.args[1]).shape == Shape::Scalar) {
-                return folded(constant(0.0));
-            }
-            break;
-        case Op::Adv: case Op::Fall: case Op::Spread: case Op::Topple:
-            if (at(in.args[0]).shape == Shape::Scalar) {
-                return folded(constant(0.0));
-            }
-            break;
-        default:
-            break;
-        }
-        return add(std::move(in));
-    }
-
-    int add(Inst in)
-    {
-        switch (in.op) {
-        case Op::Const: case Op::Param: case Op::Cell: case Op::Dt: case Op::Local: case Op::MaxOf: case Op::MinOf:
-        case Op::SumOf:
-            in.shape = Shape::Scalar;
-            break;
-        case Op::Field: case Op::Rim: case Op::Distance: case Op::Poisson: case Op::Fall: case Op::Spread: case Op::Topple:
-        case Op::Flux: case Op::LDiffuse: case Op::Lap: case Op::Gx: case Op::Gy: case Op::Gz: case Op::Diffuse: case Op::Adv:
-        case Op::Tridiag: case Op::ShiftT: case Op::GxT: case Op::GyT: case Op::GzT:
-            in.shape = Shape::Cell;
-            break;
-        default:
-            in.shape = std::any_of(in.args.begin(), in.args.end(), [&](int a) { return at(a).shape == Shape::Cell; })
-                ? Shape::Cell
-                : Shape::Scalar;
-            break;
-        }
-        switch (in.op) {
-        case Op::Lap: case Op::Gx: case Op::Gy: case Op::Gz: case Op::Diffuse: case Op::Adv: case Op::Fall: case Op::Spread:
-        case Op::Topple: case Op::Flux: case Op::GxT: case Op::GyT: case Op::GzT:
-            in.radius = 1;
-            break;
-        case Op::LDiffuse:
-            in.radius = 2;
-            break;
-        case Op::Shift: case Op::ShiftT:
-            in.radius = std::max({std::abs(in.offset[0]), std::abs(in.offset[1]), std::abs(in.offset[2])});
-            break;
-        default:
-            break;
-        }
-        in.global = in.op == Op::Distance || in.op == Op::Poisson || in.op == Op::MaxOf || in.op == Op::MinOf ||
-            in.op == Op::SumOf || in.op == Op::Tridiag;
-        in.net = in.op == Op::Mlp || in.op == Op::Flux || in.op == Op::LDiffuse;
-        std::ostringstream key;
-        key << static_cast<int>(in.op) << '|' << std::hexfloat << in.number << '|' << in.index << '|' << in.iterations << '|'
-            << in.offset[0] << ',' << in.offset[1] << ',' << in.offset[2];
-        for (const int a : in.args) {
-            key << '|' << a;
-        }
-        const auto [it, inserted] = table_.emplace(key.str(), static_cast<int>(m_.insts.size()));
-        if (!inserted) {
-            ++m_.shared;
-            return it->second;
-        }
-        m_.insts.push_back(std::move(in));
-        return it->second;
-    }
-
-    Module& m_;
-    std::map<const Node*, int> memo_;
-    std::map<std::string, int> table_;
-};
-
-void plan_stages(Module& m)
-{
-    m.materialized.assign(m.insts.size(), 0);
-    for (std::size_t id = 0; id < m.insts.size(); ++id) {
-        Inst& in = m.insts[id];
-        const int per = points(in.op);
-        int cost = 1;
-        for (const int a : in.args) {
-            Inst& arg = m.insts[static_cast<std::size_t>(a)];
-            const bool stored = m.is_stage(a);
-            int c = stored ? 1 : arg.cost;
-            // A stencil reading an expensive argument at several points stores it once instead; a tridiagonal solve
-            // walks its lines serially, so everything it reads is computed over the lattice first.
-            const bool walked = in.op == Op::Tridiag && arg.op != Op::Field;
-            if (!stored && arg.shape == Shape::Cell && arg.op != Op::Field && (walked || (in.radius > 0 && c * per > kInlineBudget))) {
-                m.materialized[static_cast<std::size_t>(a)] = 1;
-                c = 1;
-            }
-            cost += c * per;
-            ++arg.uses;
-        }
-        in.cost = std::min(cost, 1 << 20);
-    }
-}
-
-void collect_stages(const Module& m, int id, std::set<int>& out, std::set<int>& seen)
-{
-    if (!seen.insert(id).second) {
-        return;
-    }
-    for (const int a : m.insts[static_cast<std::size_t>(id)].args) {
-        collect_stages(m, a, out, seen);
-    }
-    const Inst& in = m.insts[static_cast<std::size_t>(id)];
-    if ((m.is_stage(id) && in.shape == Shape::Cell) || in.global) {
-        out.insert(id);
-    }
-}
-
-// How far from its cell the gather kernel (everything not a stage, inlined) reads each field.
-void reach(const Module& m, int id, int offset, bool root, std::map<int, int>& out)
-{
-    const Inst& in = m.insts[static_cast<std::size_t>(id)];
-    if (!root && m.is_stage(id)) {
-        return;
-    }
-    if (in.op == Op::Field) {
-        int& r = out[in.index];
-        r = std::max(r, offset);
-        return;
-    }
-    if (in.net || in.global) {
-        return;
-    }
-    for (const int a : in.args) {
-        reach(m, a, offset + in.radius, false, out);
-    }
-}
-
-Program make_program(Lowering& low, const std::string& name, const std::vector<field::Equation>& eqs, const Node* max_dt)
-{
-    Program p;
-    p.name = name;
-    std::set<int> writes;
-    for (const field::Equation& eq : eqs) {
-        p.equations.push_back(Equation{eq.target, low.lower(*eq.rhs)});
-        writes.insert(eq.target);
-    }
-    if (max_dt != nullptr) {
-        p.max_dt = low.lower(*max_dt);
-    }
-    p.writes.assign(writes.begin(), writes.end());
-    return p;
-}
-
-void finish_program(const Module& m, Program& p)
-{
-    std::set<int> stages, seen;
-    for (const Equation& eq : p.equations) {
-        collect_stages(m, eq.value, stages, seen);
-    }
-    p.stages.assign(stages.begin(), stages.end());
-    if (p.max_dt >= 0) {
-        std::set<int> limit, seen_limit;
-        collect_stages(m, p.max_dt, limit, seen_limit);
-        p.limit_stages.assign(limit.begin(), limit.end());
-    }
-    std::set<int> reads;
-    std::function<void(int)> walk = [&](int id) {
-        const Inst& in = m.insts[static_cast<std::size_t>(id)];
-        if (in.op == Op::Field) {
-            reads.insert(in.index);
-        }
-        for (const int a : in.args) {
-            walk(a);
-        }
-    };
-    for (const Equation& eq : p.equations) {
-        walk(eq.value);
-    }
-    p.reads.assign(reads.begin(), reads.end());
-    std::map<int, int> offsets;
-    for (const Equation& eq : p.equations) {
-        reach(m, eq.value, 0, true, offsets);
-    }
-    p.in_place = true;
-    for (const int t : p.writes) {
-        const auto it = offsets.find(t);
-        if (it != offsets.end() && it->second > 0) {
-            p.in_place = false;
-        }
-    }
-}
-
-void json_number(std::ostream& out, double v)
-{
-    if (!std::isfinite(v)) {
-        out << (v > 0 ? "1e300" : (v < 0 ? "-1e300" : "0"));
-        return;
-    }
-    char text[40];
-    std::snprintf(text, sizeof(text), "%.17g", v);
-    out << text;
-}
-
-void json_string(std::ostream& out, const std::string& s)
-{
-    out << '"';
-    for (const char c : s) {
-        if (c == '"' || c == '\\') {
-            out << '\\' << c;
-        } else if (static_cast<unsigned char>(c) < 0x20) {
-            out << ' ';
-        } else {
-            out << c;
-        }
-    }
-    out << '"';
-}
-
-void json_ints(std::ostream& out, const std::vector<int>& v)
-{
-    out << '[';
-    for (std::size_t i = 0; i < v.size(); ++i) {
-        out << (i ? "," : "") << v[i];
-    }
-    out << ']';
-}
-
-void json_program(std::ostream& out, const Program& p)
-{
-    out << "{\"name\":";
-    json_string(out, p.name);
-    out << ",\"equations\":[";
-    for (std::size_t e = 0; e < p.equations.size(); ++e) {
-        out << (e ? "," : "") << '[' << p.equations[e].target << ',' << p.equations[e].value << ']';
-    }
-    out << "],\"stages\":";
-    json_ints(out, p.stages);
-    out << ",\"limit_stages\":";
-    json_ints(out, p.limit_stages);
-    out << ",\"in_place\":" << (p.in_place ? "true" : "false") << ",\"max_dt\":" << p.max_dt << ",\"reads\":";
-    json_ints(out, p.reads);
-    out << ",\"writes\":";
-    json_ints(out, p.writes);
-    out << ",\"conserves\":[";
-    for (std::size_t g = 0; g < p.conserves.size(); ++g) {
-        out << (g ? "," : "");
-        json_ints(out, p.conserves[g]);
-    }
-    out << "],\"accumulate\":" << (p.accumulate ? "true" : "false") << '}';
-}
-
-} // namespace
-
 Module lower(const Law& law)
 {
     Module m;
-    Lowering low(m);
-    for (const Term& term : law.terms) {
-        m.terms.push_back(make_program(low, term.name, term.equations, term.max_dt.get()));
-        m.terms.back().conserves = term.conserves;
-        m.learned.push_back(make_program(low, term.name + "~learned", term.learned, term.learned_max_dt.get()));
-        m.learned.back().conserves = term.conserves;
+    Builder b(m);
+    for (const Term& t : law.terms) {
+        m.terms.push_back(program(b, t.name, t.equations, t.max_dt.get()));
+        m.terms.back().conserves = t.conserves;
+        m.learned.push_back(program(b, t.name + "~learned", t.learned, t.learned_max_dt.get()));
+        m.learned.back().conserves = t.conserves;
     }
-    std::set<int> started;
+    std::set<int> set_before;
     for (const Derive& d : law.derives) {
-        Program p = make_program(low, "derive " + law.fields[static_cast<std::size_t>(d.target)].name,
+        Program p = program(b, "derive " + law.fields[static_cast<std::size_t>(d.target)].name,
             {field::Equation{d.target, d.rhs}}, nullptr);
-        p.accumulate = d.accumulate && started.contains(d.target);
-        started.insert(d.target);
+        p.accumulate = d.accumulate && set_before.contains(d.target);
+        set_before.insert(d.target);
         m.derives.push_back(std::move(p));
     }
-    plan_stages(m);
-    for (Program& p : m.terms) {
-        finish_program(m, p);
-    }
-    for (Program& p : m.learned) {
-        finish_program(m, p);
+    plan(m);
+    for (auto* group : {&m.terms, &m.learned, &m.derives}) {
+        for (Program& p : *group) {
+            complete(m, p, law.fields.size());
+        }
     }
     for (Program& p : m.derives) {
-        finish_program(m, p);
-        p.in_place = false; // a derive replaces its field: its old value is not an input of the new one
+        p.in_place = false; // a derive replaces its field: the old value of its own cell is not an input
     }
     return m;
 }
 
 std::string to_json(const Module& m, const Law& law)
 {
-    std::ostringstream out;
-    out << "{\"version\":1,\"fields\":[";
-    for (std::size_t f = 0; f < law.fields.size(); ++f) {
-        const FieldDecl& d = law.fields[f];
-        out << (f ? "," : "") << "{\"name\":";
-        json_string(out, d.name);
-        out << ",\"init\":";
-        json_number(out, d.init);
-        out << ",\"lo\":";
-        json_number(out, d.lo);
-        out << ",\"hi\":";
-        json_number(out, d.hi);
-        out << ",\"clamp\":" << (d.clamp ? "true" : "false") << ",\"static\":" << (d.is_static ? "true" : "false")
-            << ",\"derived\":" << (d.derived ? "true" : "false") << '}';
+    JsonText j;
+    j.open('{').key("version").integer(1).key("fields").open('[');
+    for (const FieldDecl& f : law.fields) {
+        j.open('{').key("name").str(f.name).key("init").num(f.init).key("lo").num(f.lo).key("hi").num(f.hi);
+        j.key("clamp").boolean(f.clamp).key("static").boolean(f.is_static).key("derived").boolean(f.derived).close('}');
     }
-    out << "],\"params\":[";
+    j.close(']').key("params").open('[');
     for (std::size_t p = 0; p < law.param_names.size(); ++p) {
-        out << (p ? "," : "") << "{\"name\":";
-        json_string(out, law.param_names[p]);
-        out << ",\"value\":";
-        json_number(out, law.param_values[p]);
-        out << '}';
+        j.open('{').key("name").str(law.param_names[p]).key("value").num(law.param_values[p]).close('}');
     }
-    out << "],\"nets\":[";
-    for (std::size_t n = 0; n < law.nets.size(); ++n) {
-        const NetDecl& net = law.nets[n];
-        out << (n ? "," : "") << "{\"name\":";
-        json_string(out, net.name);
-        out << ",\"file\":";
-        json_string(out, net.file);
-        out << ",\"sizes\":";
-        json_ints(out, net.sizes);
-        out << ",\"act\":\"" << to_string(net.act) << "\",\"half\":" << (net.half ? "true" : "false")
-            << ",\"loaded\":" << (net.loaded ? "true" : "false") << '}';
+    j.close(']').key("nets").open('[');
+    for (const NetDecl& n : law.nets) {
+        j.open('{').key("name").str(n.name).key("file").str(n.file).key("sizes").ints(n.sizes);
+        j.key("act").str(to_string(n.act)).key("half").boolean(n.half).key("loaded").boolean(n.loaded).close('}');
     }
-    out << "],\"insts\":[";
+    j.close(']').key("insts").open('[');
     for (std::size_t i = 0; i < m.insts.size(); ++i) {
         const Inst& in = m.insts[i];
-        out << (i ? ",\n" : "\n") << "{\"op\":\"" << op_name(in.op) << "\",\"shape\":\""
-            << (in.shape == Shape::Cell ? "cell" : "scalar") << "\",\"number\":";
-        json_number(out, in.number);
-        out << ",\"index\":" << in.index << ",\"iterations\":" << in.iterations << ",\"offset\":[" << in.offset[0] << ','
-            << in.offset[1] << ',' << in.offset[2] << "],\"args\":";
-        json_ints(out, in.args);
-        out << ",\"radius\":" << in.radius << ",\"stage\":" << (m.is_stage(static_cast<int>(i)) ? "true" : "false") << '}';
+        j.line().open('{').key("op").str(op_name(in.op)).key("shape").str(in.shape == Shape::Cell ? "cell" : "scalar");
+        j.key("number").num(in.number).key("index").integer(in.index).key("iterations").integer(in.iterations);
+        j.key("offset").open('[').integer(in.offset[0]).integer(in.offset[1]).integer(in.offset[2]).close(']');
+        j.key("args").ints(in.args).key("radius").integer(in.radius);
+        j.key("stage").boolean(m.is_stage(static_cast<int>(i))).close('}');
     }
-    out << "],\n\"terms\":[";
-    for (std::size_t t = 0; t < m.terms.size(); ++t) {
-        out << (t ? ",\n" : "\n");
-        json_program(out, m.terms[t]);
+    j.close(']');
+    const std::pair<const char*, const std::vector<Program>*> groups[] = {
+        {"terms", &m.terms}, {"learned", &m.learned}, {"derives", &m.derives}};
+    for (const auto& [name, programs] : groups) {
+        j.key(name, true).open('[');
+        for (const Program& p : *programs) {
+            j.line();
+            program_json(j, p);
+        }
+        j.close(']');
     }
-    out << "],\n\"learned\":[";
-    for (std::size_t t = 0; t < m.learned.size(); ++t) {
-        out << (t ? ",\n" : "\n");
-        json_program(out, m.learned[t]);
-    }
-    out << "],\n\"derives\":[";
-    for (std::size_t d = 0; d < m.derives.size(); ++d) {
-        out << (d ? ",\n" : "\n");
-        json_program(out, m.derives[d]);
-    }
-    out << "]}\n";
-    return out.str();
+    j.close('}');
+    return j.str() + "\n";
 }
 
 std::string describe(const Module& m, const Law& law)
 {
     std::ostringstream out;
-    out << m.insts.size() << " instructions (" << m.shared << " found twice and kept once, " << m.folded << " folded away)\n";
+    out << "IR: " << m.insts.size() << " instructions; " << m.shared << " repeated values merged, " << m.folded
+        << " folded or simplified away\n";
     for (std::size_t i = 0; i < m.insts.size(); ++i) {
         const Inst& in = m.insts[i];
-        out << "  %" << i << " = " << op_name(in.op);
-        if (in.op == Op::Const) {
-            out << ' ' << in.number;
-        } else if (in.op == Op::Param) {
-            out << ' ' << law.param_names[static_cast<std::size_t>(in.index)];
-        } else if (in.op == Op::Field) {
-            out << ' ' << law.fields[static_cast<std::size_t>(in.index)].name;
-        } else if (in.op == Op::Tridiag) {
-            out << ' ' << "xyz"[in.index];
-        } else if (in.net) {
-            out << ' ' << law.nets[static_cast<std::size_t>(in.index)].name;
+        out << "  %" << i << " := " << op_name(in.op);
+        switch (in.op) {
+        case Op::Const: out << '(' << in.number << ')'; break;
+        case Op::Field: out << '(' << law.fields[static_cast<std::size_t>(in.index)].name << ')'; break;
+        case Op::Param: out << '(' << law.param_names[static_cast<std::size_t>(in.index)] << ')'; break;
+        case Op::Tridiag: out << '_' << "xyz"[in.index]; break;
+        default:
+            if (in.net) {
+                out << '[' << law.nets[static_cast<std::size_t>(in.index)].name << ']';
+            }
+            break;
         }
-        for (const int a : in.args) {
-            out << " %" << a;
+        for (std::size_t a = 0; a < in.args.size(); ++a) {
+            out << (a == 0 ? " " : ", ") << '%' << in.args[a];
         }
         if (in.op == Op::Shift || in.op == Op::ShiftT) {
-            out << " (" << in.offset[0] << ", " << in.offset[1] << ", " << in.offset[2] << ")";
+            out << " by (" << in.offset[0] << ", " << in.offset[1] << ", " << in.offset[2] << ')';
         }
-        out << "   [" << (in.shape == Shape::Cell ? "cell" : "scalar");
+        out << "    ; " << (in.shape == Shape::Cell ? "per cell" : "one number");
         if (in.radius > 0) {
-            out << ", reads " << in.radius << " away";
+            out << ", radius " << in.radius;
         }
         if (m.is_stage(static_cast<int>(i))) {
-            out << ", stage";
+            out << ", " << (in.global ? "whole-lattice stage" : in.net ? "net stage" : "stored stage");
         }
-        out << ", cost " << in.cost << "]\n";
+        out << ", cost " << in.cost << '\n';
     }
-    auto program = [&](const Program& p) {
+    const auto show = [&](const Program& p) {
         if (p.equations.empty()) {
             return;
         }
-        out << p.name << ": " << p.equations.size() << " equation(s), " << p.stages.size() << " stage(s), "
-            << (p.in_place ? "one kernel (in place)" : "one kernel (ping-pong buffers)") << "\n";
-        for (const Equation& e : p.equations) {
-            out << "    d " << law.fields[static_cast<std::size_t>(e.target)].name << " = %" << e.value << "\n";
+        out << p.name << " (" << (p.in_place ? "in place" : "ping-pong") << ", " << p.stages.size() << " stage(s))\n";
+        for (const int s : p.stages) {
+            out << "    first %" << s << " (" << op_name(m.insts[static_cast<std::size_t>(s)].op) << ")\n";
+        }
+        for (const Equation& eq : p.equations) {
+            out << "    " << law.fields[static_cast<std::size_t>(eq.target)].name << " += dt * %" << eq.value << '\n';
         }
         if (p.max_dt >= 0) {
-            out << "    max_dt = %" << p.max_dt << "\n";
+            out << "    step at most %" << p.max_dt << '\n';
         }
     };
-    for (const Program& p : m.terms) {
-        program(p);
-    }
-    for (const Program& p : m.learned) {
-        program(p);
-    }
-    for (const Program& p : m.derives) {
-        program(p);
+    for (const auto* group : {&m.terms, &m.learned, &m.derives}) {
+        for (const Program& p : *group) {
+            show(p);
+        }
     }
     return out.str();
 }

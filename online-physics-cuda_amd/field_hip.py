@@ -73,6 +73,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_law_term_count": (c_int, [c_void_p]),
         "fe_law_ir_json": (c_void_p, [c_void_p]),
         "fe_law_describe": (c_void_p, [c_void_p]),
+        "fe_law_text": (c_void_p, [c_void_p]),
         "fe_law_hip_source": (c_void_p, [c_void_p, ctypes.c_char_p, c_size_t]),
         "fe_string_free": (None, [c_void_p]),
         "fe_solver_create": (c_void_p, [c_void_p, c_int, c_int, c_int, c_double, ctypes.c_char_p, c_size_t]),
@@ -333,6 +334,13 @@ class Law:
 
     def describe(self) -> str:
         return _take_string(lib().fe_law_describe(self.ptr))
+
+    def text(self) -> str:
+        """The law written back as law text (compiles to the same IR)."""
+        out = _take_string(lib().fe_law_text(self.ptr))
+        if out is None:
+            raise FieldHipError("this law cannot be written as law text")
+        return out
 
     def hip_source(self) -> str:
         why = ctypes.create_string_buffer(4096)

@@ -133,3 +133,45 @@ def test_every_op_has_a_form_on_the_card():
     src = law.hip_source()
     for kernel in ("_mark", "_scan", "_signed", "_sweep", "_copy"):
         assert kernel + "(K k" in src
+
+
+def _round_trip(field_hip, law):
+    text = law.text()
+    again = field_hip.Law(text)
+    assert again.ir_json() == law.ir_json(), text
+    assert again.text() == text  # printing is a fixed point
+
+
+@pytest.mark.parametrize("name", golden_io.names())
+def test_law_text_compiles_back_to_the_same_ir(name, tmp_path):
+    # Law -> text -> Law gives the same IR on every golden law (nets read from the same weight files).
+    import field_hip
+
+    meta, _, _ = golden_io.load(name)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        os.makedirs(os.path.join("assets", "nets"))
+        for net, text in meta.get("nets", {}).items():
+            with open(os.path.join("assets", "nets", net + ".net"), "w") as f:
+                f.write(text)
+        _round_trip(field_hip, field_hip.Law(meta["law"]))
+    finally:
+        os.chdir(cwd)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="the reference's laws are only in the build container")
+@pytest.mark.parametrize("paths", reference_laws(), ids=lambda p: "+".join(os.path.basename(q) for q in p))
+def test_shipped_law_text_compiles_back_to_the_same_ir(tmp_path, paths):
+    import field_hip
+
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        try:
+            law = field_hip.Law.from_files(*paths)
+        except field_hip.FieldHipError:
+            pytest.skip("refused (checked by test_ir_json_is_the_references)")
+        _round_trip(field_hip, law)
+    finally:
+        os.chdir(cwd)
