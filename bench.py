@@ -14,12 +14,10 @@ OpenMP CPU engine on this host's cores on a bounded sample. Prints ONE JSON line
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
-import shutil
-import subprocess
 import sys
-import tempfile
 import time
 
 import numpy as np
@@ -245,7 +243,7 @@ def sine_start(n: int, noise: float, seed: int = 5):
     return t.astype(np.float32).reshape(-1), nb.astype(np.float32).reshape(-1)
 
 
-def dirichlet_heat(cpu: bool):
+def dirichlet_heat():
     """Zero-Dirichlet 3-D conduction (k = 0.1, dt = 0.4 h^2/k, substeps from max_dt = 0.15 h^2/k): the error against the
     decaying sine mode exp(-3 pi^2 k t) prod sin(pi x) and its order under refinement 16 -> 32 -> 64 at a fixed end
     time (200 steps of 64^3), and the 64^3 run with noise: a fixed-order compensated total every step."""
@@ -284,26 +282,27 @@ def dirichlet_heat(cpu: bool):
     out = {"errors_max_vs_exact": errors, "orders": orders, "substeps_per_step": s.last_substeps(),
            "steps": 200, "cells": n ** 3, "ms_per_step_with_total": ms / 200,
            "total_first_last": [totals[0], totals[-1]], "totals_decreasing": bool(np.all(np.diff(totals) < 0))}
-    if cpu:
-        final = s.get_field("T")
-        ref = cpu_baseline_lawc(DIRICHLET_LAW, (n, n, n, h), {"T": t0, "nb": nb}, 200, dt, {}, "T")
-        out["equals_reference_cpu_bits"] = bool(np.array_equal(final.view(np.uint32), ref["final"].view(np.uint32)))
-        out["cpu_baseline"] = ref["baseline"]
+    out.update(reference_check("dirichlet_heat_64", s.get_field("T")))
     return out
 
 
-def black_scholes(steps: int, cpu: bool):
-    """The Black-Scholes law in x = ln S on 4096 cells x 512 planes (every plane an option), sig 0.2, r 0.05, put K =
-    100 averaged over each cell, one substep a step at dt = max_dt = 0.4 h^2 / sig^2."""
-    nx, planes = 4096, 512
+def black_scholes_start(nx: int = 4096, planes: int = 512):
+    """x = ln S on ln 100 +- 4 (h = 8 / nx), the put payoff with strike 100 averaged over each cell, the same line in
+    every plane; returns (V0, h, dt) with dt = max_dt = 0.4 h^2 / sig^2 at sig = 0.2."""
     h = 8.0 / nx
     x0 = np.log(100.0) - 4.0
     edges = x0 + h * np.arange(nx + 1)
     lo, hi = edges[:-1], edges[1:]
     a = np.minimum(hi, np.log(100.0))
     pay = np.where(a > lo, (100.0 * (a - lo) - (np.exp(a) - np.exp(lo))) / h, 0.0)
-    v0 = np.tile(pay, planes).astype(np.float32)
-    dt = 0.4 * h * h / 0.04
+    return np.tile(pay, planes).astype(np.float32), h, 0.4 * h * h / 0.04
+
+
+def black_scholes(steps: int):
+    """The Black-Scholes law in x = ln S on 4096 cells x 512 planes (every plane an option), sig 0.2, r 0.05, put K =
+    100 averaged over each cell, one substep a step at dt = max_dt = 0.4 h^2 / sig^2."""
+    nx, planes = 4096, 512
+    v0, h, dt = black_scholes_start(nx, planes)
     s = field_hip.Solver(field_hip.Law(BLACK_SCHOLES_LAW), (nx, 1, planes), h)
     s.set_field("V", v0)
     s.set_lazy_readback(True)
@@ -311,57 +310,35 @@ def black_scholes(steps: int, cpu: bool):
     for _ in range(10):
         s.step(dt)
     mid = s.get_field("V")
+    # Timed as a caller runs it: Solver.run (captured graphs of whole steps), after the same run once to capture them.
+    s.run(dt, steps)
+    field_hip.sync()
     ev = field_hip.Events()
     ev.begin()
-    for _ in range(steps):
-        s.step(dt)
+    s.run(dt, steps)
     ms = ev.end() / steps
     cells = nx * planes
     out = {"cells": cells, "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "gbs_8B": 8.0 * cells / (ms * 1e6),
            "kernel": s.profile()["terms"][0]["kernel"]}
-    if cpu:
-        ref = cpu_baseline_lawc(BLACK_SCHOLES_LAW, (nx, 1, planes, h), {"V": v0}, 10, dt, {}, "V")
-        rel = np.max(np.abs(mid.astype(np.float64) - ref["final"]) / np.maximum(np.abs(ref["final"]), 1e-30))
-        out["max_rel_vs_reference_cpu_10_steps"] = float(rel)
-        out["equals_reference_cpu_bits"] = bool(np.array_equal(mid.view(np.uint32), ref["final"].view(np.uint32)))
-        out["cpu_baseline"] = ref["baseline"]
+    out.update(reference_check("black_scholes_4096x512", mid))
     return out
 
 
-def cpu_baseline_lawc(law: str, grid, init: dict, steps: int, dt: float, params: dict, field: str):
-    """The reference's own CPU engine (oracle/_ref/lawc, built from its sources) on the same law, start and steps, on
-    this host's cores: its final field (what the card's is compared with) and its time a step."""
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    lawc = os.path.join(ROOT, "oracle", "_ref", "lawc")
-    if not os.path.exists(lawc):
-        return {"final": np.zeros(1, np.float32), "baseline": None}
-    nx, ny, nz, h = grid
-    tmp = tempfile.mkdtemp()
-    try:
-        path = os.path.join(tmp, "law.law")
-        open(path, "w").write(law)
-        init_dir = os.path.join(tmp, "init")
-        os.makedirs(init_dir)
-        for name, values in init.items():
-            np.asarray(values, np.float32).tofile(os.path.join(init_dir, name + ".bin"))
-        out_dir, js = os.path.join(tmp, "out"), os.path.join(tmp, "t.json")
-        cmd = [lawc, "--grid", str(nx), str(ny), str(nz), repr(float(h)), "--init", init_dir, "--steps", str(steps), "--dt",
-               repr(float(dt)), "--dump", out_dir, "--json", js, "--warmup", "1"]
-        for name, value in params.items():
-            cmd += ["--param", name, repr(float(value))]
-        cmd.append(path)
-        t = time.time()
-        subprocess.run(cmd, check=True, capture_output=True, env=dict(os.environ, OMP_NUM_THREADS=str(cores)), timeout=600)
-        wall = time.time() - t
-        final = np.fromfile(os.path.join(out_dir, f"step_{steps}", field + ".bin"), dtype=np.float32)
-        ms = json.load(open(js))["median_ms"]
-        cells = nx * ny * nz
-        return {"final": final,
-                "baseline": {"value": cells / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "reference",
-                             "sample": f"reference lawc, OpenMP {cores} threads, {nx}x{ny}x{nz}, {steps} steps, median "
-                                       f"{ms:.3f} ms/step ({wall:.1f} s wall)"}}
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+def reference_check(name: str, final: np.ndarray) -> dict:
+    """The card's field against the reference CPU engine's on the same law, start and steps: tests/golden/
+    bench_reference.json holds the SHA-256 of the reference's final field bits and its time a step, both taken in the
+    build container by tests/golden/make_bench_reference.py (the reference's engine does not run on this host)."""
+    path = os.path.join(ROOT, "tests", "golden", "bench_reference.json")
+    if not os.path.exists(path):
+        return {}
+    rec = json.load(open(path)).get(name)
+    if rec is None:
+        return {}
+    digest = hashlib.sha256(np.ascontiguousarray(final, np.float32).tobytes()).hexdigest()
+    cells = int(np.asarray(final).size)
+    return {"equals_reference_cpu_bits": digest == rec["sha256"],
+            "cpu_baseline": {"value": cells / (rec["median_ms"] * 1e-3), "unit": "cell-updates/s", "cores": rec["cores"],
+                             "kind": "reference", "sample": rec["sample"]}}
 
 
 def block_mean(a, n: int, count: int, f: int) -> np.ndarray:
@@ -511,34 +488,11 @@ def measured_traffic(kernel: str, n: int):
 
 
 def cpu_baseline(n: int, budget_s: float):
-    """The reference's own CPU engine (oracle/_ref/lawc, built from its sources) on this host, a bounded sample of the
-    same step; the C restatement (oracle/) when the reference binary is absent."""
+    """The oracle's C restatement of the reference CPU path (OpenMP over this host's cores) on the same step, a bounded
+    sample: substeps of the N^3 lattice until about budget_s seconds have passed."""
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    lawc = os.path.join(ROOT, "oracle", "_ref", "lawc")
     h = 1.0 / n
     dt = 0.4 * h * h / 0.1 / 3
-    if os.path.exists(lawc):
-        tmp = tempfile.mkdtemp()
-        try:
-            law = os.path.join(tmp, "heat.law")
-            open(law, "w").write(HEAT_LAW)
-            scene = os.path.join(tmp, "s.scene")
-            open(scene, "w").write(f"grid {n} {n} {n} {h!r}\nnoise T 1.0 7\n")
-            out = os.path.join(tmp, "t.json")
-            steps = 5
-            env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-            t = time.time()
-            subprocess.run([lawc, "--scene", scene, "--steps", str(steps + 1), "--warmup", "1", "--dt", repr(dt), "--json",
-                            out, law], check=True, capture_output=True, env=env, timeout=max(60, 10 * budget_s))
-            wall = time.time() - t
-            res = json.load(open(out))
-            ms = res["median_ms"]
-            return {"value": n ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "reference",
-                    "sample": f"reference lawc (its CPU engine, OpenMP {cores} threads), d T = diffuse(k,T) with its "
-                              f"contracts, {n}^3, {steps} timed steps of one substep after 1 warm-up, median {ms:.1f} "
-                              f"ms/step ({wall:.1f} s wall)"}
-        finally:
-            shutil.rmtree(tmp, ignore_errors=True)
     from oracle import heat
     t0 = np.random.default_rng(7).random(n ** 3, dtype=np.float32)
     times, cur, t_start = [], t0, time.time()
@@ -588,8 +542,8 @@ def main() -> int:
         log(f"heston adi: {json.dumps(adi)}")
     extra = {}
     if not args.no_extra:
-        for name, fn in (("dirichlet_heat_64", lambda: dirichlet_heat(not args.no_cpu)),
-                         ("black_scholes_4096x512", lambda: black_scholes(200, not args.no_cpu)),
+        for name, fn in (("dirichlet_heat_64", dirichlet_heat),
+                         ("black_scholes_4096x512", lambda: black_scholes(200)),
                          ("learned_coarse", lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4)),
                          ("adjoint_transport", lambda: adjoint_transport(args.adjoint_size, not args.no_cpu))):
             t = time.time()

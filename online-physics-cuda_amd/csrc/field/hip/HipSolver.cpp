@@ -99,6 +99,7 @@ struct Impl final : HipBackend {
 
     // Device state.
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
+    std::vector<std::unique_ptr<Buffer>> guard_copy; // a guarded term's fields before its learned scheme ran
     std::vector<void*> now, nxt;
     Buffer P, U, A, S, L, C, N, W;
     // The contract records, one allocation read back in one copy: range counts (two u32 a field), the first bad
@@ -1018,23 +1019,29 @@ struct Impl final : HipBackend {
         }
     }
 
-    void snapshot(const std::vector<int>& fields, std::vector<std::unique_ptr<Buffer>>& saved)
+    // A guarded term's written fields are copied aside before its learned scheme runs, into one lattice a field that
+    // is allocated the first time that field is guarded and kept for the solver's life (no allocation on a step).
+    void snapshot(const std::vector<int>& fields)
     {
         const std::size_t bytes = shape.cells() * 4;
-        saved.clear();
+        guard_copy.resize(now.size());
         for (const int f : fields) {
-            saved.push_back(std::make_unique<Buffer>());
-            saved.back()->allocate(bytes);
-            check(fe_hip_memcpy_dtod_async(saved.back()->p, now[static_cast<std::size_t>(f)], bytes), "snapshot");
+            std::unique_ptr<Buffer>& copy = guard_copy[static_cast<std::size_t>(f)];
+            if (!copy) {
+                copy = std::make_unique<Buffer>();
+                copy->allocate(bytes);
+            }
+            check(fe_hip_memcpy_dtod_async(copy->p, now[static_cast<std::size_t>(f)], bytes), "snapshot");
         }
     }
 
-    void restore(const std::vector<int>& fields, std::vector<std::unique_ptr<Buffer>>& saved)
+    void restore(const std::vector<int>& fields)
     {
         const std::size_t bytes = shape.cells() * 4;
-        for (std::size_t i = 0; i < fields.size(); ++i) {
-            touched(fields[i]);
-            check(fe_hip_memcpy_dtod_async(now[static_cast<std::size_t>(fields[i])], saved[i]->p, bytes), "restore");
+        for (const int f : fields) {
+            touched(f);
+            check(fe_hip_memcpy_dtod_async(now[static_cast<std::size_t>(f)], guard_copy[static_cast<std::size_t>(f)]->p, bytes),
+                "restore");
         }
     }
 
@@ -1342,8 +1349,7 @@ struct Impl final : HipBackend {
             if (guarded) {
                 // Snapshot what the term writes, run the learned scheme and judge it now; a breach is undone and the
                 // classical scheme runs (src/field/cuda/CudaSolver.cpp:673-705 is the reference's guard).
-                std::vector<std::unique_ptr<Buffer>> saved;
-                snapshot(module.terms[t].writes, saved);
+                snapshot(module.terms[t].writes);
                 std::vector<Judge> own;
                 int slot = 0;
                 reset_records();
@@ -1357,7 +1363,7 @@ struct Impl final : HipBackend {
                     substeps[t] = n;
                     done = true;
                 } else {
-                    restore(module.terms[t].writes, saved);
+                    restore(module.terms[t].writes);
                     tp.fell_back = true;
                     ++prof.fallbacks;
                     check(fe_hip_stream_sync(), "sync");
