@@ -40,6 +40,14 @@ int fe_lnet_faces(fe_lnet* n, const float* u, const float* v, int nx, int ny, in
  * (counted: fe_lnet_fallbacks). */
 int fe_lnet_step(fe_lnet* n, const float* u, const float* v, int nx, int ny, int count, double h, double du, double dv, double k,
     double dt, int learned, double bound, float* u2, float* v2);
+/* fe_lnet_step with learned = 1, allocating nothing: no counterpart in the reference; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). The face fluxes and the breach counter live in a
+ * static device array of the library (16 MiB, one per device, held until the process ends) instead of the net's
+ * lazily grown buffers, so count * nx * ny may be at most 2^20 cells: larger batches are refused with an error (use
+ * fe_lnet_step). Every call on one device shares that array: calls that overlap on one device (other streams, other
+ * threads) race. Same kernels, same bits as fe_lnet_step; fallbacks are counted in the net as there. */
+int fe_lnet_step_noalloc(fe_lnet* n, const float* u, const float* v, int nx, int ny, int count, double h, double du, double dv,
+    double k, double dt, double bound, float* u2, float* v2);
 int fe_lnet_fallbacks(fe_lnet* n, unsigned* out);
 /* The loss's gradient for the pair (s0 -> s1 over dt) into the net's gradient buffer (and grad_out, loss_out when not
  * NULL: then it waits for the card). The loss is `weight` times the mean squared miss. Fixed-order sums: the same bits

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -688,6 +689,50 @@ unsigned blocks_for(long long n, int threads) { return static_cast<unsigned>((n 
 
 } // namespace
 
+// The scratch of fe_lnet_step_noalloc, one copy on every device the library runs on, held until the process ends: the
+// face fluxes of up to kNoAllocCells cells (two faces a cell, two fields a face) and, after them, the step's breach
+// counter. Host code never passes `scratch` itself (on the host it names a stand-in, not device memory): it passes the
+// address hipGetSymbolAddress gives for the current device, kept per device in noalloc_scratch.
+constexpr long long kNoAllocCells = 1LL << 20;
+constexpr long long kNoAllocFloats = 4 * kNoAllocCells + 64; // faces, then 256 bytes holding the breach counter
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mutex;
+float* noalloc_scratch[kMaxDevices] = {};
+
+// Resolves the current device's copy of `scratch` once (at fe_lnet_create, with the device current); 0 or a reason.
+const char* resolve_noalloc_scratch()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        return "no current HIP device for the no-allocation scratch";
+    }
+    const std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_scratch[dev] == nullptr) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) {
+            return "hipGetSymbolAddress(scratch) failed";
+        }
+        noalloc_scratch[dev] = static_cast<float*>(ptr);
+    }
+    return nullptr;
+}
+
+float* noalloc_scratch_here()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        return nullptr;
+    }
+    const std::lock_guard<std::mutex> lock(noalloc_mutex);
+    return noalloc_scratch[dev];
+}
+
+} // namespace
+
 struct fe_lnet {
     Dev<float> params, grad, m, v, partials, loss_parts, loss, F, E;
     Dev<bf16x8> frags;
@@ -747,6 +792,9 @@ fe_lnet* fe_lnet_create(const float* params, float scale, char* why, size_t why_
         hcheck(hipMemset(n->flags.p, 0, sizeof(unsigned) * 2), "memset");
         hipLaunchKernelGGL(k_pack, dim3(kFrags), dim3(64), 0, n->stream(), n->params.p, n->frags.p);
         hcheck(hipStreamSynchronize(n->stream()), "sync");
+        if (const char* bad = resolve_noalloc_scratch()) {
+            throw std::runtime_error(bad);
+        }
         return n.release();
     } catch (const std::exception& e) {
         if (why != nullptr && why_bytes > 0) {
@@ -832,6 +880,41 @@ int fe_lnet_step(fe_lnet* n, const float* u, const float* v, int nx, int ny, int
                 static_cast<float>(bound), u2, v2, nullptr);
         }
         hcheck(hipGetLastError(), "step");
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(n, e.what());
+    }
+}
+
+int fe_lnet_step_noalloc(fe_lnet* n, const float* u, const float* v, int nx, int ny, int count, double h, double du, double dv,
+    double k, double dt, double bound, float* u2, float* v2)
+{
+    if (n == nullptr || u == nullptr || v == nullptr || u2 == nullptr || v2 == nullptr || nx < 2 || ny < 2 || count < 1) {
+        return fail(n, "invalid arguments");
+    }
+    const long long cells = static_cast<long long>(nx) * ny * count;
+    if (cells > kNoAllocCells) {
+        return fail(n, "fe_lnet_step_noalloc: " + std::to_string(cells) + " cells, above its scratch of " +
+            std::to_string(kNoAllocCells) + " cells (use fe_lnet_step)");
+    }
+    float* faces = noalloc_scratch_here();
+    if (faces == nullptr) {
+        return fail(n, "fe_lnet_step_noalloc: the scratch was not resolved on this device (fe_lnet_create does it)");
+    }
+    unsigned* breach = reinterpret_cast<unsigned*>(faces + 4 * kNoAllocCells);
+    try {
+        const Lattice L = lattice(nx, ny, count, h);
+        const Reaction R{static_cast<float>(du), static_cast<float>(dv), static_cast<float>(k)};
+        const hipStream_t s = static_cast<hipStream_t>(fe_hip_stream());
+        // The kernels of fe_lnet_step's learned path, with the faces and the breach counter in the scratch: the counter
+        // is cleared by a kernel (no memset node), the fallback count stays the net's own.
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, breach, 1);
+        hipLaunchKernelGGL(k_faces, dim3(fe_lnet::kFaceBlocks), dim3(256), 0, s, n->frags.p, n->params.p, u, v, L, n->scale, 2 * cells, faces);
+        hipLaunchKernelGGL(k_step, dim3(blocks_for(cells, 256)), dim3(256), 0, s, u, v, faces, L, R, static_cast<float>(dt), 1,
+            static_cast<float>(bound), u2, v2, breach);
+        hipLaunchKernelGGL(k_fallback, dim3(blocks_for(cells, 256)), dim3(256), 0, s, u, v, L, R, static_cast<float>(dt), breach, u2, v2,
+            n->flags.p + 1);
+        hcheck(hipGetLastError(), "step_noalloc");
         return 0;
     } catch (const std::exception& e) {
         return fail(n, e.what());

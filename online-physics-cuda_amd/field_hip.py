@@ -129,6 +129,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_lnet_faces": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p]),
         "fe_lnet_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_double,
                                  c_int, c_double, c_void_p, c_void_p]),
+        "fe_lnet_step_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double, c_double,
+                                         c_double, c_double, c_void_p, c_void_p]),
         "fe_lnet_fallbacks": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint)]),
         "fe_lnet_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
@@ -623,6 +625,11 @@ class LearnedNet:
     def step(self, u, v, nx, ny, count, h, du, dv, k, dt, u2, v2, learned=True, bound=1e3) -> None:
         self._check(lib().fe_lnet_step(self.ptr, self._p(u), self._p(v), nx, ny, count, h, du, dv, k, dt, int(learned), bound,
                                        self._p(u2), self._p(v2)), "step")
+
+    def step_noalloc(self, u, v, nx, ny, count, h, du, dv, k, dt, u2, v2, bound=1e3) -> None:
+        """The learned step allocating nothing (its scratch is the library's static device array; at most 2^20 cells)."""
+        self._check(lib().fe_lnet_step_noalloc(self.ptr, self._p(u), self._p(v), nx, ny, count, h, du, dv, k, dt, bound,
+                                               self._p(u2), self._p(v2)), "step_noalloc")
 
     def fallbacks(self) -> int:
         c = ctypes.c_uint()
