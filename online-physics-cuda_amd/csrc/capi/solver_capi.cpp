@@ -244,7 +244,8 @@ char* fe_solver_profile_json(fe_solver* s)
         const std::vector<std::string>* kernels = backend != nullptr ? &backend->gather_kernels() : nullptr;
         json_string(out, kernels != nullptr && t < kernels->size() ? (*kernels)[t] : std::string());
         out << ",\"ms\":" << tp.ms << ",\"launches\":" << tp.launches << ",\"substeps\":" << tp.substeps
-            << ",\"learned\":" << (tp.learned ? "true" : "false") << ",\"fell_back\":" << (tp.fell_back ? "true" : "false") << '}';
+            << ",\"idle\":" << (tp.idle ? "true" : "false") << ",\"learned\":" << (tp.learned ? "true" : "false")
+            << ",\"fell_back\":" << (tp.fell_back ? "true" : "false") << '}';
     }
     out << "]}";
     return dup(out.str());

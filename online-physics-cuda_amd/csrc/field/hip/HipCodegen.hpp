@@ -66,6 +66,10 @@ struct WindowKernel {
     int vx = 1;
     int ty = 0;
     int block_y = 0;
+    // A conduction term's flux-carrying window (vx = 4; preferred where nx is a multiple of 4): 1 with waves that only
+    // load the halo rows, 2 with the halo rows loaded by the tile's edge rows; 0 the generic window.
+    int flux = 0;
+    int zchunk = 0; // planes a tile; 0: the program's
 };
 
 struct ProgramCode {

@@ -2,6 +2,7 @@
 #include "field/hip/HipSolver.hpp"
 
 #include "fe_hip.h"
+#include "field/LawBounds.hpp"
 #include "field/hip/HipCodegen.hpp"
 #include "field/ir/LawIR.hpp"
 #include "kernels/net_mfma.hpp"
@@ -69,6 +70,7 @@ constexpr unsigned kBlockX = 64;
 constexpr unsigned kBlockY = 4;
 constexpr int kReduceBlocks = 1024; // blocks of a reduction's first pass (its partials fold in one block)
 constexpr int kTotals = 1024;      // doubles of conserved totals judged at once
+constexpr int kRanges = 1024;      // range checks judged at once (a record each: two counts and the first bad cell)
 constexpr int kMaxWindowRegisters = 168; // a lane's registers that still leave a window gather three waves a SIMD
 
 struct Impl final : HipBackend {
@@ -88,10 +90,19 @@ struct Impl final : HipBackend {
     long long step_count = 0;
     bool lazy = false;
     std::vector<char> stale; // per field: the card holds newer cells than the host
+    // What is known of each field's range, for skipping terms with nothing to do (the reference's rule: a term whose
+    // every rate is exactly 0 is not dispatched, src/field/cuda/CudaSolver.cpp:586-595). `known`: at the start of the
+    // next step, measured on the card (k_bounds) or carried; `flowing`: carried through the terms of the step in
+    // progress (flow_bounds: as wide as the cells or wider, so a term with work is never skipped). Measured after
+    // every eager step, and every kTightenEvery lazy steps; carried otherwise.
+    std::vector<Bounds> known, flowing;
+    long long lazy_steps = 0;
+    static constexpr int kTightenEvery = 8;
     // Launch-shape overrides, read once from the environment at construction (for measuring shapes against each
     // other): FIELD_HIP_TILE="<cells a lane>x<rows>", FIELD_HIP_ZCHUNK=<planes a tile>, FIELD_HIP_DIRECT=1,
     // FIELD_HIP_REMAP=0|1 (tiles in XCD runs), FIELD_HIP_STREAM=0|1 (streaming stores).
     int tile_vx = 0, tile_ty = 0, zchunk_override = 0, remap_override = -1, stream_override = -1;
+    int tile_flux = 0; // FIELD_HIP_TILE named a flux-carrying window: 1 "f4x..", 2 "h4x.."
     bool force_direct = false;
     std::string last_gather;
     std::vector<std::string> gathers; // per term: the gather kernel its last substep launched
@@ -102,6 +113,7 @@ struct Impl final : HipBackend {
     std::vector<std::unique_ptr<Buffer>> guard_copy; // a guarded term's fields before its learned scheme ran
     std::vector<void*> now, nxt;
     Buffer P, U, A, S, L, C, N, W;
+    Buffer bounds_keys; // 2 u32 a field: the least and greatest order key (k_bounds)
     // The contract records, one allocation read back in one copy: range counts (two u32 a field), the first bad
     // cell (a u64 a field), then the conserved totals (kTotals doubles); `judged` the pinned host copy.
     struct View {
@@ -227,7 +239,7 @@ struct Impl final : HipBackend {
             launch_cells(code.gather, dt);
             return;
         }
-        int zchunk = zchunk_override > 0 ? zchunk_override : code.zchunk;
+        int zchunk = zchunk_override > 0 ? zchunk_override : (w->zchunk > 0 ? w->zchunk : code.zchunk);
         int tx = (shape.nx + 64 * w->vx - 1) / (64 * w->vx);
         int ty = (shape.ny + w->ty - 1) / w->ty;
         int tz = (shape.nz + zchunk - 1) / zchunk;
@@ -256,24 +268,32 @@ struct Impl final : HipBackend {
         return 8.0 * static_cast<double>(shape.nx) * shape.ny * shape.nz * layout.fields <= 256.0 * 1024 * 1024;
     }
 
-    // The window gather for this lattice: four cells a lane when rows are whole float4s, with the tallest tile that
-    // still leaves at least 512 tiles (two per CU) to spread (measured on MI355X, conduction at z-chunks of 32: 256^3
+    // The window gather for this lattice. Where rows are whole float4s: a conduction term's flux-carrying window (the
+    // form with halo rows loaded by the edge rows while the fields fit the last-level cache, the form with halo waves
+    // past it), else four cells a lane; else a cell a lane. Among windows of one kind, the tallest tile that still
+    // leaves at least 512 tiles (two per CU) to spread (measured on MI355X, the generic window at z-chunks of 32: 256^3
     // 29.1 us with 256x4 tiles against 31.2 with 256x2 and 39.9 with 256x6; 512^3 219 us with 256x6 against 251 and
-    // 253); FIELD_HIP_TILE (e.g. "4x2", "1x6") picks one by its cells a lane and rows. None: the direct gather.
+    // 253). FIELD_HIP_TILE (e.g. "4x2", "1x6", "f4x4" or "h4x8" for the two flux-carrying forms) picks one. None: the
+    // direct gather.
     const hip::WindowKernel* window_for(const hip::ProgramCode& code) const
     {
         if (!code.window || code.windows.empty() || force_direct) {
             return nullptr;
         }
         const hip::WindowKernel* best = nullptr;
-        const int zc = zchunk_override > 0 ? zchunk_override : code.zchunk;
-        const long long tz = (shape.nz + zc - 1) / zc;
+        const bool cached = fits_llc();
+        auto rank = [&](const hip::WindowKernel& w) {
+            if (w.flux != 0) {
+                return (w.flux == 2) == cached ? 3 : 2;
+            }
+            return w.vx == 4 ? 1 : 0;
+        };
         for (const hip::WindowKernel& w : code.windows) {
             if (w.vx == 4 && shape.nx % 4 != 0) {
                 continue;
             }
             if (tile_vx > 0) {
-                if (w.vx == tile_vx && w.ty == tile_ty) {
+                if (w.vx == tile_vx && w.ty == tile_ty && w.flux == tile_flux) {
                     return &w;
                 }
                 continue;
@@ -281,8 +301,10 @@ struct Impl final : HipBackend {
             if (heavy.count(w.name) != 0) {
                 continue;
             }
+            const int zc = zchunk_override > 0 ? zchunk_override : (w.zchunk > 0 ? w.zchunk : code.zchunk);
+            const long long tz = (shape.nz + zc - 1) / zc;
             const long long tiles = 1LL * ((shape.nx + 64 * w.vx - 1) / (64 * w.vx)) * ((shape.ny + w.ty - 1) / w.ty) * tz;
-            if (best == nullptr || w.vx > best->vx || (w.vx == best->vx && w.ty > best->ty && tiles >= 512)) {
+            if (best == nullptr || rank(w) > rank(*best) || (rank(w) == rank(*best) && w.ty > best->ty && tiles >= 512)) {
                 best = &w;
             }
         }
@@ -292,8 +314,10 @@ struct Impl final : HipBackend {
     void read_overrides()
     {
         if (const char* t = std::getenv("FIELD_HIP_TILE")) {
-            if (std::sscanf(t, "%dx%d", &tile_vx, &tile_ty) != 2) {
+            tile_flux = t[0] == 'f' ? 1 : (t[0] == 'h' ? 2 : 0);
+            if (std::sscanf(t + (tile_flux != 0 ? 1 : 0), "%dx%d", &tile_vx, &tile_ty) != 2) {
                 tile_vx = tile_ty = 0;
+                tile_flux = 0;
             }
         }
         if (const char* z = std::getenv("FIELD_HIP_ZCHUNK")) {
@@ -394,6 +418,7 @@ struct Impl final : HipBackend {
         S.allocate(tridiag ? 2 * cells * 8 : 8);
         L.allocate(static_cast<std::size_t>(3 * kReduceBlocks) * 8);
         C.allocate(std::max<std::size_t>(1, nf) * 4);
+        bounds_keys.allocate(std::max<std::size_t>(1, 2 * nf) * 4);
         std::vector<std::size_t> net_offsets;
         net_host = hip::net_blob(*law, net_offsets);
         N.allocate(std::max<std::size_t>(1, net_host.size()) * 4);
@@ -428,7 +453,7 @@ struct Impl final : HipBackend {
         half_rows.allocate(row_bytes);
         half_outs.allocate(out_bytes);
         half_faces.allocate(faces ? cells * 3 * 4 : 0);
-        const std::size_t rec_bytes = std::max<std::size_t>(1, nf) * 8, first_bytes = std::max<std::size_t>(1, nf) * 8;
+        const std::size_t rec_bytes = static_cast<std::size_t>(kRanges) * 8, first_bytes = static_cast<std::size_t>(kRanges) * 8;
         contract.allocate(rec_bytes + first_bytes + static_cast<std::size_t>(kTotals) * 8);
         records.p = contract.p;
         firsts.p = static_cast<char*>(contract.p) + rec_bytes;
@@ -439,6 +464,7 @@ struct Impl final : HipBackend {
         }
         check(fe_hip_host_alloc(&judged, contract.bytes), "pinned allocation");
         stale.assign(nf, 0);
+        known.assign(nf, Bounds{});
     }
 
     // A trainer may write a law's net weights between steps (Solver::law_for_training, fe_solver_set_net_weights):
@@ -508,6 +534,7 @@ struct Impl final : HipBackend {
     void slide(FieldSet& fields, const FieldEdits& edits)
     {
         const std::size_t cells = shape.cells();
+        std::fill(known.begin(), known.end(), Bounds{}); // measured again after the step
         if (edits.shift[0] != 0 || edits.shift[1] != 0 || edits.shift[2] != 0) {
             if (spare.p == nullptr) {
                 spare.allocate(cells * 4);
@@ -577,13 +604,18 @@ struct Impl final : HipBackend {
             if (!dirty) {
                 continue;
             }
-            // The card only ever holds flushed floats: what the CPU reads of a denormal is its zero (DAZ).
+            // The card only ever holds flushed floats: what the CPU reads of a denormal is its zero (DAZ). The same pass
+            // takes the field's range (NaN skipped).
             std::vector<float>& v = fields.values(static_cast<int>(f));
+            float lo = std::numeric_limits<float>::infinity(), hi = -std::numeric_limits<float>::infinity();
             for (float& x : v) {
                 if (x != 0.0f && std::fabs(x) < 1.17549435e-38f) {
                     x = std::copysign(0.0f, x);
                 }
+                lo = x < lo ? x : lo;
+                hi = hi < x ? x : hi;
             }
+            known[f] = Bounds{lo, hi};
             check(fe_hip_memcpy_htod_async(now[f], v.data(), cells * 4), "field upload");
             prof.upload_bytes += cells * 4;
             stale[f] = 0;
@@ -780,10 +812,11 @@ struct Impl final : HipBackend {
         note_total(field, slot);
     }
 
-    // The conserved total (sum, sum of magnitudes) of a field into totals[slot], [slot + 1]; with `ranged` the same
-    // pass checks (and clamps) the field's range.
-    void total(int field, int slot, bool ranged = false)
+    // The conserved total (sum, sum of magnitudes) of a field into totals[slot], [slot + 1]; with a range record
+    // (`record` >= 0) the same pass checks (and clamps) the field's range into it.
+    void total(int field, int slot, int record = -1)
     {
+        const bool ranged = record >= 0;
         const long long cells = static_cast<long long>(shape.cells());
         long long chunk = std::max(1LL, (cells + kReduceBlocks - 1) / kReduceBlocks);
         const int blocks = static_cast<int>((cells + chunk - 1) / chunk);
@@ -792,8 +825,8 @@ struct Impl final : HipBackend {
         const FieldDecl& d = law->fields[static_cast<std::size_t>(field)];
         double lo = d.lo, hi = d.hi;
         int clamp = d.clamp ? 1 : 0;
-        unsigned* rec = ranged ? records.as<unsigned>() + 2 * field : nullptr;
-        unsigned long long* first = firsts.as<unsigned long long>() + field;
+        unsigned* rec = ranged ? records.as<unsigned>() + 2 * record : nullptr;
+        unsigned long long* first = firsts.as<unsigned long long>() + (ranged ? record : 0);
         void* argv1[] = {k.data(), &field, &chunk, &parts, &lo, &hi, &clamp, &rec, &first};
         check(fe_hip_launch(fn("k_total"), static_cast<unsigned>(blocks), 1, 1, 256, 1, 1, 0, argv1), "total launch");
         double* dst = totals.as<double>() + slot;
@@ -806,13 +839,15 @@ struct Impl final : HipBackend {
         }
     }
 
-    void range(int field)
+    // A range check of a field into a record of its own (the counts and first bad cell of this check alone, as the
+    // CPU reports each term's check, src/field/Solver.cpp:463-511).
+    void range(int field, int record)
     {
         const FieldDecl& d = law->fields[static_cast<std::size_t>(field)];
         double lo = d.lo, hi = d.hi;
         int clamp = d.clamp ? 1 : 0;
-        unsigned* rec = records.as<unsigned>() + 2 * field;
-        unsigned long long* first = firsts.as<unsigned long long>() + field;
+        unsigned* rec = records.as<unsigned>() + 2 * record;
+        unsigned long long* first = firsts.as<unsigned long long>() + record;
         launch_cells("k_range", 0.0, {&field, &lo, &hi, &clamp, &rec, &first});
         if (clamp) {
             touched(field);
@@ -832,14 +867,15 @@ struct Impl final : HipBackend {
 
     void reset_records()
     {
-        const std::size_t nf = std::max<std::size_t>(1, law->fields.size());
-        check(fe_hip_memset_d32_async(records.p, 0u, nf * 2), "records reset");
-        check(fe_hip_memset_d32_async(firsts.p, 0xffffffffu, nf * 2), "records reset");
+        check(fe_hip_memset_d32_async(records.p, 0u, static_cast<std::size_t>(kRanges) * 2), "records reset");
+        check(fe_hip_memset_d32_async(firsts.p, 0xffffffffu, static_cast<std::size_t>(kRanges) * 2), "records reset");
+        next_record = 0;
     }
+    int next_record = 0; // the next free range record since the records were last reset
 
     struct Judge {
         std::string term;
-        std::vector<int> writes;
+        std::vector<std::pair<int, int>> ranged; // (field, its range record) for every field the program wrote
         std::vector<std::vector<int>> groups;
         int slot = 0; // totals[slot + 4g ...]: before sum, before mag, after sum, after mag
         int substeps = 1;
@@ -883,7 +919,7 @@ struct Impl final : HipBackend {
         const double sub_dt = dt / substeps;
         Judge j;
         j.term = term;
-        j.writes = p.writes;
+
         j.groups = p.conserves;
         j.substeps = substeps;
         j.slot = total_slot;
@@ -918,9 +954,11 @@ struct Impl final : HipBackend {
                 for (int q = 0; q < m; ++q) {
                     const int f = group[static_cast<std::size_t>(q)];
                     const bool writes = std::find(p.writes.begin(), p.writes.end(), f) != p.writes.end() && ranged.count(f) == 0;
-                    total(f, slot + 2 * m + 2 * q, writes);
+                    const int record = writes ? next_record++ : -1;
+                    total(f, slot + 2 * m + 2 * q, record);
                     note_total(f, slot + 2 * m + 2 * q);
                     if (writes) {
+                        j.ranged.emplace_back(f, record);
                         ranged.insert(f);
                         if (law->fields[static_cast<std::size_t>(f)].clamp) {
                             cached_totals.erase(f); // the clamp changed it after its total
@@ -931,7 +969,8 @@ struct Impl final : HipBackend {
             }
             for (const int f : p.writes) {
                 if (ranged.count(f) == 0) {
-                    range(f);
+                    j.ranged.emplace_back(f, next_record);
+                    range(f, next_record++);
                 }
             }
             judges.push_back(j);
@@ -997,9 +1036,9 @@ struct Impl final : HipBackend {
                     violations.push_back(v);
                 }
             }
-            for (const int f : j.writes) {
-                const unsigned out = rec[2 * static_cast<std::size_t>(f)];
-                const unsigned nan = rec[2 * static_cast<std::size_t>(f) + 1];
+            for (const auto& [f, r] : j.ranged) {
+                const unsigned out = rec[2 * static_cast<std::size_t>(r)];
+                const unsigned nan = rec[2 * static_cast<std::size_t>(r) + 1];
                 if (out + nan == 0) {
                     continue;
                 }
@@ -1008,7 +1047,7 @@ struct Impl final : HipBackend {
                 v.kind = nan > 0 ? Violation::Kind::NotFinite : Violation::Kind::Range;
                 v.term = j.term;
                 v.field = d.name;
-                const unsigned long long i = first[static_cast<std::size_t>(f)];
+                const unsigned long long i = first[static_cast<std::size_t>(r)];
                 const std::size_t nx = static_cast<std::size_t>(shape.nx), ny = static_cast<std::size_t>(shape.ny);
                 std::ostringstream det;
                 det << out + nan << " cell(s); first at (" << i % nx << "," << (i / nx) % ny << "," << i / (nx * ny)
@@ -1082,9 +1121,9 @@ struct Impl final : HipBackend {
                 }
             }
             for (const int f : targets) {
-                range(f);
+                j.ranged.emplace_back(f, next_record);
+                range(f, next_record++);
             }
-            j.writes = targets;
             judges.push_back(j);
         }
     }
@@ -1159,7 +1198,7 @@ struct Impl final : HipBackend {
 
     // A step the card can take from a graph: lazy, no contracts due, no guarded learned term, and substeps that need
     // no reduction read back (max_dt of params, h and constants only, or not respected).
-    bool replayable(bool respect, bool learned, const FieldEdits& edits) const
+    bool replayable(bool respect, bool learned, const FieldEdits& edits)
     {
         if (!lazy || edits.all || edits.shifted()) {
             return false;
@@ -1177,7 +1216,7 @@ struct Impl final : HipBackend {
                 return false;
             }
         }
-        return true;
+        return !any_idle(); // a graph runs every term; a step that skips one is taken eagerly
     }
 
     [[nodiscard]] const std::vector<std::string>& gather_kernels() const override { return gathers; }
@@ -1204,6 +1243,7 @@ struct Impl final : HipBackend {
             if (length > 0 && replayable(respect, learned, edits)) {
                 replay_pair(dt, respect, length, substeps);
                 step_count += length;
+                carry_bounds(dt, length, substeps);
                 for (const auto* progs : {&module.terms, &module.derives}) {
                     for (const ir::Program& p : *progs) {
                         for (const int f : p.writes) {
@@ -1303,6 +1343,103 @@ struct Impl final : HipBackend {
         run_derives(false, judges);
     }
 
+    // ---- ranges and idle terms ----
+
+    // Whether a program has nothing to do this step: every equation's rate is exactly 0 at every cell, provably from
+    // the fields known to be zero (always_zero) or from the carried ranges (bounds_of is [0, 0]); the reference's test
+    // (src/field/cuda/CudaSolver.cpp:590-592). An empty program is idle.
+    bool idle(const std::vector<Equation>& equations) const
+    {
+        std::vector<char> zero(flowing.size(), 0);
+        for (std::size_t f = 0; f < flowing.size(); ++f) {
+            zero[f] = flowing[f].none() ? 1 : 0;
+        }
+        for (const Equation& eq : equations) {
+            if (!always_zero(*eq.rhs, zero, law->param_values) && !bounds_of(*eq.rhs, flowing, law->param_values, shape.cell).none()) {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // A derive recomputes its target from other fields: its range is unknown until measured.
+    void derives_unbounded()
+    {
+        for (const Derive& d : law->derives) {
+            flowing[static_cast<std::size_t>(d.target)] = Bounds{};
+        }
+    }
+
+    // Queues the measurement of the fields' ranges (k_bounds, one pass over each) into bounds_keys.
+    void queue_measure(const std::vector<int>& which)
+    {
+        int nf = static_cast<int>(law->fields.size());
+        unsigned* keys = bounds_keys.as<unsigned>();
+        void* rargs[] = {&keys, &nf};
+        check(fe_hip_launch(fn("k_bounds_reset"), static_cast<unsigned>((2 * nf + 255) / 256), 1, 1, 256, 1, 1, 0, rargs), "launch k_bounds_reset");
+        std::vector<std::uint64_t> k = args(0.0);
+        for (int f : which) {
+            void* bargs[] = {k.data(), &f, &keys, &nf};
+            check(fe_hip_launch(fn("k_bounds"), 2048, 1, 1, 256, 1, 1, 0, bargs), "launch k_bounds");
+            ++prof.launches;
+        }
+        prof.launches += 1;
+    }
+
+    // The measured ranges into `known` (after the card has finished the copy into `host`).
+    void take_measure(const std::vector<int>& which, const std::vector<unsigned>& host)
+    {
+        const std::size_t nf = law->fields.size();
+        auto value = [](unsigned key) {
+            const unsigned b = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+            float v = 0.0f;
+            std::memcpy(&v, &b, 4);
+            return static_cast<double>(v);
+        };
+        for (const int f : which) {
+            const unsigned lo = host[static_cast<std::size_t>(f)], hi = host[nf + static_cast<std::size_t>(f)];
+            if (lo == 0xffffffffu && hi == 0u) { // every cell NaN: the empty range, as a look over the cells finds
+                known[static_cast<std::size_t>(f)] = Bounds{std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity()};
+            } else {
+                known[static_cast<std::size_t>(f)] = Bounds{value(lo), value(hi)};
+            }
+        }
+    }
+
+    // `steps` replayed steps carried through flow_bounds on the host (no measurement), with each term's substeps.
+    void carry_bounds(double dt, int steps, const std::vector<int>& substeps)
+    {
+        for (int s = 0; s < steps; ++s) {
+            flowing = known;
+            for (std::size_t t = 0; t < law->terms.size(); ++t) {
+                flow_bounds(law->terms[t].equations, dt, t < substeps.size() ? substeps[t] : 1, *law, shape.cell, flowing);
+            }
+            derives_unbounded();
+            bool same = true;
+            for (std::size_t f = 0; f < known.size(); ++f) {
+                same = same && flowing[f].lo == known[f].lo && flowing[f].hi == known[f].hi;
+            }
+            known = flowing;
+            if (same) {
+                break; // a fixed point: further steps carry nothing further
+            }
+        }
+    }
+
+    // Whether any term would be skipped at the next step (then the step is taken eagerly, not from a graph that runs
+    // every term).
+    bool any_idle()
+    {
+        flowing = known;
+        for (std::size_t t = 0; t < law->terms.size(); ++t) {
+            if (!layout.terms[t].gather.empty() && idle(law->terms[t].equations)) {
+                return true;
+            }
+            flow_bounds(law->terms[t].equations, 0.0, 1, *law, shape.cell, flowing);
+        }
+        return false;
+    }
+
     void step_locked(double dt, bool respect_max_dt, bool learned, Law& l, FieldSet& fields, FieldEdits& edits,
         std::vector<char>& changed, std::vector<Violation>& violations, std::vector<int>& substeps)
     {
@@ -1329,6 +1466,13 @@ struct Impl final : HipBackend {
             reset_records();
         }
         substeps.assign(module.terms.size(), 0);
+        flowing = known;
+        std::vector<char> wrote(nf, 0); // fields a program that ran wrote this step
+        auto note_writes = [&](const ir::Program& p) {
+            for (const int f : p.writes) {
+                wrote[static_cast<std::size_t>(f)] = 1;
+            }
+        };
         std::size_t ev = 0;
         for (std::size_t t = 0; t < module.terms.size(); ++t) {
             TermProfile& tp = prof.terms[t];
@@ -1336,19 +1480,25 @@ struct Impl final : HipBackend {
             const int launches0 = prof.launches;
             check(fe_hip_event_record(event(ev)), "event");
             const bool guarded = learned && !module.learned[t].equations.empty();
-            // Judge what is pending before the records are reused: a guarded term judges its own at once, and the
-            // totals buffer holds kTotals doubles.
+            // Judge what is pending before the records are reused: a guarded term judges its own at once, the totals
+            // buffer holds kTotals doubles and the range records kRanges checks.
             const int need = std::max(total_slots(module.terms[t]), total_slots(module.learned[t]));
-            if (contracts && !judges.empty() && (guarded || total_slot + need > kTotals)) {
+            const int checks = static_cast<int>(std::max(module.terms[t].writes.size(), module.learned[t].writes.size()));
+            if (contracts && !judges.empty() && (guarded || total_slot + need > kTotals || next_record + checks > kRanges)) {
                 judge(judges, violations);
                 judges.clear();
                 total_slot = 0;
                 reset_records();
             }
             bool done = false;
-            if (guarded) {
+            if (guarded && idle(law->terms[t].learned)) {
+                tp.idle = true; // the learned scheme has nothing to do, so the term holds as it is
+                substeps[t] = 1;
+                done = true;
+            } else if (guarded) {
                 // Snapshot what the term writes, run the learned scheme and judge it now; a breach is undone and the
                 // classical scheme runs (src/field/cuda/CudaSolver.cpp:673-705 is the reference's guard).
+                const std::vector<Bounds> flowing_before = flowing;
                 snapshot(module.terms[t].writes);
                 std::vector<Judge> own;
                 int slot = 0;
@@ -1362,16 +1512,25 @@ struct Impl final : HipBackend {
                     tp.learned = true;
                     substeps[t] = n;
                     done = true;
+                    flow_bounds(law->terms[t].learned, dt, n, *law, shape.cell, flowing);
+                    note_writes(module.learned[t]);
                 } else {
                     restore(module.terms[t].writes);
+                    flowing = flowing_before;
                     tp.fell_back = true;
                     ++prof.fallbacks;
                     check(fe_hip_stream_sync(), "sync");
                 }
             }
-            if (!done) {
+            if (!done && idle(law->terms[t].equations)) {
+                // Nothing to do: not dispatched, and its contracts hold without counting (src/field/Solver.cpp:341-347).
+                tp.idle = true;
+                substeps[t] = 1;
+            } else if (!done) {
                 substeps[t] = run_program(module.terms[t], layout.terms[t], module.terms[t].name, dt, respect_max_dt, contracts,
                     violations, judges, total_slot);
+                flow_bounds(law->terms[t].equations, dt, substeps[t], *law, shape.cell, flowing);
+                note_writes(module.terms[t]);
             }
             check(fe_hip_event_record(event(ev + 1)), "event");
             gathers.resize(module.terms.size());
@@ -1386,9 +1545,37 @@ struct Impl final : HipBackend {
         if (derive_timed) {
             check(fe_hip_event_record(event(ev++)), "event");
         }
+        if (contracts && !judges.empty() && next_record + static_cast<int>(law->fields.size()) > kRanges) {
+            judge(judges, violations);
+            judges.clear();
+            total_slot = 0;
+            reset_records();
+        }
         run_derives(contracts, judges);
         if (derive_timed) {
             check(fe_hip_event_record(event(ev++)), "event");
+        }
+        derives_unbounded();
+        for (const ir::Program& p : module.derives) {
+            note_writes(p);
+        }
+        // The ranges the next step starts from: measured on the card after an eager step and every kTightenEvery lazy
+        // steps (one pass over each field written, read back with what else this step reads), carried otherwise.
+        known = flowing;
+        const bool measure = !lazy || (lazy_steps++ % kTightenEvery) == 0;
+        std::vector<int> measured;
+        std::vector<unsigned> keys;
+        if (measure) {
+            for (std::size_t f = 0; f < nf; ++f) {
+                if (wrote[f]) {
+                    measured.push_back(static_cast<int>(f));
+                }
+            }
+            if (!measured.empty()) {
+                queue_measure(measured);
+                keys.assign(2 * nf, 0u);
+                check(fe_hip_memcpy_dtoh_async(keys.data(), bounds_keys.p, keys.size() * 4), "bounds readback");
+            }
         }
         if (contracts) {
             judge(judges, violations);
@@ -1406,6 +1593,10 @@ struct Impl final : HipBackend {
             }
         } else {
             read_changed(changed);
+        }
+        if (!measured.empty()) {
+            check(fe_hip_stream_sync(), "sync");
+            take_measure(measured, keys);
         }
         for (std::size_t f = 0; f < nf; ++f) {
             stale[f] = stale[f] || changed[f];
@@ -1455,7 +1646,19 @@ struct Impl final : HipBackend {
         check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, law->fields.size())), "change flags reset");
         std::vector<Judge> none;
         run_derives(false, none);
+        std::vector<int> measured;
+        for (const Derive& d : law->derives) {
+            if (std::find(measured.begin(), measured.end(), d.target) == measured.end()) {
+                measured.push_back(d.target);
+            }
+        }
+        std::vector<unsigned> keys(2 * law->fields.size(), 0u);
+        if (!measured.empty()) {
+            queue_measure(measured);
+            check(fe_hip_memcpy_dtoh_async(keys.data(), bounds_keys.p, keys.size() * 4), "bounds readback");
+        }
         read_changed(changed);
+        take_measure(measured, keys);
         for (std::size_t f = 0; f < law->fields.size(); ++f) {
             stale[f] = stale[f] || changed[f];
         }
@@ -1479,6 +1682,7 @@ struct Impl final : HipBackend {
     void* device_field(int f) override
     {
         touched(f); // whoever holds the address may write through it: a cached total no longer stands for it
+        known[static_cast<std::size_t>(f)] = Bounds{}; // and its range is no longer known
         return field_pointer(f);
     }
     // A caller wrote field f's cells on the card: the host copy is stale until fetched.
@@ -1486,6 +1690,7 @@ struct Impl final : HipBackend {
     {
         stale[static_cast<std::size_t>(f)] = 1;
         touched(f);
+        known[static_cast<std::size_t>(f)] = Bounds{};
     }
 
     // The field's fixed-order compensated total and total magnitude, from the card (waits for it).

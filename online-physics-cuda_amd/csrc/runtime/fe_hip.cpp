@@ -61,7 +61,7 @@ void open_once()
     s.info.compute_units = prop.multiProcessorCount;
     s.info.hbm_bytes = prop.totalGlobalMem;
     s.info.lds_bytes = static_cast<int>(prop.sharedMemPerBlock);
-    hipRuntimeGetVersion(&s.info.runtime_version);
+    (void)hipRuntimeGetVersion(&s.info.runtime_version);
     std::string arch = prop.gcnArchName;
     const std::size_t colon = arch.find(':');
     s.arch = colon == std::string::npos ? arch : arch.substr(0, colon);
@@ -198,7 +198,7 @@ int fe_hip_compile(const char* source, const char* name, const char* const* opti
         opts.emplace_back(options[i]);
     }
     int rt = 0;
-    hipRuntimeGetVersion(&rt);
+    (void)hipRuntimeGetVersion(&rt);
     std::uint64_t key = fnv1a(source, std::strlen(source), 1469598103934665603ull);
     for (const std::string& o : opts) {
         key = fnv1a(o.data(), o.size(), key);
@@ -426,7 +426,7 @@ int fe_hip_graph_end_capture(fe_hip_graph* out)
     }
     hipGraphExec_t exec = nullptr;
     r = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     *out = reinterpret_cast<fe_hip_graph>(exec);
     return static_cast<int>(r);
 }

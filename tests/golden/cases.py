@@ -346,6 +346,144 @@ NET_CASES = [
          nets={"w": net_text("w", [5, 16, 4], "relu", 43, 0.3)}, atol_moved=0.03),
 ]
 
+# Terms with nothing to do: the CPU engine does not run a term whose every rate is exactly zero by its interval rules
+# (src/field/Solver.cpp:341-347), so fields it would touch keep their bits (a -0 stays -0, where adding a +0 rate would
+# make it +0). `idle_cold`: no fire anywhere, so `spread` and `settle` are idle every step; `idle_spark`: one spark,
+# so they wake as the fire spreads.
+IDLE_LAW = """field fire init 0 range -100 100
+field T init 0 range -1000 1000
+field ash init 0 range -100 100
+
+param burn = 3.0
+
+term spread
+  d fire = diffuse(0.2, fire)
+  max_dt = 0.15 * h * h / 0.2
+
+term heat
+  d T = burn * fire + diffuse(0.1, T)
+
+term settle
+  d ash = 0.5 * fire * fire
+"""
+
+
+# A world of many terms with the op mix of the reference's shipped world law (threshold changes of state through step,
+# min, max and pos; shifts; falling, spreading and toppling matter; diffusion with a field conductivity; gradients;
+# upwind advection; a distance and a Poisson solve; reductions; clamped and static fields; derives), written for this
+# project. The reference holds its CUDA backend to its CPU on 12 steps of its world law on a 48x40x24 scene
+# (src/tests/cuda_parity_smoke.cpp:83-121); `valley` is that check on this law and scene.
+VALLEY_LAW = """field rock  init 0   range 0 1.01 static
+field sand  init 0   range 0 1.01
+field water init 0   range 0 1.01
+field ice   init 0   range 0 1.01
+field steam init 0   range 0 1.01
+field fuel  init 0   range 0 1
+field flame init 0   range 0 1.01 clamp
+field smoke init 0   range 0 100 clamp
+field ash   init 0   range 0 1.01
+field T     init 290 range 0 8000
+field wet   init 0   range 0 3
+field near  init 0   range -1000 1000
+derive wet = water + ice
+derive wet += 0.5 * steam
+derive near = distance(flame)
+
+param g = 2.0
+param k_rock = 0.6
+param k_air = 0.05
+param burn_at = 600
+param burn_rate = 0.8
+param heat_of_burning = 900
+
+term settle
+  let room = pos(1 - rock - sand - water - ice)
+  d sand = fall(sand, rock + ice, g) + 0.2 * topple(sand, rock, 1)
+  conserves sand
+  max_dt = 0.4 * h / g
+
+term pour
+  d water = fall(water, rock + sand + ice, g) + spread(water, rock + sand + ice, 0.3) + 0.1 * topple(water, rock + sand, 1)
+  conserves water
+  max_dt = 0.4 * h / g
+
+term conduct
+  d T = diffuse(k_air + (k_rock - k_air) * (rock + sand + 0.5 * water), T)
+  max_dt = 0.15 * h * h / k_rock
+
+term burn
+  let hot = step(T - burn_at)
+  let rate = burn_rate * min(fuel, hot * (0.2 + flame))
+  d fuel = -rate
+  d flame = 2 * rate - 0.5 * flame * step(0.05 - fuel)
+  d T = heat_of_burning * rate
+  d ash = 0.3 * rate
+
+term catch
+  let neighbours = max(max(shift(flame, 1, 0, 0), shift(flame, -1, 0, 0)), max(shift(flame, 0, 1, 0), shift(flame, 0, -1, 0)))
+  d flame = 0.5 * fuel * step(neighbours - 0.3) * step(T - 450)
+
+term rise
+  d smoke = 0.6 * flame - 0.3 * gz(smoke) - 0.02 * smoke + 0.05 * lap(smoke)
+  d T = -0.1 * adv(T, 0, 0, 0.4 * flame)
+
+term freeze
+  let cold = step(273 - T)
+  d ice = 0.5 * water * cold - 0.5 * ice * step(T - 273)
+  d water = -0.5 * water * cold + 0.5 * ice * step(T - 273)
+  conserves water + ice
+
+term boil
+  let boiling = pos(T - 373) / 100
+  d steam = 0.4 * min(water, boiling) - 0.05 * steam * step(350 - T)
+  d water = -0.4 * min(water, boiling) + 0.05 * steam * step(350 - T)
+  d T = -40 * min(water, boiling)
+
+term vapour
+  d steam = -0.5 * gz(steam) + 0.1 * lap(steam) + 0.01 * (shift(steam, 0, 0, -1) - steam)
+
+term wind
+  d smoke = 0.05 * (maxof(T) - T) / (maxof(T) + 1) * gx(smoke) + 0.02 * gy(smoke)
+
+term radiate
+  d T = -0.0002 * pos(T - 290) * (1 - rock) + 0.01 * sqrt(pos(T - 290)) * step(0.5 - near)
+
+term press
+  d water = 0.0005 * poisson(wet - 0.4, 8) * (1 - rock)
+
+term crumble
+  d sand = 0.02 * rock * step(T - 1200) * (1 - sand)
+
+term cool_ash
+  d ash = -0.01 * ash * step(350 - T) + 0.005 * max(shift(ash, 0, 0, 1) - ash, 0)
+"""
+
+
+def valley_init(nx=48, ny=40, nz=24):
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rock = (z < 5).astype(np.float32)
+    sand = ((z >= 5) & (z < 7) & (x < 20)).astype(np.float32)
+    fuel = ((z >= 5) & (z < 8) & (x >= 12) & (x < 30) & (y >= 10) & (y < 30)).astype(np.float32)
+    water = ((z >= 5) & (z < 10) & (x >= 34) & (x < 44) & (y >= 4) & (y < 14)).astype(np.float32)
+    ice = ((z >= 5) & (z < 7) & (x >= 40) & (y >= 30)).astype(np.float32)
+    t = np.full(x.shape, 290.0)
+    t[(x - 20) ** 2 + (y - 20) ** 2 + (z - 7) ** 2 <= 9] = 1400.0
+    t[ice > 0] = 250.0
+    t = t + np.random.default_rng(71).random(x.shape) * 0.5
+    flame = np.where((x - 20) ** 2 + (y - 20) ** 2 + (z - 7) ** 2 <= 4, 0.6, 0.0)
+    out = {"rock": rock, "sand": sand, "water": water, "ice": ice, "fuel": fuel, "T": t, "flame": flame}
+    return {k: np.ascontiguousarray(v, np.float32).ravel() for k, v in out.items()}
+
+
+def _idle_init(n, spark):
+    fire = np.zeros(n, np.float32)
+    if spark:
+        fire[n // 2 + 3] = 5.0
+    t = _uniform(n, 61, -1.0, 1.0)
+    t[::7] = -0.0
+    return {"fire": fire, "T": t, "ash": np.full(n, -0.0, np.float32)}
+
+
 CASES = [
     dict(name="heat_20x14x9", law=HEAT_LAW, grid=(20, 14, 9, 0.5), dt=0.9, steps=4,
          params={"k": 0.1}, init={"T": _uniform(20 * 14 * 9, 1)}),
@@ -375,6 +513,9 @@ CASES = [
          init=heston_state(32, 16, HESTON_PLANES)),
     dict(name="heston_hv_3", law=HESTON_HV_LAW, grid=(32, 16, 3, 6.0 / 32), dt=0.5 / 8, steps=8, params={},
          init=heston_state(32, 16, HESTON_PLANES)),
+    dict(name="valley", law=VALLEY_LAW, grid=(48, 40, 24, 0.5), dt=0.25, steps=12, params={}, init=valley_init()),
+    dict(name="idle_cold", law=IDLE_LAW, grid=(12, 10, 6, 0.5), dt=0.2, steps=6, params={}, init=_idle_init(12 * 10 * 6, False)),
+    dict(name="idle_spark", law=IDLE_LAW, grid=(12, 10, 6, 0.5), dt=0.2, steps=6, params={}, init=_idle_init(12 * 10 * 6, True)),
 ] + [
     dict(name="op_" + key, law=op_law(expr), grid=G_OPS, dt=0.02, steps=4, params={},
          init=_ops_init(int(np.prod(G_OPS[:3])), 100 + i))

@@ -20,6 +20,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 HERE = os.path.dirname(os.path.abspath(__file__))
 LAWC = os.path.join(ROOT, "oracle", "_ref", "lawc")
+VIOLATIONS = os.path.join(ROOT, "oracle", "_ref", "ref_violations")
 sys.path.insert(0, HERE)
 
 from cases import CASES  # noqa: E402
@@ -52,7 +53,14 @@ def run_reference(case: dict) -> dict:
         final_dir = os.path.join(out_dir, f"step_{case['steps']}")
         meta = json.load(open(os.path.join(final_dir, "meta.json")))
         final = {name: np.fromfile(os.path.join(final_dir, name + ".bin"), dtype=np.float32) for name in meta["fields"]}
-        return {"final": final, "stdout": res.stdout, "stderr": res.stderr}
+        # The last step's violations, from the same engine run the same way (oracle/ref_violations.cpp).
+        viol = subprocess.run([VIOLATIONS, law_path, str(nx), str(ny), str(nz), repr(float(h)), init_dir, str(case["steps"]),
+                               repr(float(case["dt"]))], capture_output=True, text=True, cwd=tmp,
+                              env=dict(os.environ, OMP_NUM_THREADS="4"))
+        if viol.returncode != 0:
+            raise RuntimeError(f"ref_violations failed for {case['name']}: {viol.stderr}")
+        last = [json.loads(line) for line in viol.stdout.splitlines() if line.strip()]
+        return {"final": final, "stdout": res.stdout, "stderr": res.stderr, "final_violations": last}
 
 
 def main() -> None:
@@ -76,6 +84,7 @@ def main() -> None:
         if "atol_moved" in case:
             moved = max(float(np.max(np.abs(out["final"][f].astype(np.float64) - case["init"][f]))) for f in case["init"])
             meta["atol"] = case["atol_moved"] * moved + 1e-3
+        meta["final_violations"] = out["final_violations"]
         meta["reference_stdout"] = out["stdout"]
         meta["reference_stderr"] = out["stderr"]
         arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)

@@ -42,7 +42,14 @@ def test_engine_matches_reference_bits(device, name):
     for field, expected in final.items():
         got = solver.get_field(field)
         assert golden_io.same(meta, got, expected), f"{name}/{field}: max |diff| {np.max(np.abs(got - expected))}"
-    assert solver.violations() == [] or all(v["kind"] != "conservation" for v in solver.violations())
+    if "final_violations" in meta:
+        # The last step's broken contracts are the reference's: the same terms, fields, kinds and cell counts.
+        got_v = {(v["term"], v["field"], v["kind"], int(v["detail"].split()[0]) if v["kind"] == "range" else -1)
+                 for v in solver.violations()}
+        want_v = {(v["term"], v["field"], v["kind"], v["cells"] if v["kind"] == "range" else -1) for v in meta["final_violations"]}
+        assert got_v == want_v
+    else:
+        assert solver.violations() == [] or all(v["kind"] != "conservation" for v in solver.violations())
 
 
 @pytest.mark.parametrize("name", ["heat_20x14x9", "multi", "diffreact_rk4", "net_flux_tanh", "net_ldiffuse_relu"])
@@ -327,3 +334,60 @@ def test_new_net_weights_are_taken_up(device, name, net):
     for f in init:
         assert np.array_equal(changed[f].view(np.uint32), fresh[f].view(np.uint32)), f
     assert any(not np.array_equal(changed[f], kept[f]) for f in init)
+
+
+@pytest.mark.parametrize("lazy", [False, True])
+def test_idle_terms_are_not_dispatched(device, lazy):
+    """A term whose every rate is provably zero is skipped, as the CPU engine skips it: with no fire anywhere, `spread`
+    and `settle` stay idle (and `ash`, all -0, keeps its sign: the golden case holds the CPU's bits)."""
+    import field_hip
+
+    meta, init, final = golden_io.load("idle_cold")
+    _, solver = run_engine(field_hip, meta, init, lazy=lazy, contracts=1000 if lazy else 1)
+    idle = {t["name"]: t["idle"] for t in solver.profile()["terms"]}
+    assert idle == {"spread": True, "heat": False, "settle": True}
+    for field, expected in final.items():
+        assert exact(solver.get_field(field), expected), field
+
+
+@pytest.mark.parametrize("name", ["idle_cold", "idle_spark"])
+def test_run_equals_steps_when_terms_wake_or_sleep(device, name):
+    """Solver.run replays graphs only while no term is idle; the state after a run is the state after the same steps
+    one at a time, and the reference's."""
+    import field_hip
+
+    meta, init, final = golden_io.load(name)
+    nx, ny, nz, h = meta["grid"]
+    s = field_hip.Solver(golden_io.make_law(field_hip, meta), (nx, ny, nz), h)
+    for f, values in init.items():
+        s.set_field(f, values)
+    s.set_lazy_readback(True)
+    s.check_contracts_every(1000)
+    s.run(meta["dt"], meta["steps"])
+    for field, expected in final.items():
+        assert exact(s.get_field(field), expected), field
+
+
+def test_many_term_world_law_runs_as_the_reference(device):
+    """The valley law (12 fields, 14 terms, the world law's op mix) on the 48x40x24 scene, 12 steps: the reference
+    holds its CUDA backend to its CPU there to 1e-4 and two runs to the same digest (src/tests/cuda_parity_smoke.cpp:
+    83-121); here a lazy Solver.run (graph replays where no contract falls due) must give the reference CPU's bits,
+    and so must a second run."""
+    import field_hip
+
+    meta, init, final = golden_io.load("valley")
+    nx, ny, nz, h = meta["grid"]
+    digests = []
+    for _ in range(2):
+        s = field_hip.Solver(golden_io.make_law(field_hip, meta), (nx, ny, nz), h)
+        for f, values in init.items():
+            s.set_field(f, values)
+        s.refresh_derived()
+        s.set_lazy_readback(True)
+        s.check_contracts_every(5)
+        s.run(meta["dt"], meta["steps"])
+        for field, expected in final.items():
+            got = s.get_field(field)
+            assert golden_io.same(meta, got, expected), f"{field}: max |diff| {np.max(np.abs(got - expected))}"
+        digests.append(s.digest())
+    assert digests[0] == digests[1]
