@@ -136,6 +136,12 @@ def _declare(lib: ctypes.CDLL) -> None:
                                      c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
         "fe_lnet_adam": (c_int, [c_void_p, c_double, c_double]),
         "fe_lnet_coarsen": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+        "fe_mppi_create": (c_void_p, [c_int, c_int, c_int, c_int, ctypes.c_char_p, c_size_t]),
+        "fe_mppi_free": (None, [c_void_p]),
+        "fe_mppi_error": (ctypes.c_char_p, [c_void_p]),
+        "fe_mppi_set_dynamics": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+        "fe_mppi_set_cost": (c_int, [c_void_p, c_void_p, c_void_p, c_float]),
+        "fe_mppi_plan": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_uint64, c_float, c_float, c_void_p, c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -670,3 +676,50 @@ def coarsen(fine, fnx, fny, count, factor, coarse) -> None:
     p = lambda a: a.ptr if isinstance(a, DeviceArray) else a  # noqa: E731
     if lib().fe_lnet_coarsen(p(fine), fnx, fny, count, factor, p(coarse)) != 0:
         raise FieldHipError("coarsen: invalid arguments")
+
+
+class MppiPlanner:
+    """A world model's MPPI planner on the card (include/fe_mppi.h): dynamics F + 2 -> hidden -> hidden -> F (tanh),
+    `samples` futures of `horizon` steps for every agent, a weighted quadratic cost about a target."""
+
+    def __init__(self, features: int, hidden: int, horizon: int, samples: int):
+        why = ctypes.create_string_buffer(1024)
+        self.ptr = lib().fe_mppi_create(features, hidden, horizon, samples, why, len(why))
+        if not self.ptr:
+            raise FieldHipError("planner refused: " + why.value.decode())
+        self.features, self.hidden, self.horizon, self.samples = features, hidden, horizon, samples
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fe_mppi_free(self.ptr)
+            self.ptr = None
+
+    def _check(self, status: int, what: str) -> None:
+        if status != 0:
+            raise FieldHipError(f"{what}: {lib().fe_mppi_error(self.ptr).decode()}")
+
+    def set_dynamics(self, weights, in_shift, in_scale, scale) -> None:
+        import numpy as np
+
+        arrays = [np.ascontiguousarray(a, np.float32) for a in (weights, in_shift, in_scale, scale)]
+        self._keep = arrays
+        self._check(lib().fe_mppi_set_dynamics(self.ptr, *[a.ctypes.data for a in arrays]), "set_dynamics")
+
+    def set_cost(self, target, weight, action_weight: float) -> None:
+        import numpy as np
+
+        t, w = np.ascontiguousarray(target, np.float32), np.ascontiguousarray(weight, np.float32)
+        self._check(lib().fe_mppi_set_cost(self.ptr, t.ctypes.data, w.ctypes.data, float(action_weight)), "set_cost")
+
+    def plan(self, obs, ids, mean, tick: int, sigma: float, dt: float, costs: bool = False):
+        """(new mean agents x horizon x 2, costs agents x samples or None)."""
+        import numpy as np
+
+        obs = np.ascontiguousarray(obs, np.float32)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.ascontiguousarray(mean, np.float32).copy()
+        agents = ids.size
+        c = np.empty((agents, self.samples), np.float32) if costs else None
+        self._check(lib().fe_mppi_plan(self.ptr, agents, obs.ctypes.data, ids.ctypes.data, int(tick), float(sigma), float(dt),
+                                       out.ctypes.data, c.ctypes.data if costs else None), "plan")
+        return out, c
