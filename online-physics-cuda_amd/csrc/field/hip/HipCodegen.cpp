@@ -93,6 +93,51 @@ __device__ __forceinline__ void add(Sum& a, double v)
     a.s = t;
 }
 __device__ __forceinline__ void merge(Sum& a, const Sum& b) { add(a, b.s); a.c += b.c; }
+// A pair folded across a wave's 64 lanes by a butterfly (lane i with lane i ^ o, o = 32, 16, ..., 1: cross-lane moves,
+// no LDS): the same tree every run, so the same bits; lane 0 holds the wave's total.
+__device__ __forceinline__ void wave_fold(Sum& a)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const Sum b = {__shfl_xor(a.s, o), __shfl_xor(a.c, o)};
+        merge(a, b);
+    }
+}
+// The least and greatest across a wave, the same butterfly (a comparison that a NaN never wins, as the host's).
+__device__ __forceinline__ void wave_bounds(double& lo, double& hi)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+        lo = a < lo ? a : lo;
+        hi = hi < b ? b : hi;
+    }
+}
+// A 256-thread block's four waves, in order: lanes 0 leave their wave's values in w[4][4]; thread 0 folds them.
+__device__ __forceinline__ void block_pairs(Sum& a, Sum& b, double (*w)[4])
+{
+    wave_fold(a); wave_fold(b);
+    const int t = (int)threadIdx.x;
+    if ((t & 63) == 0) { w[t >> 6][0] = a.s; w[t >> 6][1] = a.c; w[t >> 6][2] = b.s; w[t >> 6][3] = b.c; }
+    __syncthreads();
+    if (t == 0) {
+        a = Sum{w[0][0], w[0][1]}; b = Sum{w[0][2], w[0][3]};
+        for (int q = 1; q < 4; ++q) { merge(a, Sum{w[q][0], w[q][1]}); merge(b, Sum{w[q][2], w[q][3]}); }
+    }
+}
+__device__ __forceinline__ void block_sum_bounds(Sum& a, double& lo, double& hi, double (*w)[4])
+{
+    wave_fold(a); wave_bounds(lo, hi);
+    const int t = (int)threadIdx.x;
+    if ((t & 63) == 0) { w[t >> 6][0] = a.s; w[t >> 6][1] = a.c; w[t >> 6][2] = lo; w[t >> 6][3] = hi; }
+    __syncthreads();
+    if (t == 0) {
+        a = Sum{w[0][0], w[0][1]}; lo = w[0][2]; hi = w[0][3];
+        for (int q = 1; q < 4; ++q) {
+            merge(a, Sum{w[q][0], w[q][1]});
+            lo = w[q][2] < lo ? w[q][2] : lo;
+            hi = hi < w[q][3] ? w[q][3] : hi;
+        }
+    }
+}
 
 // A field's range contract (src/field/Solver.cpp:463-511): counts of non-finite and out-of-range cells and the least
 // index of a bad one (integer atomics: the order does not matter). A clamped field's finite strays are clamped instead.
@@ -149,7 +194,7 @@ extern "C" __global__ void k_scatter(K k, int field, const float* packed, int x0
 extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 chunk, double* out, double lo, double hi, int clamp,
     unsigned* rec, u64* first)
 {
-    __shared__ double ss[256], sc[256], ms[256], mc[256];
+    __shared__ double wv[4][4];
     const int t = (int)threadIdx.x;
     const i64 begin = (i64)blockIdx.x * chunk;
     i64 end = begin + chunk; if (end > k.cells) end = k.cells;
@@ -183,41 +228,33 @@ extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 ch
             }
         }
     }
-    ss[t] = s.s; sc[t] = s.c; ms[t] = m.s; mc[t] = m.c;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) {
-            Sum a = {ss[t], sc[t]}; merge(a, Sum{ss[t + w], sc[t + w]}); ss[t] = a.s; sc[t] = a.c;
-            Sum b = {ms[t], mc[t]}; merge(b, Sum{ms[t + w], mc[t + w]}); ms[t] = b.s; mc[t] = b.c;
-        }
-        __syncthreads();
-    }
-    if (t == 0) { out[2 * blockIdx.x] = ss[0] + sc[0]; out[2 * blockIdx.x + 1] = ms[0] + mc[0]; }
+    block_pairs(s, m, wv);
+    if (t == 0) { out[2 * blockIdx.x] = s.s + s.c; out[2 * blockIdx.x + 1] = m.s + m.c; }
 }
 
 // Pass 2: one block folds `parts` pairs in order into dst[0], dst[1].
 extern "C" __global__ __launch_bounds__(256) void k_total_final(const double* parts, int count, double* dst)
 {
-    __shared__ double ss[256], sc[256], ms[256], mc[256];
+    __shared__ double wv[4][4];
     const int t = (int)threadIdx.x;
     Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
     for (int i = t; i < count; i += 256) { add(s, parts[2 * i]); add(m, parts[2 * i + 1]); }
-    ss[t] = s.s; sc[t] = s.c; ms[t] = m.s; mc[t] = m.c;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) {
-            Sum a = {ss[t], sc[t]}; merge(a, Sum{ss[t + w], sc[t + w]}); ss[t] = a.s; sc[t] = a.c;
-            Sum b = {ms[t], mc[t]}; merge(b, Sum{ms[t + w], mc[t + w]}); ms[t] = b.s; mc[t] = b.c;
-        }
-        __syncthreads();
-    }
-    if (t == 0) { dst[0] = ss[0] + sc[0]; dst[1] = ms[0] + mc[0]; }
+    block_pairs(s, m, wv);
+    if (t == 0) { dst[0] = s.s + s.c; dst[1] = m.s + m.c; }
 }
 
 // acc[0..1] += src[0..1] in double (a field's total added on the card, in call order: no wait for the host).
 extern "C" __global__ void k_acc(const double* src, double* acc)
 {
     if (threadIdx.x == 0) { acc[0] = acc[0] + src[0]; acc[1] = acc[1] + src[1]; }
+}
+
+// Words [0, zeros) to 0 and [zeros, zeros + ones) to all ones: the change flags, range counts and first bad cells of a
+// step cleared in one launch.
+extern "C" __global__ void k_fill(unsigned* p, int zeros, int ones)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < zeros) p[i] = 0u; else if (i < zeros + ones) p[i] = 0xffffffffu;
 }
 
 // Clears n words (the change flags) with a kernel: a graph replays kernels, and fills recorded as memset nodes have
@@ -237,15 +274,21 @@ __device__ __forceinline__ unsigned order_key(float v)
     const unsigned b = __float_as_uint(v);
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
+__device__ __forceinline__ void bound_key(float x, unsigned& lo, unsigned& hi)
+{
+    if (x == x) { const unsigned key = order_key(x); lo = min(lo, key); hi = max(hi, key); }
+}
 extern "C" __global__ __launch_bounds__(256) void k_bounds(K k, int field, unsigned* out, int fields)
 {
     unsigned lo = 0xffffffffu, hi = 0u;
     const float* v = k.now[field];
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < k.cells; i += stride) {
-        const float x = v[i];
-        if (x == x) { const unsigned key = order_key(x); lo = min(lo, key); hi = max(hi, key); }
+    const i64 quads = k.cells / 4; // fields are 256-byte aligned: whole float4s, then the tail
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < quads; i += stride) {
+        const float4 q = reinterpret_cast<const float4*>(v)[i];
+        bound_key(q.x, lo, hi); bound_key(q.y, lo, hi); bound_key(q.z, lo, hi); bound_key(q.w, lo, hi);
     }
+    for (i64 i = 4 * quads + (i64)blockIdx.x * blockDim.x + threadIdx.x; i < k.cells; i += stride) bound_key(v[i], lo, hi);
     for (int o = 32; o > 0; o >>= 1) { lo = min(lo, (unsigned)__shfl_xor((int)lo, o)); hi = max(hi, (unsigned)__shfl_xor((int)hi, o)); }
     if ((threadIdx.x & 63u) == 0u) { atomicMin(out + field, lo); atomicMax(out + fields + field, hi); }
 }
@@ -258,7 +301,7 @@ extern "C" __global__ void k_bounds_reset(unsigned* out, int fields)
 // A reduction's partials (sum, least, most per block, from k_reduce passes) folded in order into U[uslot].
 extern "C" __global__ __launch_bounds__(256) void k_reduce_final(K k, int kind, int parts, int uslot)
 {
-    __shared__ double ss[256], sc[256], lo[256], hi[256];
+    __shared__ double wv[4][4];
     const int t = (int)threadIdx.x;
     Sum s = {0.0, 0.0}; double a = k.L[1]; double b = k.L[2];
     for (int i = t; i < parts; i += 256) {
@@ -266,17 +309,8 @@ extern "C" __global__ __launch_bounds__(256) void k_reduce_final(K k, int kind, 
         a = k.L[3 * i + 1] < a ? k.L[3 * i + 1] : a;
         b = b < k.L[3 * i + 2] ? k.L[3 * i + 2] : b;
     }
-    ss[t] = s.s; sc[t] = s.c; lo[t] = a; hi[t] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) {
-            Sum p = {ss[t], sc[t]}; merge(p, Sum{ss[t + w], sc[t + w]}); ss[t] = p.s; sc[t] = p.c;
-            lo[t] = lo[t + w] < lo[t] ? lo[t + w] : lo[t];
-            hi[t] = hi[t] < hi[t + w] ? hi[t + w] : hi[t];
-        }
-        __syncthreads();
-    }
-    if (t == 0) ((double*)k.U)[uslot] = kind == 0 ? ss[0] + sc[0] : (kind == 1 ? hi[0] : lo[0]);
+    block_sum_bounds(s, a, b, wv);
+    if (t == 0) ((double*)k.U)[uslot] = kind == 0 ? s.s + s.c : (kind == 1 ? b : a);
 }
 )HIP";
 
@@ -982,7 +1016,7 @@ private:
             const int arg = in.args[0];
             const int kind = in.op == Op::SumOf ? 0 : (in.op == Op::MaxOf ? 1 : 2);
             out_ << "extern \"C\" __global__ __launch_bounds__(256) void " << s.kernel << "(K k, i64 chunk) {\n"
-                 << "    __shared__ double ss[256], sc[256], lo[256], hi[256];\n"
+                 << "    __shared__ double wv[4][4];\n"
                  << "    const int t = (int)threadIdx.x; const i64 begin = (i64)blockIdx.x * chunk; i64 end = begin + chunk;\n"
                  << "    if (end > k.cells) end = k.cells;\n";
             out_ << "    const Direct m(k);\n";
@@ -1000,12 +1034,8 @@ private:
                 out_ << "        const double q = (double)v" << arg << "(k, m, x, y, z);\n";
             }
             out_ << "        add(s, q); a = q < a ? q : a; b = b < q ? q : b; }\n"
-                 << "    ss[t] = s.s; sc[t] = s.c; lo[t] = a; hi[t] = b; __syncthreads();\n"
-                 << "    for (int w = 128; w > 0; w >>= 1) {\n"
-                 << "        if (t < w) { Sum p = {ss[t], sc[t]}; merge(p, Sum{ss[t + w], sc[t + w]}); ss[t] = p.s; sc[t] = p.c;\n"
-                 << "            lo[t] = lo[t + w] < lo[t] ? lo[t + w] : lo[t]; hi[t] = hi[t] < hi[t + w] ? hi[t + w] : hi[t]; }\n"
-                 << "        __syncthreads(); }\n"
-                 << "    if (t == 0) { k.L[3 * blockIdx.x] = ss[0] + sc[0]; k.L[3 * blockIdx.x + 1] = lo[0]; k.L[3 * blockIdx.x + 2] = hi[0]; }\n"
+                 << "    block_sum_bounds(s, a, b, wv);\n"
+                 << "    if (t == 0) { k.L[3 * blockIdx.x] = s.s + s.c; k.L[3 * blockIdx.x + 1] = a; k.L[3 * blockIdx.x + 2] = b; }\n"
                  << "}\n";
             (void)kind;
             break;
@@ -1433,8 +1463,12 @@ private:
     // through its +x face is the one its right neighbour receives, and the flux through its +z face is carried into
     // the next plane as that plane's -z flux, with the plane's values widened to double once. A face past the
     // lattice's edge reads the cell itself: kf * (c - c) = +0 for finite c and kf, and adding +0 to a sum that starts
-    // at +0 changes no bit. A NaN sum (an infinity or NaN among the inputs) flags the lane's plane, which is taken
-    // again after the tile in the exact forms of the direct gather, as the generic window does.
+    // at +0 changes no bit. A NaN sum (an infinity or NaN among the inputs) is summed again at once from the same face
+    // fluxes, the missing faces left out, as the CPU sums them.
+    // TOTAL (a step that checks the term's contracts): the tile also totals the new values and their magnitudes in a
+    // fixed order (each lane its cells in order in double, at most 4 x 64 of them; the lanes' sums folded as compensated
+    // pairs by a butterfly, the waves in order) for the "after" side of the conservation contract, tsum[2 tile..], and
+    // counts the cells outside the field's range (rec, first): the separate totals pass is not needed.
     void flux_kernels(ProgramCode& code, const std::string& cellfn, int target, int kk)
     {
         const std::vector<int> targets{target};
@@ -1453,8 +1487,9 @@ private:
             std::ostringstream s;
             s << "template <bool P2> struct " << MK << " { static constexpr bool kPow2 = P2; };\n"
               << "__shared__ __attribute__((aligned(16))) float " << LDS << "[" << 2 * (TY + 2) * 264 << "];\n"
-              << "template <bool P2, bool XH> __device__ __forceinline__ void run_" << name
-              << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK, const bool REMAP, const bool STREAM) {\n"
+              << "template <bool P2, bool XH, bool TOTAL> __device__ __forceinline__ void run_" << name
+              << "(const K& k, int zchunk, int tiles_x, int tiles_y, int tiles_z, const bool TRACK, const bool REMAP, const bool STREAM,\n"
+              << "    double* tsum, unsigned* rec, u64* first, double lo_r, double hi_r) {\n"
               << "    const int tl = REMAP ? xcd_tile((int)blockIdx.x, tiles_x * tiles_y * tiles_z) : (int)blockIdx.x;\n"
               << "    const int lane = (int)threadIdx.x; const int r = (int)threadIdx.y;\n"
               << "    const int x = (tl % tiles_x) * 256 + 4 * lane;\n"
@@ -1506,7 +1541,7 @@ private:
               << "           fzm3 = kf * ((double)lo.w - c3);\n"
               << "    int buf = 0;\n"
               << "    bool seen[1] = {false};\n"
-              << "    unsigned long long redo = 0; // planes z0 + b whose sums came out NaN\n"
+              << "    double ts = 0.0, tm = 0.0; unsigned nout = 0, nbad = 0; u64 firstbad = ~0ull;\n"
               << "    auto step = [&](const int z, const Pl& cur, const Pl& nxt, Pl& far) {\n"
               << "        load(far, z + 3);\n"
               << "        const float4 c = cur.v;\n"
@@ -1534,8 +1569,23 @@ private:
               << "            double q1 = 0.0 + (-f01); q1 += f12; q1 += kf * ((double)ym.y - c1); q1 += kf * ((double)yp.y - c1); q1 += fzm1; q1 += fzp1;\n"
               << "            double q2 = 0.0 + (-f12); q2 += f23; q2 += kf * ((double)ym.z - c2); q2 += kf * ((double)yp.z - c2); q2 += fzm2; q2 += fzp2;\n"
               << "            double q3 = 0.0 + (-f23); q3 += fxp; q3 += kf * ((double)ym.w - c3); q3 += kf * ((double)yp.w - c3); q3 += fzm3; q3 += fzp3;\n"
-              << "            const bool fine = !(q0 != q0 || q1 != q1 || q2 != q2 || q3 != q3);\n"
-              << "            if (!fine) redo |= 1ull << (z - z0);\n"
+              << "            if (q0 != q0 || q1 != q1 || q2 != q2 || q3 != q3) {\n"
+              // An infinity or NaN among the inputs: the faces the cells have, in the CPU's order, from the plane still
+              // in LDS (nothing extra kept in registers for this rare path but the incoming -z fluxes).
+              << "                const bool xl = x > 0, xr = x + 4 < k.nx, yl = y > 0, yr = y + 1 < k.ny, zl = z > 0, zr = z + 1 < k.nz;\n"
+              << "                const float4 ya = *reinterpret_cast<const float4*>(row - 264 + 4 + 4 * lane);\n"
+              << "                const float4 yb = *reinterpret_cast<const float4*>(row + 264 + 4 + 4 * lane);\n"
+              << "                const double gm = kf * ((double)row[3 + 4 * lane] - c0), gp = kf * ((double)row[8 + 4 * lane] - c3);\n"
+              << "                const double g01 = kf * (c1 - c0), g12 = kf * (c2 - c1), g23 = kf * (c3 - c2);\n"
+              << "                q0 = 0.0; if (xl) q0 += gm; q0 += g01; if (yl) q0 += kf * ((double)ya.x - c0); if (yr) q0 += kf * ((double)yb.x - c0);\n"
+              << "                if (zl) q0 += fzm0; if (zr) q0 += kf * (d0 - c0);\n"
+              << "                q1 = 0.0; q1 += -g01; q1 += g12; if (yl) q1 += kf * ((double)ya.y - c1); if (yr) q1 += kf * ((double)yb.y - c1);\n"
+              << "                if (zl) q1 += fzm1; if (zr) q1 += kf * (d1 - c1);\n"
+              << "                q2 = 0.0; q2 += -g12; q2 += g23; if (yl) q2 += kf * ((double)ya.z - c2); if (yr) q2 += kf * ((double)yb.z - c2);\n"
+              << "                if (zl) q2 += fzm2; if (zr) q2 += kf * (d2 - c2);\n"
+              << "                q3 = 0.0; q3 += -g23; if (xr) q3 += gp; if (yl) q3 += kf * ((double)ya.w - c3); if (yr) q3 += kf * ((double)yb.w - c3);\n"
+              << "                if (zl) q3 += fzm3; if (zr) q3 += kf * (d3 - c3);\n"
+              << "            }\n"
               << "            q0 = div_hh<" << MK << "<P2>>(k, q0); q1 = div_hh<" << MK << "<P2>>(k, q1);\n"
               << "            q2 = div_hh<" << MK << "<P2>>(k, q2); q3 = div_hh<" << MK << "<P2>>(k, q3);\n"
               << "            const f4v nv = {fe_ftz(c.x + fe_rf((double)fe_rf(q0) * k.dt)), fe_ftz(c.y + fe_rf((double)fe_rf(q1) * k.dt)),\n"
@@ -1545,7 +1595,21 @@ private:
               << "            if (TRACK) { asm volatile(\"\");\n"
               << "                const bool ch = __float_as_uint(nv.x) != __float_as_uint(c.x) || __float_as_uint(nv.y) != __float_as_uint(c.y) ||\n"
               << "                    __float_as_uint(nv.z) != __float_as_uint(c.z) || __float_as_uint(nv.w) != __float_as_uint(c.w);\n"
-              << "                seen[0] = seen[0] || (ch && fine); }\n"
+              << "                seen[0] = seen[0] || ch; }\n"
+              << "            if (TOTAL) {\n"
+              << "                const double v0 = (double)nv.x, v1 = (double)nv.y, v2 = (double)nv.z, v3 = (double)nv.w;\n"
+              << "                ts += v0; tm += fabs(v0); ts += v1; tm += fabs(v1); ts += v2; tm += fabs(v2); ts += v3; tm += fabs(v3);\n"
+              // out of range or not finite (the rare path: counted, and the least index kept)
+              << "                const bool b0 = !(v0 >= lo_r && v0 <= hi_r), b1 = !(v1 >= lo_r && v1 <= hi_r), b2 = !(v2 >= lo_r && v2 <= hi_r),\n"
+              << "                           b3 = !(v3 >= lo_r && v3 <= hi_r);\n"
+              << "                if (b0 || b1 || b2 || b3) {\n"
+              << "                    const double vs[4] = {v0, v1, v2, v3};\n"
+              << "                    const bool bs[4] = {b0, b1, b2, b3};\n"
+              << "#pragma unroll\n"
+              << "                    for (int q = 0; q < 4; ++q) {\n"
+              << "                        if (!bs[q]) continue;\n"
+              << "                        if (finite_d(vs[q])) ++nout; else ++nbad;\n"
+              << "                        const u64 i = (u64)z * (u64)k.nxy + (u64)y * (u64)k.nx + (u64)(x + q); firstbad = i < firstbad ? i : firstbad; } } }\n"
               << "            fzm0 = -fzp0; fzm1 = -fzp1; fzm2 = -fzp2; fzm3 = -fzp3;\n"
               << "        }\n"
               << "        buf ^= 1; c0 = d0; c1 = d1; c2 = d2; c3 = d3;\n"
@@ -1555,16 +1619,38 @@ private:
               << "    int z = z0;\n"
               << "    for (; z + 4 <= z1; z += 4) { step(z, A, B, D); step(z + 1, B, C, A); step(z + 2, C, D, B); step(z + 3, D, A, C); }\n"
               << "    if (z < z1) { step(z, A, B, D); if (z + 1 < z1) { step(z + 1, B, C, A); if (z + 2 < z1) step(z + 2, C, D, B); } }\n"
-              << redo_body(cellfn, 4, targets, false)
               << "    if (TRACK) mark(k, " << t << ", seen[0]);\n"
+              << "    if (TOTAL) {\n"
+              << "        if (nout) atomicAdd(rec, nout);\n"
+              << "        if (nbad) atomicAdd(rec + 1, nbad);\n"
+              << "        if (firstbad != ~0ull) atomicMin(first, firstbad);\n"
+              << "        __shared__ double wv[" << WAVES << "][4];\n"
+              << "        Sum ps = {ts, 0.0}, pm = {tm, 0.0};\n"
+              << "        wave_fold(ps); wave_fold(pm);\n"
+              << "        if (lane == 0) { wv[r][0] = ps.s; wv[r][1] = ps.c; wv[r][2] = pm.s; wv[r][3] = pm.c; }\n"
+              << "        __syncthreads();\n"
+              << "        if (lane == 0 && r == 0) {\n"
+              << "            Sum a = {wv[0][0], wv[0][1]}, b = {wv[0][2], wv[0][3]};\n"
+              << "            for (int q = 1; q < " << WAVES << "; ++q) { merge(a, Sum{wv[q][0], wv[q][1]}); merge(b, Sum{wv[q][2], wv[q][3]}); }\n"
+              << "            tsum[2 * tl] = a.s + a.c; tsum[2 * tl + 1] = b.s + b.c; } }\n"
               << "}\n";
-            s << "extern \"C\" __global__ __launch_bounds__(" << 64 * WAVES << ") void " << name
-              << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int mode) {\n"
-              << "    const bool tr = (mode & 1) != 0, rm = (mode & 2) != 0, st = (mode & 4) != 0;\n"
-              << "    if ((k.flags & 3) == 3) { if (tiles_x > 1) run_" << name << "<true, true>(k, zchunk, tiles_x, tiles_y, tiles_z, tr, rm, st);\n"
-              << "        else run_" << name << "<true, false>(k, zchunk, tiles_x, tiles_y, tiles_z, tr, rm, st); }\n"
-              << "    else { if (tiles_x > 1) run_" << name << "<false, true>(k, zchunk, tiles_x, tiles_y, tiles_z, tr, rm, st);\n"
-              << "        else run_" << name << "<false, false>(k, zchunk, tiles_x, tiles_y, tiles_z, tr, rm, st); }\n}\n";
+            // Four kernels: rows within one tile or spanning several (the second loads halo columns, which costs
+            // registers the first should not pay for: name + "x"), each as the step and as the step that also takes
+            // the contract's totals (name + "_t").
+            for (const bool xh : {false, true}) {
+                for (const bool total : {false, true}) {
+                    s << "extern \"C\" __global__ __launch_bounds__(" << 64 * WAVES << ")" << (total ? " __attribute__((amdgpu_waves_per_eu(4)))" : "") << " void " << name << (xh ? "x" : "") << (total ? "_t" : "")
+                      << "(K k, int zchunk, int tiles_x, int tiles_y, int tiles_z, int mode, double* tsum, unsigned* rec, u64* first, double lo,\n"
+                      << "    double hi) {\n"
+                      << "    const bool tr = (mode & 1) != 0, rm = (mode & 2) != 0, st = (mode & 4) != 0;\n";
+                    for (const bool p2 : {true, false}) {
+                        s << "    " << (p2 ? "if ((k.flags & 3) == 3) " : "else ") << "run_" << name << "<" << (p2 ? "true" : "false") << ", "
+                          << (xh ? "true" : "false") << ", " << (total ? "true" : "false")
+                          << ">(k, zchunk, tiles_x, tiles_y, tiles_z, tr, rm, st, tsum, rec, first, lo, hi);\n";
+                    }
+                    s << "}\n";
+                }
+            }
             out_ << s.str();
             WindowKernel w;
             w.name = name;

@@ -112,8 +112,8 @@ struct Impl final : HipBackend {
     std::vector<std::unique_ptr<Buffer>> now_buf, next_buf;
     std::vector<std::unique_ptr<Buffer>> guard_copy; // a guarded term's fields before its learned scheme ran
     std::vector<void*> now, nxt;
-    Buffer P, U, A, S, L, C, N, W;
-    Buffer bounds_keys; // 2 u32 a field: the least and greatest order key (k_bounds)
+    Buffer P, U, A, S, L, N, W;
+    Buffer tile_sums;   // a flux window's per-tile totals on a contract step (2 doubles a tile)
     // The contract records, one allocation read back in one copy: range counts (two u32 a field), the first bad
     // cell (a u64 a field), then the conserved totals (kTotals doubles); `judged` the pinned host copy.
     struct View {
@@ -124,9 +124,12 @@ struct Impl final : HipBackend {
             return static_cast<T*>(p);
         }
     };
+    // The front of that allocation holds the change flags (a u32 a field) and the bounds keys (two u32 a field), so that a
+    // step's readback is one copy of a prefix: [flags][keys][records][firsts][totals].
     Buffer contract;
-    View records, firsts, totals;
+    View C, bounds_keys, records, firsts, totals;
     void* judged = nullptr;
+    bool judged_has_flags = false; // the last copy into `judged` took the change flags and bounds keys as they are now
     // `half` nets on the matrix cores: their packed weights, and the row, output and face buffers their stages share
     // (a chunk of kHalfChunk cells at a time).
     static constexpr long long kHalfChunk = 1LL << 20;
@@ -230,8 +233,30 @@ struct Impl final : HipBackend {
         ++prof.launches;
     }
 
-    // A term's gather: a thread a cell, or (window kernels) z-marching tiles of 64 x ty cells.
-    void launch_gather(const hip::ProgramCode& code, double dt)
+    // A conserved field's "after" total and range taken by a flux-carrying window as it writes the field (the
+    // contract step's last gather): where the tile sums go, and the range record and the field's bounds.
+    struct FusedTotal {
+        int field = -1;
+        int record = -1;
+        long long tiles = 0;
+    };
+
+    // The flux-carrying window the gather would launch for `code`, when it can take the totals of `field` on the way:
+    // the program writes that field alone, and the field is not clamped (a clamp would have to come between the total
+    // and the store). nullptr otherwise.
+    const hip::WindowKernel* fusable(const ir::Program& p, const hip::ProgramCode& code, int field) const
+    {
+        const hip::WindowKernel* w = window_for(code);
+        if (w == nullptr || w->flux == 0 || p.writes.size() != 1 || p.writes[0] != field || law->fields[static_cast<std::size_t>(field)].clamp) {
+            return nullptr;
+        }
+        return w;
+    }
+
+    // A term's gather: a thread a cell, or (window kernels) z-marching tiles of 64 x ty cells. With `fused` (a flux
+    // window the caller checked with fusable) the gather also leaves the tiles' totals in tile_sums and counts the
+    // range into its record.
+    void launch_gather(const hip::ProgramCode& code, double dt, FusedTotal* fused = nullptr)
     {
         const hip::WindowKernel* w = window_for(code);
         last_gather = w == nullptr ? code.gather : w->name;
@@ -252,9 +277,35 @@ struct Impl final : HipBackend {
         const bool remap = remap_override >= 0 ? remap_override != 0 : fits;
         const bool stream = stream_override >= 0 ? stream_override != 0 : !fits;
         int mode = (tracking ? 1 : 0) | (remap ? 2 : 0) | (stream ? 4 : 0);
-        void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz, &mode};
-        check(fe_hip_launch(fn(w->name), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(w->block_y), 1, 0, argv),
-            ("launch " + w->name).c_str());
+        if (w->flux == 0) {
+            void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz, &mode};
+            check(fe_hip_launch(fn(w->name), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(w->block_y), 1, 0, argv),
+                ("launch " + w->name).c_str());
+            ++prof.launches;
+            return;
+        }
+        // Flux windows: rows spanning several tiles take the kernel that loads halo columns; a contract step's may take
+        // the totals.
+        double* tsum = nullptr;
+        unsigned* rec = nullptr;
+        unsigned long long* first = nullptr;
+        double lo = 0.0, hi = 0.0;
+        if (fused != nullptr) {
+            if (tile_sums.bytes < static_cast<std::size_t>(total) * 16) {
+                tile_sums.allocate(static_cast<std::size_t>(total) * 16);
+            }
+            tsum = tile_sums.as<double>();
+            rec = records.as<unsigned>() + 2 * fused->record;
+            first = firsts.as<unsigned long long>() + fused->record;
+            const FieldDecl& d = law->fields[static_cast<std::size_t>(fused->field)];
+            lo = d.lo;
+            hi = d.hi;
+            fused->tiles = total;
+        }
+        const std::string name = w->name + (tx > 1 ? "x" : "") + (fused != nullptr ? "_t" : "");
+        void* argv[] = {k.data(), &zchunk, &tx, &ty, &tz, &mode, &tsum, &rec, &first, &lo, &hi};
+        check(fe_hip_launch(fn(name), static_cast<unsigned>(total), 1, 1, 64, static_cast<unsigned>(w->block_y), 1, 0, argv),
+            ("launch " + name).c_str());
         ++prof.launches;
     }
 
@@ -417,8 +468,6 @@ struct Impl final : HipBackend {
         }
         S.allocate(tridiag ? 2 * cells * 8 : 8);
         L.allocate(static_cast<std::size_t>(3 * kReduceBlocks) * 8);
-        C.allocate(std::max<std::size_t>(1, nf) * 4);
-        bounds_keys.allocate(std::max<std::size_t>(1, 2 * nf) * 4);
         std::vector<std::size_t> net_offsets;
         net_host = hip::net_blob(*law, net_offsets);
         N.allocate(std::max<std::size_t>(1, net_host.size()) * 4);
@@ -454,15 +503,19 @@ struct Impl final : HipBackend {
         half_outs.allocate(out_bytes);
         half_faces.allocate(faces ? cells * 3 * 4 : 0);
         const std::size_t rec_bytes = static_cast<std::size_t>(kRanges) * 8, first_bytes = static_cast<std::size_t>(kRanges) * 8;
-        contract.allocate(rec_bytes + first_bytes + static_cast<std::size_t>(kTotals) * 8);
-        records.p = contract.p;
-        firsts.p = static_cast<char*>(contract.p) + rec_bytes;
-        totals.p = static_cast<char*>(contract.p) + rec_bytes + first_bytes;
+        const std::size_t flag_bytes = (std::max<std::size_t>(1, nf) * 4 + 7) / 8 * 8, key_bytes = std::max<std::size_t>(1, nf) * 8;
+        contract.allocate(flag_bytes + key_bytes + rec_bytes + first_bytes + static_cast<std::size_t>(kTotals) * 8);
+        C.p = contract.p;
+        bounds_keys.p = static_cast<char*>(contract.p) + flag_bytes;
+        records.p = static_cast<char*>(contract.p) + flag_bytes + key_bytes;
+        firsts.p = static_cast<char*>(records.p) + rec_bytes;
+        totals.p = static_cast<char*>(firsts.p) + first_bytes;
         if (judged != nullptr) {
             fe_hip_host_free(judged);
             judged = nullptr;
         }
         check(fe_hip_host_alloc(&judged, contract.bytes), "pinned allocation");
+
         stale.assign(nf, 0);
         known.assign(nf, Bounds{});
     }
@@ -867,9 +920,15 @@ struct Impl final : HipBackend {
 
     void reset_records()
     {
-        check(fe_hip_memset_d32_async(records.p, 0u, static_cast<std::size_t>(kRanges) * 2), "records reset");
-        check(fe_hip_memset_d32_async(firsts.p, 0xffffffffu, static_cast<std::size_t>(kRanges) * 2), "records reset");
+        fill_words(records.as<unsigned>(), 2 * kRanges, 2 * kRanges);
         next_record = 0;
+    }
+    // `zeros` words from p to 0, then `ones` to all ones, in one launch.
+    void fill_words(unsigned* p, int zeros, int ones)
+    {
+        void* argv[] = {&p, &zeros, &ones};
+        check(fe_hip_launch(fn("k_fill"), static_cast<unsigned>((zeros + ones + 255) / 256), 1, 1, 256, 1, 1, 0, argv), "launch k_fill");
+        ++prof.launches;
     }
     int next_record = 0; // the next free range record since the records were last reset
 
@@ -932,11 +991,21 @@ struct Impl final : HipBackend {
                 total_slot += 4 * static_cast<int>(group.size());
             }
         }
+        // A conduction term conserving the one field it writes: its window takes the "after" total on the last substep.
+        FusedTotal fused;
+        const bool fuse = contracts && p.conserves.size() == 1 && p.conserves[0].size() == 1 &&
+            fusable(p, code, p.conserves[0][0]) != nullptr;
         for (int s = 0; s < substeps; ++s) {
             for (const int st : code.stages) {
                 run_stage(st, sub_dt);
             }
-            launch_gather(code, sub_dt);
+            if (fuse && s + 1 == substeps) {
+                fused.field = p.conserves[0][0];
+                fused.record = next_record++;
+                launch_gather(code, sub_dt, &fused);
+            } else {
+                launch_gather(code, sub_dt);
+            }
             swap(code.swaps);
         }
         for (const int f : p.writes) {
@@ -954,6 +1023,19 @@ struct Impl final : HipBackend {
                 for (int q = 0; q < m; ++q) {
                     const int f = group[static_cast<std::size_t>(q)];
                     const bool writes = std::find(p.writes.begin(), p.writes.end(), f) != p.writes.end() && ranged.count(f) == 0;
+                    if (f == fused.field) {
+                        // taken by the window: its tiles' totals folded in order
+                        double* parts = tile_sums.as<double>();
+                        int count = static_cast<int>(fused.tiles);
+                        double* dst = totals.as<double>() + slot + 2 * m + 2 * q;
+                        void* fargs[] = {&parts, &count, &dst};
+                        check(fe_hip_launch(fn("k_total_final"), 1, 1, 1, 256, 1, 1, 0, fargs), "total final launch");
+                        ++prof.launches;
+                        note_total(f, slot + 2 * m + 2 * q);
+                        j.ranged.emplace_back(f, fused.record);
+                        ranged.insert(f);
+                        continue;
+                    }
                     const int record = writes ? next_record++ : -1;
                     total(f, slot + 2 * m + 2 * q, record);
                     note_total(f, slot + 2 * m + 2 * q);
@@ -1004,8 +1086,9 @@ struct Impl final : HipBackend {
         }
         const std::size_t head = static_cast<std::size_t>(static_cast<char*>(totals.p) - static_cast<char*>(contract.p));
         check(fe_hip_memcpy_dtoh_async(judged, contract.p, head + static_cast<std::size_t>(used) * 8), "records readback");
+        judged_has_flags = true; // the copy began at the change flags and bounds keys
         check(fe_hip_stream_sync(), "sync");
-        const unsigned* rec = static_cast<const unsigned*>(judged);
+        const unsigned* rec = reinterpret_cast<const unsigned*>(static_cast<const char*>(judged) + (static_cast<char*>(records.p) - static_cast<char*>(contract.p)));
         const unsigned long long* first =
             reinterpret_cast<const unsigned long long*>(static_cast<const char*>(judged) + (static_cast<char*>(firsts.p) - static_cast<char*>(contract.p)));
         const double* tot = reinterpret_cast<const double*>(static_cast<const char*>(judged) + head);
@@ -1458,7 +1541,7 @@ struct Impl final : HipBackend {
         const bool read_flags = !lazy || contracts;
         tracking = read_flags;
         if (read_flags) {
-            check(fe_hip_memset_d32_async(C.p, 0u, std::max<std::size_t>(1, nf)), "change flags reset");
+            fill_words(C.as<unsigned>(), static_cast<int>(std::max<std::size_t>(1, nf)), 0);
         }
         std::vector<Judge> judges;
         int total_slot = 0;
@@ -1564,7 +1647,6 @@ struct Impl final : HipBackend {
         known = flowing;
         const bool measure = !lazy || (lazy_steps++ % kTightenEvery) == 0;
         std::vector<int> measured;
-        std::vector<unsigned> keys;
         if (measure) {
             for (std::size_t f = 0; f < nf; ++f) {
                 if (wrote[f]) {
@@ -1573,13 +1655,22 @@ struct Impl final : HipBackend {
             }
             if (!measured.empty()) {
                 queue_measure(measured);
-                keys.assign(2 * nf, 0u);
-                check(fe_hip_memcpy_dtoh_async(keys.data(), bounds_keys.p, keys.size() * 4), "bounds readback");
             }
         }
+        // One copy and one wait for all the step reads back: the change flags and the bounds keys lead the contract
+        // buffer, so the judge's copy of the records and totals brings them along; without contracts, that prefix alone.
+        judged_has_flags = false;
         if (contracts) {
             judge(judges, violations);
         }
+        if ((read_flags || !measured.empty()) && !judged_has_flags) {
+            const std::size_t prefix = static_cast<std::size_t>(static_cast<char*>(records.p) - static_cast<char*>(contract.p));
+            check(fe_hip_memcpy_dtoh_async(judged, contract.p, prefix), "change flags readback");
+            check(fe_hip_stream_sync(), "sync");
+        }
+        const unsigned* back = static_cast<const unsigned*>(judged);
+        const unsigned* keys_back = reinterpret_cast<const unsigned*>(static_cast<const char*>(judged) +
+            (static_cast<char*>(bounds_keys.p) - static_cast<char*>(contract.p)));
         if (!read_flags) {
             // Nothing comes back and nothing is waited for: every field a program writes is taken as changed, and the
             // card's time is read from the events when the profile is asked for.
@@ -1592,11 +1683,13 @@ struct Impl final : HipBackend {
                 }
             }
         } else {
-            read_changed(changed);
+            changed.assign(nf, 0);
+            for (std::size_t f = 0; f < nf; ++f) {
+                changed[f] = back[f] != 0 ? 1 : 0;
+            }
         }
         if (!measured.empty()) {
-            check(fe_hip_stream_sync(), "sync");
-            take_measure(measured, keys);
+            take_measure(measured, std::vector<unsigned>(keys_back, keys_back + 2 * nf));
         }
         for (std::size_t f = 0; f < nf; ++f) {
             stale[f] = stale[f] || changed[f];
