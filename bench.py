@@ -324,6 +324,64 @@ def black_scholes(steps: int):
     return out
 
 
+IMPLICIT_X_LAW = """field T init 0 range -1000000 1000000
+param k = 0.01
+term crank_nicolson
+  let a = 0.5 * k * dt / (h * h)
+  d T = (tridiag_x(0 - a, 1 + 2 * a, 0 - a, T + a * (shift(T, 1, 0, 0) - 2 * T + shift(T, -1, 0, 0))) - T) / dt
+"""
+
+IMPLICIT_YZ_LAW = """field T init 0 range -1000000 1000000
+field w init 1 range 0 10 static
+param k = 0.05
+term implicit_yz
+  let a = k * w * dt / (h * h)
+  let y1 = tridiag_y(0 - a, 1 + 2 * a, 0 - a, T)
+  d T = (tridiag_z(0 - 0.5 * a, 1 + a, 0 - 0.5 * a, y1) - T) / dt
+"""
+
+
+def implicit_start(n: int):
+    """The implicit-lines start: T seeded uniform [0, 1), w uniform [0.5, 2)."""
+    rng = np.random.default_rng(11)
+    return rng.random(n ** 3, dtype=np.float32), (0.5 + 1.5 * rng.random(n ** 3)).astype(np.float32)
+
+
+def implicit_lines(n: int, steps: int, check_steps: int = 3):
+    """Laws whose step solves a tridiagonal system along every lattice line (Thomas in double, the reference CPU's
+    arithmetic): Crank-Nicolson along x (one number a coefficient, so c' stays on chip) and an implicit y-then-z step
+    with a field-dependent operator, on n^3. Each is held to the reference CPU engine's bits after check_steps steps,
+    then timed through Solver.run."""
+    h = 1.0 / n
+    t0, w0 = implicit_start(n)
+    out = {}
+    for name, law, dt, fields in (("crank_nicolson_x", IMPLICIT_X_LAW, 0.5 * h * h / 0.01, {"T": t0}),
+                                  ("implicit_yz", IMPLICIT_YZ_LAW, 0.5 * h * h / 0.05, {"T": t0, "w": w0})):
+        s = field_hip.Solver(field_hip.Law(law), (n, n, n), h)
+        for f, v in fields.items():
+            s.set_field(f, v)
+        s.refresh_derived()
+        s.set_lazy_readback(True)
+        s.check_contracts_every(1 << 30)
+        for _ in range(check_steps):
+            s.step(dt)
+        rec = reference_check(f"{name}_{n}", s.get_field("T"))
+        per_term = []
+        for _ in range(5):
+            s.step(dt)
+            per_term.append(s.profile()["terms"][0]["ms"])
+        s.run(dt, steps)
+        field_hip.sync()
+        ev = field_hip.Events()
+        ev.begin()
+        s.run(dt, steps)
+        ms = ev.end() / steps
+        cells = n ** 3
+        out[name] = {"cells": cells, "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+                     "term_ms": float(np.median(per_term)), "gbs_8B": 8.0 * cells / (ms * 1e6), **rec}
+    return out
+
+
 def reference_check(name: str, final: np.ndarray) -> dict:
     """The card's field against the reference CPU engine's on the same law, start and steps: tests/golden/
     bench_reference.json holds the SHA-256 of the reference's final field bits and its time a step, both taken in the
@@ -521,12 +579,25 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the Dirichlet, Black-Scholes, learned and adjoint cases")
     ap.add_argument("--learned-steps", type=int, default=3000)
     ap.add_argument("--adjoint-size", type=int, default=256)
+    ap.add_argument("--section", help="run one of the other cases alone and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         return 2
     info = field_hip.require_device()
     log(f"device {info.name.decode()} ({info.arch.decode()}, {info.compute_units} CUs)")
+
+    sections = {"dirichlet_heat_64": dirichlet_heat,
+                "black_scholes_4096x512": lambda: black_scholes(200),
+                "implicit_lines_256": lambda: implicit_lines(256, 50),
+                "learned_coarse": lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4),
+                "adjoint_transport": lambda: adjoint_transport(args.adjoint_size, not args.no_cpu)}
+    if args.section:
+        if args.section not in sections:
+            print(f"no case {args.section!r} (cases: {', '.join(sections)})", file=sys.stderr)
+            return 2
+        print(json.dumps({args.section: sections[args.section]()}, default=float), flush=True)
+        return 0
 
     main_case = conduction(args.size, args.steps, args.warmup)
     log(f"{args.size}^3: {json.dumps(main_case)}")
@@ -542,10 +613,7 @@ def main() -> int:
         log(f"heston adi: {json.dumps(adi)}")
     extra = {}
     if not args.no_extra:
-        for name, fn in (("dirichlet_heat_64", dirichlet_heat),
-                         ("black_scholes_4096x512", lambda: black_scholes(200)),
-                         ("learned_coarse", lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4)),
-                         ("adjoint_transport", lambda: adjoint_transport(args.adjoint_size, not args.no_cpu))):
+        for name, fn in sections.items():
             t = time.time()
             extra[name] = fn()
             log(f"{name} ({time.time() - t:.1f} s): {json.dumps(extra[name], default=float)}")

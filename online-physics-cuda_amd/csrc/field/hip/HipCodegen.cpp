@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <set>
+#include <functional>
 #include <sstream>
 
 namespace fieldengine::field::hip {
@@ -447,6 +448,7 @@ private:
             } else if (in.op == Op::Tridiag) {
                 s.kind = StageCode::Kind::Tridiag;
                 s.axis = in.index;
+                s.uniform = scalar(in.args[0]) && scalar(in.args[1]) && scalar(in.args[2]);
                 lay_.slot[static_cast<std::size_t>(id)] = lay_.aux_slots++;
             } else if ((in.op == Op::Mlp || in.op == Op::Flux || in.op == Op::LDiffuse) && in.shape == ir::Shape::Cell &&
                 law_.nets[static_cast<std::size_t>(in.index)].half) {
@@ -1040,31 +1042,9 @@ private:
             (void)kind;
             break;
         }
-        case StageCode::Kind::Tridiag: {
-            // A thread a line: Thomas in double with the edge cell repeating past the ends (its neighbour's coefficient
-            // on the diagonal), as the CPU solves it (src/field/Law.cpp:1098-1138); c' and d' in the scratch S.
-            static const char* lines[3] = {"k.ny * k.nz", "k.nx * k.nz", "k.nx * k.ny"};
-            static const char* coords[3] = {"int x = 0; int y = (int)(line % k.ny); int z = (int)(line / k.ny); const int n = k.nx;",
-                "int x = (int)(line % k.nx); int y = 0; int z = (int)(line / k.nx); const int n = k.ny;",
-                "int x = (int)(line % k.nx); int y = (int)(line / k.nx); int z = 0; const int n = k.nz;"};
-            const char* c = s.axis == 0 ? "x" : (s.axis == 1 ? "y" : "z");
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) {\n    const Direct m(k);\n"
-                 << "    const i64 line = (i64)blockIdx.x * blockDim.x + threadIdx.x; if (line >= (i64)" << lines[s.axis] << ") return;\n"
-                 << "    " << coords[s.axis] << "\n"
-                 << "    double cp = 0.0; double dp = 0.0;\n"
-                 << "    for (int i = 0; i < n; ++i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
-                 << "        const double a = " << d(in.args[0]) << "; const double u = " << d(in.args[2]) << ";\n"
-                 << "        const double b = " << d(in.args[1]) << " + (i == 0 ? a : 0.0) + (i + 1 == n ? u : 0.0);\n"
-                 << "        const double piv = i == 0 ? b : b - a * cp;\n"
-                 << "        cp = i + 1 == n ? 0.0 : u / piv;\n"
-                 << "        dp = (" << d(in.args[3]) << " - (i == 0 ? 0.0 : a * dp)) / piv;\n"
-                 << "        k.S[q] = cp; k.S[k.cells + q] = dp; }\n"
-                 << "    double next = 0.0;\n"
-                 << "    for (int i = n - 1; i >= 0; --i) { " << c << " = i; const i64 q = IDX(x, y, z);\n"
-                 << "        next = k.S[k.cells + q] - k.S[q] * next; AT(" << slot << ", x, y, z) = fe_rf(next); }\n"
-                 << "}\n";
+        case StageCode::Kind::Tridiag:
+            tridiag_kernels(s, in, slot);
             break;
-        }
         case StageCode::Kind::Distance:
             distance_kernels(s, in, slot);
             break;
@@ -1075,6 +1055,234 @@ private:
             net_half_kernels(s, in, slot);
             break;
         }
+    }
+
+    // tridiag_x|y|z: a thread a line, Thomas in double with the edge cell repeating past the ends (its neighbour's
+    // coefficient on the diagonal), as the CPU solves it (src/field/Law.cpp:1098-1138): the same operations in the same
+    // order, so each line's answer is the CPU's to the bit. 64 lines a block, so a 256^3 lattice has one wave a SIMD
+    // and nothing else to hide memory latency behind: every sweep fetches its next operands into registers while it
+    // runs the current ones (double buffering).
+    //  - Lines along y or z: neighbouring threads take neighbouring x, so each step's reads and writes are one coalesced
+    //    row; runs of kTridiagRun steps; d' (and c') in the scratch S at the cells' own indices.
+    //  - Lines along x: staged through LDS 32 cells at a time. The block's 64 threads read the cell-shaped coefficients
+    //    of its 64 rows with lanes along x (coalesced; floats: the CPU widens each cell's float), then each thread walks
+    //    its own row from LDS; d' (and c') go to S interleaved by line (S[i * lines + line]: contiguous across the
+    //    threads), and a chunk's answers leave through LDS the same way.
+    //  - a, b and u one number each (uniform): c' is the same for every line, so thread 0's is kept in LDS for the
+    //    backward sweep and only d' goes through S. That kernel is for lines up to kTridiagLdsLine cells (the launch
+    //    gives the LDS); <kernel>_s, the general form, takes longer ones.
+    static constexpr int kTridiagRun = 16;
+
+    void tridiag_kernels(const StageCode& s, const ir::Inst& in, int slot)
+    {
+        tridiag_kernel(s, in, slot, s.uniform);
+        if (s.uniform) {
+            tridiag_kernel(s, in, slot, false);
+        }
+    }
+
+    void tridiag_kernel(const StageCode& s, const ir::Inst& in, int slot, bool u)
+    {
+        const char* names[4] = {"a", "b", "u", "r"};
+        std::vector<int> cell; // the cell-shaped coefficients (by position)
+        std::ostringstream fetch;
+        for (int q = 0; q < 4; ++q) {
+            const int arg = in.args[static_cast<std::size_t>(q)];
+            if (scalar(arg)) {
+                fetch << "    const double s_" << names[q] << " = " << d(arg) << ";\n";
+            } else {
+                cell.push_back(q);
+            }
+        }
+        auto val = [&](int q, const std::string& at) {
+            return scalar(in.args[static_cast<std::size_t>(q)]) ? "s_" + std::string(names[q]) : "(double)" + at;
+        };
+        // One forward step at i, the cell-shaped coefficients read through `at`; c' to LDS (thread 0) or S[cq].
+        auto forward = [&](const std::function<std::string(int)>& at, const std::string& cq, const std::string& pad) {
+            std::ostringstream f;
+            f << pad << "const double a = " << val(0, at(0)) << "; const double uu = " << val(2, at(2)) << ";\n"
+              << pad << "const double b = " << val(1, at(1)) << " + (i == 0 ? a : 0.0) + (i + 1 == n ? uu : 0.0);\n"
+              << pad << "const double piv = i == 0 ? b : b - a * cp;\n"
+              << pad << "cp = i + 1 == n ? 0.0 : uu / piv;\n"
+              << pad << "dp = (" << val(3, at(3)) << " - (i == 0 ? 0.0 : a * dp)) / piv;\n"
+              << pad << (u ? std::string("if (t == 0) t_cp[i] = cp;") : "k.S[" + cq + "] = cp;") << " k.S[k.cells + " << cq << "] = dp;\n";
+            return f.str();
+        };
+        // Register arrays: `prefix`<name>[len] for each cell-shaped coefficient / for d' (and c' unless uniform).
+        auto coef_arrays = [&](const std::string& prefix, const std::string& len) {
+            std::string r;
+            for (int q : cell) {
+                r += "float " + prefix + names[q] + "[" + len + "]; ";
+            }
+            return r;
+        };
+        auto coef_load = [&](const std::string& prefix, const std::string& x, const std::string& y, const std::string& z) {
+            std::string r;
+            for (int q : cell) {
+                r += " " + prefix + names[q] + "[j] = " + v(in.args[static_cast<std::size_t>(q)], x, y, z) + ";";
+            }
+            return r;
+        };
+        auto coef_zero = [&](const std::string& prefix) {
+            std::string r;
+            for (int q : cell) {
+                r += " " + prefix + names[q] + "[j] = 0.0f;";
+            }
+            return r;
+        };
+        auto coef_move = [&](const std::string& to, const std::string& from) {
+            std::string r;
+            for (int q : cell) {
+                r += " " + to + names[q] + "[j] = " + from + names[q] + "[j];";
+            }
+            return r;
+        };
+        auto sweep_arrays = [&](const std::string& prefix, const std::string& len) {
+            return "double " + prefix + "d[" + len + "]; " + (u ? std::string() : "double " + prefix + "c[" + len + "]; ");
+        };
+        auto sweep_load = [&](const std::string& prefix, const std::string& q) {
+            return prefix + "d[j] = k.S[k.cells + " + q + "];" + (u ? std::string() : " " + prefix + "c[j] = k.S[" + q + "];");
+        };
+        auto sweep_zero = [&](const std::string& prefix) {
+            return prefix + "d[j] = 0.0;" + (u ? std::string() : " " + prefix + "c[j] = 0.0;");
+        };
+        auto sweep_move = [&](const std::string& to, const std::string& from) {
+            return to + "d[j] = " + from + "d[j];" + (u ? std::string() : " " + to + "c[j] = " + from + "c[j];");
+        };
+        auto cp_of = [&](const std::string& prefix) { return u ? std::string("t_cp[i]") : prefix + "c[j]"; };
+
+        out_ << "extern \"C\" __global__ __launch_bounds__(64) void " << s.kernel << (s.uniform && !u ? "_s" : "")
+             << "(K k) {\n    const Direct m(k);\n";
+        if (u) {
+            out_ << "    extern __shared__ double t_cp[];\n";
+        }
+        out_ << fetch.str() << "    const int t = (int)threadIdx.x;\n";
+        const std::string R = std::to_string(kTridiagRun);
+        if (s.axis != 0) {
+            const bool ya = s.axis == 1;
+            const std::string yi = ya ? "i" : "y", zi = ya ? "z" : "i";
+            const std::string at = "IDX(x, " + yi + ", " + zi + ")";
+            out_ << "    const i64 lines = (i64)k.nx * " << (ya ? "k.nz" : "k.ny") << "; const i64 line = (i64)blockIdx.x * 64 + t;\n"
+                 << "    const bool live = line < lines; const int n = " << (ya ? "k.ny" : "k.nz") << ";\n"
+                 << "    const int x = (int)(line % k.nx); const int " << (ya ? "z" : "y") << " = (int)(line / k.nx);\n"
+                 << "    double cp = 0.0; double dp = 0.0;\n"
+                 << "    if (live) {\n"
+                 << "        " << coef_arrays("f_", R) << coef_arrays("n_", R) << "\n"
+                 << "#pragma unroll\n"
+                 << "        for (int j = 0; j < " << R << "; ++j) { const int i = j; if (i < n) {" << coef_load("f_", "x", yi, zi)
+                 << " } else {" << coef_zero("f_") << " } }\n"
+                 << "        for (int i0 = 0; i0 < n; i0 += " << R << ") {\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) { const int i = i0 + " << R << " + j; if (i < n) {"
+                 << coef_load("n_", "x", yi, zi) << " } else {" << coef_zero("n_") << " } }\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) { const int i = i0 + j; if (i < n) { const i64 q = " << at << ";\n"
+                 << forward([&](int q) { return "f_" + std::string(names[q]) + "[j]"; }, "q", "                ")
+                 << "            } }\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) {" << coef_move("f_", "n_") << " }\n"
+                 << "        }\n"
+                 << "    }\n";
+            if (u) {
+                out_ << "    __syncthreads();\n";
+            }
+            out_ << "    if (live) {\n"
+                 << "        double next = 0.0;\n"
+                 << "        " << sweep_arrays("g_", R) << sweep_arrays("h_", R) << "\n"
+                 << "#pragma unroll\n"
+                 << "        for (int j = 0; j < " << R << "; ++j) { const int i = n - 1 - j; if (i >= 0) { const i64 q = " << at << "; "
+                 << sweep_load("g_", "q") << " } else { " << sweep_zero("g_") << " } }\n"
+                 << "        for (int i1 = n - 1; i1 >= 0; i1 -= " << R << ") {\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) { const int i = i1 - " << R << " - j; if (i >= 0) { const i64 q = " << at
+                 << "; " << sweep_load("h_", "q") << " } else { " << sweep_zero("h_") << " } }\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) { const int i = i1 - j; if (i >= 0) { next = g_d[j] - " << cp_of("g_")
+                 << " * next; AT(" << slot << ", x, " << yi << ", " << zi << ") = fe_rf(next); } }\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < " << R << "; ++j) { " << sweep_move("g_", "h_") << " }\n"
+                 << "        }\n"
+                 << "    }\n}\n";
+            return;
+        }
+        // Lines along x. A thread reads for the staging the cells (x0 + (t & 31), row (t >> 5) + 2 r), r < 32, clamped
+        // into the chunk and the block's rows so every read is in bounds (what falls outside is not stored).
+        for (int q : cell) {
+            out_ << "    __shared__ float t_" << names[q] << "[64][33];\n";
+        }
+        auto stage_load = [&](const std::string& x0, const std::string& w) {
+            return "            { const int l = min((t >> 5) + 2 * j, rows - 1); const int c = " + x0 + " + min(t & 31, " + w + " - 1);" +
+                   coef_load("g_", "c", "t_y[l]", "t_z[l]") + " }\n";
+        };
+        out_ << "    __shared__ float t_o[64][33]; __shared__ int t_y[64]; __shared__ int t_z[64];\n"
+             << "    const i64 lines = (i64)k.ny * k.nz; const i64 line0 = (i64)blockIdx.x * 64;\n"
+             << "    const i64 line = line0 + t; const bool live = line < lines; const int n = k.nx;\n"
+             << "    const int rows = lines - line0 < 64 ? (int)(lines - line0) : 64;\n"
+             << "    if (live) { t_y[t] = (int)(line % k.ny); t_z[t] = (int)(line / k.ny); }\n"
+             << "    __syncthreads();\n"
+             << "    double cp = 0.0; double dp = 0.0;\n";
+        if (!cell.empty()) {
+            out_ << "    " << coef_arrays("g_", "32") << "\n"
+                 << "#pragma unroll\n"
+                 << "    for (int j = 0; j < 32; ++j)\n" << stage_load("0", "(n < 32 ? n : 32)");
+        }
+        out_ << "    for (int x0 = 0; x0 < n; x0 += 32) {\n"
+             << "        const int w = n - x0 < 32 ? n - x0 : 32;\n"
+             << "        __syncthreads();\n";
+        if (!cell.empty()) {
+            std::string st;
+            for (int q : cell) {
+                st += " t_" + std::string(names[q]) + "[l][c] = g_" + names[q] + "[j];";
+            }
+            out_ << "#pragma unroll\n"
+                 << "        for (int j = 0; j < 32; ++j) { const int l = (t >> 5) + 2 * j, c = t & 31; if (l < rows && c < w) {" << st << " } }\n"
+                 << "        __syncthreads();\n"
+                 << "        if (x0 + 32 < n) {\n"
+                 << "            const int w2 = n - x0 - 32 < 32 ? n - x0 - 32 : 32;\n"
+                 << "#pragma unroll\n"
+                 << "            for (int j = 0; j < 32; ++j)\n" << "    " << stage_load("x0 + 32", "w2")
+                 << "        }\n";
+        }
+        out_ << "        if (live) {\n"
+             << "            for (int j = 0; j < w; ++j) {\n"
+             << "                const int i = x0 + j; const i64 q = (i64)i * lines + line;\n"
+             << forward([&](int q) { return "t_" + std::string(names[q]) + "[t][j]"; }, "q", "                ")
+             << "            }\n"
+             << "        }\n"
+             << "    }\n"
+             << "    __syncthreads();\n"
+             << "    double next = 0.0;\n"
+             << "    const int last = ((n - 1) / 32) * 32;\n"
+             << "    " << sweep_arrays("g_", "32") << (u ? sweep_arrays("h_", "32") : std::string()) << "\n"
+             << "#pragma unroll\n"
+             << "    for (int j = 0; j < 32; ++j) { const int i = last + j; if (live && i < n) { const i64 q = (i64)i * lines + line; "
+             << sweep_load("g_", "q") << " } else { " << sweep_zero("g_") << " } }\n"
+             << "    for (int x0 = last; x0 >= 0; x0 -= 32) {\n"
+             << "        const int w = n - x0 < 32 ? n - x0 : 32;\n";
+        const std::string load_prev = "for (int j = 0; j < 32; ++j) { const int i = x0 - 32 + j; if (live && x0 > 0) { const i64 q = (i64)i * lines + line; ";
+        if (u) {
+            out_ << "#pragma unroll\n"
+                 << "        " << load_prev << sweep_load("h_", "q") << " } else { " << sweep_zero("h_") << " } }\n";
+        }
+        out_ << "        if (live) {\n"
+             << "#pragma unroll\n"
+             << "            for (int j = 31; j >= 0; --j) { const int i = x0 + j; if (j < w) { next = g_d[j] - " << cp_of("g_")
+             << " * next; t_o[t][j] = fe_rf(next); } }\n"
+             << "        }\n"
+             << "        __syncthreads();\n"
+             << "#pragma unroll\n"
+             << "        for (int j = 0; j < 32; ++j) { const int l = (t >> 5) + 2 * j, c = t & 31; if (l < rows && c < w) AT(" << slot
+             << ", x0 + c, t_y[l], t_z[l]) = t_o[l][c]; }\n"
+             << "        __syncthreads();\n";
+        if (u) {
+            out_ << "#pragma unroll\n"
+                 << "        for (int j = 0; j < 32; ++j) { " << sweep_move("g_", "h_") << " }\n";
+        } else {
+            out_ << "#pragma unroll\n"
+                 << "        " << load_prev << sweep_load("g_", "q") << " } else { " << sweep_zero("g_") << " } }\n";
+        }
+        out_ << "    }\n"
+             << "}\n";
     }
 
     // A `half` net's op in three kernels around the library's MFMA forward (see StageCode::Kind::NetHalf). The inputs,

@@ -44,6 +44,7 @@ struct StageCode {
     int inst = -1;
     std::string kernel;     // the stage's kernel (Reduce: its per-block pass; the final pass is k_reduce_final)
     int axis = 0;           // Tridiag: the axis its lines run along
+    bool uniform = false;   // Tridiag: a, b and u one number each, so c' is every line's: kept in LDS (see below)
     Op reduce = Op::SumOf;
     int iterations = 0;     // Poisson: sweeps a solve
     int words = -1;         // Distance, Poisson: the first of its four state words
@@ -54,6 +55,10 @@ struct StageCode {
     int in_pad = 0;
     int outs = 0;
 };
+
+// A uniform Tridiag stage keeps c' in LDS for lines up to this long (the launch gives it 8 bytes a cell of the line);
+// longer lines take its <kernel>_s, which keeps c' in the scratch with d'.
+constexpr int kTridiagLdsLine = 4096;
 
 // A window gather (z-marching tiles, off-cell field reads from an LDS ring of planes): launched as a 1-D grid of
 // tiles_x * tiles_y * tiles_z blocks of 64 x block_y threads with (K, zchunk, tiles_x, tiles_y, tiles_z, mode: bit 0

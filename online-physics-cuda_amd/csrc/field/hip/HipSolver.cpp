@@ -729,7 +729,11 @@ struct Impl final : HipBackend {
             const long long lines = s.axis == 0 ? 1LL * shape.ny * shape.nz : (s.axis == 1 ? 1LL * shape.nx * shape.nz : 1LL * shape.nx * shape.ny);
             std::vector<std::uint64_t> k = args(dt);
             void* argv[] = {k.data()};
-            check(fe_hip_launch(fn(s.kernel), static_cast<unsigned>((lines + 63) / 64), 1, 1, 64, 1, 1, 0, argv), "tridiag launch");
+            // A uniform stage keeps c' (8 bytes a cell of a line) in LDS when the line fits; <kernel>_s otherwise.
+            const int n = s.axis == 0 ? shape.nx : (s.axis == 1 ? shape.ny : shape.nz);
+            const bool lds = s.uniform && n <= hip::kTridiagLdsLine;
+            check(fe_hip_launch(fn(s.uniform && !lds ? s.kernel + "_s" : s.kernel), static_cast<unsigned>((lines + 63) / 64), 1, 1, 64,
+                      1, 1, lds ? static_cast<unsigned>(n) * 8u : 0u, argv), "tridiag launch");
             ++prof.launches;
             break;
         }

@@ -93,6 +93,23 @@ term implicit_yz
   d T = (tridiag_z(0 - 0.5 * a, 1 + a, 0 - 0.5 * a, y1) - T) / dt
 """
 
+# Implicit lines on a lattice where x-lines span three 32-cell chunks (the last ragged) and a second, partial block of
+# 64 lines: one number a coefficient (uniform) and field-dependent ones, along x and along z.
+TRIDIAG_LINES_LAW = """# uniform and field-dependent implicit lines along x and z
+field T init 0 range -1000000 1000000
+field w init 1 range 0 10 static
+
+param k = 0.04
+
+term lines
+  let a = k * dt / (h * h)
+  let b = a * w
+  let x1 = tridiag_x(0 - a, 1 + 2 * a, 0 - a, T)
+  let x2 = tridiag_x(0 - b, 1 + 2 * b, 0 - b, x1 + 0.1 * w)
+  let z1 = tridiag_z(0 - 0.5 * b, 1 + b, 0 - 0.5 * b, x2)
+  d T = (tridiag_z(0 - 0.5 * a, 1 + a, 0 - 0.5 * a, z1) - T) / dt
+"""
+
 # Every op alone on a lattice uneven on every axis (each is its own law: A moved by the op, B static input).
 OPS = {
     "lap": "d A = lap(A)",
@@ -502,6 +519,10 @@ CASES = [
                           "v": np.random.default_rng(9).standard_normal(32 * 24 * 3).astype(np.float32)}),
     dict(name="crank_nicolson", law=CRANK_NICOLSON_LAW, grid=(48, 5, 3, 1.0 / 48), dt=0.02, steps=6,
          params={}, init={"T": _uniform(48 * 5 * 3, 10)}),
+    dict(name="tridiag_lines", law=TRIDIAG_LINES_LAW, grid=(70, 9, 8, 0.5), dt=0.4, steps=3,
+         params={}, init={"T": _uniform(70 * 9 * 8, 31), "w": _uniform(70 * 9 * 8, 32, 0.5, 2.0)}),
+    dict(name="crank_nicolson_long", law=CRANK_NICOLSON_LAW, grid=(4500, 2, 1, 1.0 / 4500), dt=2e-5, steps=3,
+         params={}, init={"T": _uniform(4500 * 2, 33)}),
     dict(name="adi_yz", law=ADI_YZ_LAW, grid=(6, 17, 11, 0.25), dt=0.3, steps=4,
          params={}, init={"T": _uniform(6 * 17 * 11, 12), "w": _uniform(6 * 17 * 11, 13, 0.5, 2.0)}),
     dict(name="multi", law=MULTI_LAW, grid=(18, 11, 7, 0.4), dt=0.25, steps=5, params={},

@@ -1,5 +1,5 @@
-"""Writes tests/golden/bench_reference.json: the reference CPU engine's results for the bench's Dirichlet and
-Black-Scholes cases, as SHA-256 digests of its final field bits plus its time a step.
+"""Writes tests/golden/bench_reference.json: the reference CPU engine's results for the bench's Dirichlet,
+Black-Scholes and implicit-lines cases, as SHA-256 digests of its final field bits plus its time a step.
 
 The bench holds the card's fields to these digests (bit identity) and reports the time as the CPU baseline of those
 cases. The reference's `lawc` is built from its sources (make -f oracle/Makefile.ref) and runs only here, in the build
@@ -59,6 +59,15 @@ def main() -> None:
     out["black_scholes_4096x512"] = {"sha256": hashlib.sha256(final.tobytes()).hexdigest(), "steps": 10, "median_ms": ms,
                                      "cores": cores, "sample": f"reference lawc (its CPU engine), OpenMP {cores} threads in "
                                      f"the build container, 4096x1x512, 10 steps, median {ms:.3f} ms/step"}
+    n = 256
+    h = 1.0 / n
+    t0, w0 = bench.implicit_start(n)
+    for name, law, dt, init in (("crank_nicolson_x", bench.IMPLICIT_X_LAW, 0.5 * h * h / 0.01, {"T": t0}),
+                                ("implicit_yz", bench.IMPLICIT_YZ_LAW, 0.5 * h * h / 0.05, {"T": t0, "w": w0})):
+        final, ms = run(law, (n, n, n, h), init, 3, dt, "T", cores)
+        out[f"{name}_{n}"] = {"sha256": hashlib.sha256(final.tobytes()).hexdigest(), "steps": 3, "median_ms": ms,
+                              "cores": cores, "sample": f"reference lawc (its CPU engine), OpenMP {cores} threads in the "
+                              f"build container, {n}^3, 3 steps, median {ms:.1f} ms/step"}
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bench_reference.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
