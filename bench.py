@@ -382,6 +382,53 @@ def implicit_lines(n: int, steps: int, check_steps: int = 3):
     return out
 
 
+def mppi_plan(cpu: bool, plans: int = 20):
+    """The world model's MPPI planner (include/fe_mppi.h): 256 agents, 512 futures of 32 steps each, dynamics 16 + 2 ->
+    64 -> 64 -> 16 (tanh) on fp16 MFMA; seeded random weights and observations. A plan is one fe_mppi_plan call (it
+    uploads the observations and plans, runs the futures and the update, and brings the plans back)."""
+    F, HID, H, K, agents = 16, 64, 32, 512, 256
+    rng = np.random.default_rng(21)
+    w1 = rng.uniform(-1, 1, (HID, F + 2)) * np.sqrt(3.0 / (F + 2))
+    w2 = rng.uniform(-1, 1, (HID, HID)) * np.sqrt(3.0 / HID)
+    w3 = rng.uniform(-1, 1, (F, HID)) * 0.5 * np.sqrt(3.0 / HID)
+    flat = np.concatenate([w1.ravel(), rng.uniform(-0.1, 0.1, HID), w2.ravel(), rng.uniform(-0.1, 0.1, HID), w3.ravel(),
+                           rng.uniform(-0.05, 0.05, F)]).astype(np.float32)
+    shift, inscale, scale = np.zeros(F + 2, np.float32), np.ones(F + 2, np.float32), np.ones(F, np.float32)
+    target, weight = rng.uniform(-0.5, 0.5, F).astype(np.float32), np.ones(F, np.float32)
+    pl = field_hip.MppiPlanner(F, HID, H, K)
+    pl.set_dynamics(flat, shift, inscale, scale)
+    pl.set_cost(target, weight, 0.01)
+    obs = rng.uniform(-1, 1, (agents, F)).astype(np.float32)
+    ids = np.arange(agents, dtype=np.uint32)
+    mean = np.zeros((agents, H, 2), np.float32)
+    for tick in range(3):
+        mean, _ = pl.plan(obs, ids, mean, tick=tick, sigma=0.3, dt=0.1)
+    times = []
+    for tick in range(3, 3 + plans):
+        t = time.perf_counter()
+        mean, _ = pl.plan(obs, ids, mean, tick=tick, sigma=0.3, dt=0.1)
+        times.append((time.perf_counter() - t) * 1e3)
+    ms = float(np.median(times))
+    flop = 2.0 * agents * K * H * ((F + 2) * HID + HID * HID + HID * F)
+    out = {"agents": agents, "samples": K, "horizon": H, "features": F, "hidden": HID, "ms_per_plan": ms,
+           "futures_per_s": agents * K / (ms * 1e-3), "mfma_tflops": flop / (ms * 1e9),
+           "of_fp16_dense_peak": flop / (ms * 1e9) / 2500.0, "plan_finite": bool(np.all(np.isfinite(mean)))}
+    if cpu:
+        from oracle import mppi as ref
+        W1, b1, W2, b2, W3, b3 = ref.split(flat, F, HID)
+        params = {"W1": W1, "b1": b1, "W2": W2, "b2": b2, "W3": W3, "b3": b3, "shift": shift.astype(np.float64),
+                  "inscale": inscale.astype(np.float64), "scale": scale.astype(np.float64),
+                  "target": target.astype(np.float64), "weight": weight.astype(np.float64), "action_weight": 0.01}
+        sample = 2
+        t = time.perf_counter()
+        ref.plan(params, obs[:sample].astype(np.float64), ids[:sample], mean[:sample].astype(np.float64), 99, 0.3, 0.1, K)
+        sec = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": sample * K / sec, "unit": "futures/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle NumPy restatement (float64), {sample} agents x {K} futures x {H} steps, "
+                                         f"{sec:.1f} s"}
+    return out
+
+
 def reference_check(name: str, final: np.ndarray) -> dict:
     """The card's field against the reference CPU engine's on the same law, start and steps: tests/golden/
     bench_reference.json holds the SHA-256 of the reference's final field bits and its time a step, both taken in the
@@ -590,6 +637,7 @@ def main() -> int:
     sections = {"dirichlet_heat_64": dirichlet_heat,
                 "black_scholes_4096x512": lambda: black_scholes(200),
                 "implicit_lines_256": lambda: implicit_lines(256, 50),
+                "mppi_plan": lambda: mppi_plan(not args.no_cpu),
                 "learned_coarse": lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4),
                 "adjoint_transport": lambda: adjoint_transport(args.adjoint_size, not args.no_cpu)}
     if args.section:

@@ -9,7 +9,9 @@
  * exp(-(J - J_min) / lambda), lambda a quarter of the cost spread (the reference's rule), summed in a fixed order.
  *
  * This build's differences, by design: the dynamics run on the matrix cores (fp16 operands, fp32 accumulation,
- * v_mfma_f32_16x16x16_f16, 16 futures a wave), not one thread a future; the cost is a weighted quadratic about a
+ * v_mfma_f32_16x16x16_f16, 16 futures a wave), not one thread a future; the noise is a hash of this build's own
+ * (splitmix64's finaliser over its own key, then Box-Muller), so plans are reproducible here but not the reference's
+ * bit for bit; the cost is a weighted quadratic about a
  * target, sum_f w_f (s_f - target_f)^2 + action_weight |a|^2 a step, set by the caller (the reference generates the
  * cost from a creature's `plan` line, a part of the law language this build's front end leaves to the reference).
  *

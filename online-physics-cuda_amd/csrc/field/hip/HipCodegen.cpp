@@ -285,13 +285,27 @@ extern "C" __global__ __launch_bounds__(256) void k_bounds(K k, int field, unsig
     const float* v = k.now[field];
     const i64 stride = (i64)gridDim.x * blockDim.x;
     const i64 quads = k.cells / 4; // fields are 256-byte aligned: whole float4s, then the tail
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < quads; i += stride) {
-        const float4 q = reinterpret_cast<const float4*>(v)[i];
+    const float4* v4 = reinterpret_cast<const float4*>(v);
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < quads; i += 4 * stride) { // four loads in flight a lane
+        float4 q[4];
+        for (int u = 0; u < 4; ++u) q[u] = v4[i + u * stride];
+        for (int u = 0; u < 4; ++u) { bound_key(q[u].x, lo, hi); bound_key(q[u].y, lo, hi); bound_key(q[u].z, lo, hi); bound_key(q[u].w, lo, hi); }
+    }
+    for (; i < quads; i += stride) {
+        const float4 q = v4[i];
         bound_key(q.x, lo, hi); bound_key(q.y, lo, hi); bound_key(q.z, lo, hi); bound_key(q.w, lo, hi);
     }
     for (i64 i = 4 * quads + (i64)blockIdx.x * blockDim.x + threadIdx.x; i < k.cells; i += stride) bound_key(v[i], lo, hi);
     for (int o = 32; o > 0; o >>= 1) { lo = min(lo, (unsigned)__shfl_xor((int)lo, o)); hi = max(hi, (unsigned)__shfl_xor((int)hi, o)); }
-    if ((threadIdx.x & 63u) == 0u) { atomicMin(out + field, lo); atomicMax(out + fields + field, hi); }
+    // the block's four waves through LDS, then one pair of atomics a block (fewer on the one contended address)
+    __shared__ unsigned wl[4], wh[4];
+    if ((threadIdx.x & 63u) == 0u) { wl[threadIdx.x >> 6] = lo; wh[threadIdx.x >> 6] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        lo = min(min(wl[0], wl[1]), min(wl[2], wl[3])); hi = max(max(wh[0], wh[1]), max(wh[2], wh[3]));
+        atomicMin(out + field, lo); atomicMax(out + fields + field, hi);
+    }
 }
 extern "C" __global__ void k_bounds_reset(unsigned* out, int fields)
 {
