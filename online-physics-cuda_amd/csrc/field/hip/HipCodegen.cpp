@@ -268,8 +268,9 @@ extern "C" __global__ void k_zero(unsigned* p, int n)
 
 // A field's least and greatest value (NaN skipped, as the host's look over a lattice skips it), for deciding which terms
 // have nothing to do: floats as order keys (a key's unsigned order is the float's order), so the least and greatest are
-// integer atomics and the answer does not depend on the order. out[field] the least key, out[fields + field] the
-// greatest; k_bounds_reset starts them at the empty range.
+// integer atomics and the answer does not depend on the order. out[field * FE_BOUND_SLOTS + s] the least key,
+// out[(fields + field) * FE_BOUND_SLOTS + s] the greatest, s a block's slot (the host folds the slots); k_bounds_reset
+// starts them at the empty range.
 __device__ __forceinline__ unsigned order_key(float v)
 {
     const unsigned b = __float_as_uint(v);
@@ -304,13 +305,14 @@ extern "C" __global__ __launch_bounds__(256) void k_bounds(K k, int field, unsig
     __syncthreads();
     if (threadIdx.x == 0u) {
         lo = min(min(wl[0], wl[1]), min(wl[2], wl[3])); hi = max(max(wh[0], wh[1]), max(wh[2], wh[3]));
-        atomicMin(out + field, lo); atomicMax(out + fields + field, hi);
+        const int slot = (int)(blockIdx.x % FE_BOUND_SLOTS);
+        atomicMin(out + field * FE_BOUND_SLOTS + slot, lo); atomicMax(out + (fields + field) * FE_BOUND_SLOTS + slot, hi);
     }
 }
 extern "C" __global__ void k_bounds_reset(unsigned* out, int fields)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i < 2 * fields) out[i] = i < fields ? 0xffffffffu : 0u;
+    if (i < 2 * fields * FE_BOUND_SLOTS) out[i] = i < fields * FE_BOUND_SLOTS ? 0xffffffffu : 0u;
 }
 
 // A reduction's partials (sum, least, most per block, from k_reduce passes) folded in order into U[uslot].
@@ -363,6 +365,7 @@ public:
              << "    long long nxy;\n    long long cells;\n    double h;\n    double dt;\n    double inv_h;\n    double inv_hh;\n"
              << "    int nx;\n    int ny;\n    int nz;\n    int flags;\n    double P[" << np << "];\n};\n";
         lay_.kernel_arg_bytes = (2 * nf + 6) * 8 + 6 * 8 + 4 * 4 + 8 * np;
+        out_ << "#define FE_BOUND_SLOTS " << kBoundSlots << "\n";
         out_ << kPrelude;
         // Reductions' results are read once a kernel into registers (a load inside the z loop would be repeated and,
         // waited for, would drain the loads in flight).

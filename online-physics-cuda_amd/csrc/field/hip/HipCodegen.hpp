@@ -56,6 +56,10 @@ struct StageCode {
     int outs = 0;
 };
 
+// k_bounds spreads its integer atomics over this many slots a field (block b uses slot b % kBoundSlots), so no single
+// address takes every block's; the host folds the slots. Keys: [lo: field][slot], then [hi: field][slot].
+constexpr int kBoundSlots = 32;
+
 // A uniform Tridiag stage keeps c' in LDS for lines up to this long (the launch gives it 8 bytes a cell of the line);
 // longer lines take its <kernel>_s, which keeps c' in the scratch with d'.
 constexpr int kTridiagLdsLine = 4096;

@@ -503,7 +503,7 @@ struct Impl final : HipBackend {
         half_outs.allocate(out_bytes);
         half_faces.allocate(faces ? cells * 3 * 4 : 0);
         const std::size_t rec_bytes = static_cast<std::size_t>(kRanges) * 8, first_bytes = static_cast<std::size_t>(kRanges) * 8;
-        const std::size_t flag_bytes = (std::max<std::size_t>(1, nf) * 4 + 7) / 8 * 8, key_bytes = std::max<std::size_t>(1, nf) * 8;
+        const std::size_t flag_bytes = (std::max<std::size_t>(1, nf) * 4 + 7) / 8 * 8, key_bytes = std::max<std::size_t>(1, nf) * 8 * hip::kBoundSlots;
         contract.allocate(flag_bytes + key_bytes + rec_bytes + first_bytes + static_cast<std::size_t>(kTotals) * 8);
         C.p = contract.p;
         bounds_keys.p = static_cast<char*>(contract.p) + flag_bytes;
@@ -1463,7 +1463,8 @@ struct Impl final : HipBackend {
         int nf = static_cast<int>(law->fields.size());
         unsigned* keys = bounds_keys.as<unsigned>();
         void* rargs[] = {&keys, &nf};
-        check(fe_hip_launch(fn("k_bounds_reset"), static_cast<unsigned>((2 * nf + 255) / 256), 1, 1, 256, 1, 1, 0, rargs), "launch k_bounds_reset");
+        check(fe_hip_launch(fn("k_bounds_reset"), static_cast<unsigned>((2 * nf * hip::kBoundSlots + 255) / 256), 1, 1, 256, 1, 1, 0, rargs),
+            "launch k_bounds_reset");
         std::vector<std::uint64_t> k = args(0.0);
         for (int f : which) {
             void* bargs[] = {k.data(), &f, &keys, &nf};
@@ -1483,8 +1484,13 @@ struct Impl final : HipBackend {
             std::memcpy(&v, &b, 4);
             return static_cast<double>(v);
         };
+        const std::size_t slots = hip::kBoundSlots;
         for (const int f : which) {
-            const unsigned lo = host[static_cast<std::size_t>(f)], hi = host[nf + static_cast<std::size_t>(f)];
+            unsigned lo = 0xffffffffu, hi = 0u;
+            for (std::size_t q = 0; q < slots; ++q) {
+                lo = std::min(lo, host[static_cast<std::size_t>(f) * slots + q]);
+                hi = std::max(hi, host[(nf + static_cast<std::size_t>(f)) * slots + q]);
+            }
             if (lo == 0xffffffffu && hi == 0u) { // every cell NaN: the empty range, as a look over the cells finds
                 known[static_cast<std::size_t>(f)] = Bounds{std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity()};
             } else {
@@ -1693,7 +1699,7 @@ struct Impl final : HipBackend {
             }
         }
         if (!measured.empty()) {
-            take_measure(measured, std::vector<unsigned>(keys_back, keys_back + 2 * nf));
+            take_measure(measured, std::vector<unsigned>(keys_back, keys_back + 2 * nf * hip::kBoundSlots));
         }
         for (std::size_t f = 0; f < nf; ++f) {
             stale[f] = stale[f] || changed[f];
@@ -1749,7 +1755,7 @@ struct Impl final : HipBackend {
                 measured.push_back(d.target);
             }
         }
-        std::vector<unsigned> keys(2 * law->fields.size(), 0u);
+        std::vector<unsigned> keys(2 * law->fields.size() * hip::kBoundSlots, 0u);
         if (!measured.empty()) {
             queue_measure(measured);
             check(fe_hip_memcpy_dtoh_async(keys.data(), bounds_keys.p, keys.size() * 4), "bounds readback");
