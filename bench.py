@@ -288,14 +288,16 @@ def dirichlet_heat():
 
 def black_scholes_start(nx: int = 4096, planes: int = 512):
     """x = ln S on ln 100 +- 4 (h = 8 / nx), the put payoff with strike 100 averaged over each cell, the same line in
-    every plane; returns (V0, h, dt) with dt = max_dt = 0.4 h^2 / sig^2 at sig = 0.2."""
+    every plane; returns (V0, h, dt) with dt = max_dt = 0.4 h^2 / sig^2 at sig = 0.2, computed as the law computes it
+    (sig * sig is 0.04000000000000001 in double: a dt of 0.4 h^2 / 0.04 would be a hair above max_dt, two substeps)."""
     h = 8.0 / nx
     x0 = np.log(100.0) - 4.0
     edges = x0 + h * np.arange(nx + 1)
     lo, hi = edges[:-1], edges[1:]
     a = np.minimum(hi, np.log(100.0))
     pay = np.where(a > lo, (100.0 * (a - lo) - (np.exp(a) - np.exp(lo))) / h, 0.0)
-    return np.tile(pay, planes).astype(np.float32), h, 0.4 * h * h / 0.04
+    sig = 0.2
+    return np.tile(pay, planes).astype(np.float32), h, 0.4 * h * h / (sig * sig)
 
 
 def black_scholes(steps: int):
@@ -427,6 +429,41 @@ def mppi_plan(cpu: bool, plans: int = 20):
                                "sample": f"oracle NumPy restatement (float64), {sample} agents x {K} futures x {H} steps, "
                                          f"{sec:.1f} s"}
     return out
+
+
+def hbm_roof(n: int = 512, reps: int = 10):
+    """The attainable HBM rate at n^3 floats (512^3: 512 MiB each way, twice the last-level cache): a copy and a read
+    swept over loads in flight a lane (1, 2, 4, 8), ordinary or nontemporal accesses, and 2, 4 or 8 blocks of 256
+    threads a CU; the best of each, by events over `reps` back-to-back launches, beside the copy kernel the bench uses."""
+    cells = n ** 3
+    a = field_hip.DeviceArray.from_host(np.random.default_rng(3).random(cells, dtype=np.float32))
+    b = field_hip.DeviceArray(cells)
+    cus = field_hip.require_device().compute_units
+    ev = field_hip.Events()
+
+    def timed(launch):
+        for _ in range(3):
+            launch()
+        field_hip.sync()
+        ev.begin()
+        for _ in range(reps):
+            launch()
+        return ev.end() / reps
+
+    runs = []
+    for read_only in (False, True):
+        for inflight in (1, 2, 4, 8):
+            for nt in (False, True):
+                for per_cu in (2, 4, 8):
+                    ms = timed(lambda: field_hip.roof_f32(a, None if read_only else b, inflight, nt, read_only, per_cu * cus))
+                    moved = (1 if read_only else 2) * 4.0 * cells
+                    runs.append({"read_only": read_only, "inflight": inflight, "nontemporal": nt, "blocks_per_cu": per_cu,
+                                 "ms": ms, "gbs": moved / (ms * 1e6)})
+    copy_ms = timed(lambda: field_hip.copy_f32(a, b))
+    best_copy = max((r for r in runs if not r["read_only"]), key=lambda r: r["gbs"])
+    best_read = max((r for r in runs if r["read_only"]), key=lambda r: r["gbs"])
+    return {"n": n, "bytes_each_way": 4 * cells, "bench_copy_gbs": 8.0 * cells / (copy_ms * 1e6), "best_copy": best_copy,
+            "best_read": best_read, "runs": runs}
 
 
 def reference_check(name: str, final: np.ndarray) -> dict:
@@ -638,6 +675,7 @@ def main() -> int:
                 "black_scholes_4096x512": lambda: black_scholes(200),
                 "implicit_lines_256": lambda: implicit_lines(256, 50),
                 "mppi_plan": lambda: mppi_plan(not args.no_cpu),
+                "hbm_roof_512": lambda: hbm_roof(512),
                 "learned_coarse": lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4),
                 "adjoint_transport": lambda: adjoint_transport(args.adjoint_size, not args.no_cpu)}
     if args.section:

@@ -29,6 +29,11 @@ int fe_heat_handwritten_cfg(const float* in, float* out, int nx, int ny, int nz,
     int zchunk, int remap, int variant);
 /* b[i] = a[i], n floats: the bandwidth roof the reference measures beside its stencil (bench/stencil_bench.cpp:41-45). */
 int fe_copy_f32(const float* a, float* b, size_t n);
+/* No counterpart in the reference: the sweep that establishes the attainable HBM rate. b[i] = a[i] (read_only = 0) or a
+ * read of a alone (read_only = 1; b may then be NULL), n floats (a multiple of 4, 16-byte aligned), `inflight` float4
+ * loads in flight a lane (1, 2, 4 or 8), nontemporal = 1 for nontemporal loads and stores, `blocks` blocks of 256
+ * threads (grid-stride). Queued on the library's stream; -1 for arguments it does not take. */
+int fe_roof_f32(const float* a, float* b, size_t n, int inflight, int nontemporal, int read_only, int blocks);
 
 /* ---- the law compiler ------------------------------------------------------------------------------------------- */
 

@@ -297,6 +297,45 @@ __global__ __launch_bounds__(64 * (TY + 2)) void heat_exact_x4(HeatArgs a)
     }
 }
 
+// The attainable HBM rate: a copy (or a read) of float4s, D loads in flight a lane (grid-stride), ordinary or
+// nontemporal accesses. The read form keeps its loads alive through a sum that is stored only in a case that cannot
+// occur for the sweep's data (finite floats sum to a finite value, never NaN).
+typedef float roof_f4 __attribute__((ext_vector_type(4)));
+template <int D, bool NT, bool RO>
+__global__ __launch_bounds__(256) void roof_f32x4(const roof_f4* __restrict__ a, roof_f4* __restrict__ b, std::size_t n4,
+    float* __restrict__ sink)
+{
+    const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+    std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    float acc = 0.0f;
+    for (; i + (D - 1) * stride < n4; i += D * stride) {
+        roof_f4 v[D];
+        for (int u = 0; u < D; ++u) {
+            v[u] = NT ? __builtin_nontemporal_load(a + i + u * stride) : a[i + u * stride];
+        }
+        for (int u = 0; u < D; ++u) {
+            if (RO) {
+                acc += v[u].x + v[u].y + v[u].z + v[u].w;
+            } else if (NT) {
+                __builtin_nontemporal_store(v[u], b + i + u * stride);
+            } else {
+                b[i + u * stride] = v[u];
+            }
+        }
+    }
+    for (; i < n4; i += stride) {
+        const roof_f4 v = a[i];
+        if (RO) {
+            acc += v.x + v.y + v.z + v.w;
+        } else {
+            b[i] = v;
+        }
+    }
+    if (RO && acc != acc) {
+        sink[0] = acc;
+    }
+}
+
 __global__ __launch_bounds__(256) void copy_f32x4(const float4* __restrict__ a, float4* __restrict__ b, std::size_t n4)
 {
     // Four 128-bit loads in flight a lane before the first store.
@@ -443,6 +482,36 @@ int fe_copy_f32(const float* a, float* b, size_t n)
     if (n4 * 4 < n) {
         hipLaunchKernelGGL(copy_f32_tail, dim3(1), dim3(4), 0, s, a, b, n4 * 4, n);
     }
+    return static_cast<int>(hipGetLastError());
+}
+
+int fe_roof_f32(const float* a, float* b, size_t n, int inflight, int nontemporal, int read_only, int blocks)
+{
+    if (a == nullptr || (b == nullptr && !read_only) || n % 4 != 0 || blocks < 1 || blocks > (1 << 20) ||
+        (inflight != 1 && inflight != 2 && inflight != 4 && inflight != 8)) {
+        return -1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(fe_hip_stream());
+    const roof_f4* a4 = reinterpret_cast<const roof_f4*>(a);
+    roof_f4* b4 = reinterpret_cast<roof_f4*>(b);
+    float* sink = b; // written only if a sum is NaN (never for finite data); a read-only run may pass b = NULL
+    const std::size_t n4 = n / 4;
+    const dim3 g(static_cast<unsigned>(blocks)), t(256);
+#define FE_ROOF(D)                                                                                                      \
+    if (inflight == D) {                                                                                                \
+        if (read_only) {                                                                                                \
+            if (nontemporal) hipLaunchKernelGGL((roof_f32x4<D, true, true>), g, t, 0, s, a4, b4, n4, sink);             \
+            else hipLaunchKernelGGL((roof_f32x4<D, false, true>), g, t, 0, s, a4, b4, n4, sink);                        \
+        } else {                                                                                                        \
+            if (nontemporal) hipLaunchKernelGGL((roof_f32x4<D, true, false>), g, t, 0, s, a4, b4, n4, sink);            \
+            else hipLaunchKernelGGL((roof_f32x4<D, false, false>), g, t, 0, s, a4, b4, n4, sink);                       \
+        }                                                                                                               \
+    }
+    FE_ROOF(1)
+    FE_ROOF(2)
+    FE_ROOF(4)
+    FE_ROOF(8)
+#undef FE_ROOF
     return static_cast<int>(hipGetLastError());
 }
 

@@ -59,6 +59,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_heat_handwritten_cfg": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_double,
                                             c_int, c_int, c_int, c_int]),
         "fe_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t]),
+        "fe_roof_f32": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_int, c_int, c_int]),
         "fe_hip_compile": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), c_int,
                                    ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p),
                                    ctypes.POINTER(c_int)]),
@@ -275,6 +276,15 @@ def copy_f32(src: DeviceArray, dst: DeviceArray) -> None:
     if src.count != dst.count:
         raise FieldHipError("copy_f32: sizes differ")
     check(lib().fe_copy_f32(src.ptr, dst.ptr, src.count), "copy_f32")
+
+
+def roof_f32(src: DeviceArray, dst: Optional[DeviceArray], inflight: int, nontemporal: bool, read_only: bool,
+             blocks: int) -> None:
+    """The HBM roof sweep's kernel (include/fe_field.h fe_roof_f32): a copy, or a read of src alone."""
+    if dst is not None and dst.count != src.count:
+        raise FieldHipError("roof_f32: sizes differ")
+    check(lib().fe_roof_f32(src.ptr, dst.ptr if dst is not None else None, src.count, inflight, int(nontemporal),
+                            int(read_only), blocks), "roof_f32")
 
 
 def _take_string(ptr) -> Optional[str]:
