@@ -375,7 +375,18 @@ public:
              << "; }\n"
              << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
              << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const { return k.now[F][IDX(x, y, z)]; }\n"
-             << "};\n";
+             << "};\n"
+             // The direct view specialised: W32, reads with 32-bit index arithmetic (lattices of fewer than 2^31 cells: a
+             // 64-bit product a read is several instructions, and a law's gather makes dozens of reads a cell); P2, h a
+             // power of two, so a division by h or h^2 is the exact product with the reciprocal and no division is left
+             // in the code (a run-time choice between the two would have the card compute both).
+             << "template <bool W32, bool P2> struct DirectT : Direct {\n    static constexpr bool kPow2 = P2;\n"
+             << "    __device__ explicit DirectT(const K& k) : Direct(k) {}\n"
+             << "    template <int F> __device__ __forceinline__ float fld(const K& k, int x, int y, int z) const\n"
+             << "    { return W32 ? k.now[F][(unsigned)z * (unsigned)k.nxy + (unsigned)y * (unsigned)k.nx + (unsigned)x] : k.now[F][IDX(x, y, z)]; }\n"
+             << "    template <int F> __device__ __forceinline__ float fld_any(const K& k, int x, int y, int z) const { return fld<F>(k, x, y, z); }\n"
+             << "};\n"
+             << "#define FE_POW2(k) (((k).flags & 3) == 3)\n";
         std::vector<std::size_t> offsets;
         (void)net_blob(law_, offsets);
         if (!law_.nets.empty()) {
@@ -1027,8 +1038,9 @@ private:
         const int slot = lay_.slot[static_cast<std::size_t>(id)];
         switch (s.kind) {
         case StageCode::Kind::Stored:
-            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD const Direct m(k); AT(" << slot
-                 << ", x, y, z) = b" << id << "(k, m, x, y, z); }\n";
+            out_ << "extern \"C\" __global__ void " << s.kernel << "(K k) { CELL_HEAD\n"
+                 << "    if (FE_POW2(k)) { const DirectT<false, true> m(k); AT(" << slot << ", x, y, z) = b" << id << "(k, m, x, y, z); }\n"
+                 << "    else { const Direct m(k); AT(" << slot << ", x, y, z) = b" << id << "(k, m, x, y, z); }\n}\n";
             break;
         case StageCode::Kind::Reduce: {
             // Pass 1: block b sums (or bounds) cells [b * chunk, ...), lane t every 256th; a fixed tree; L[3b..3b+2].
@@ -1618,9 +1630,9 @@ private:
             code.swaps = targets;
         }
         const int nt = static_cast<int>(targets.size());
-        // The direct kernel: a thread a cell.
-        g << "extern \"C\" __global__ void " << code.gather << "(K k) { CELL_HEAD\n    const Direct m(k);\n"
-          << "    float o[" << nt << "], n[" << nt << "];\n    " << cellfn << "(k, m, x, y, z, o, n);\n"
+        // The direct kernel: a thread a cell; `<gather>_32` the same with 32-bit reads (lattices under 2^31 cells).
+        g << "template <class M> __device__ __forceinline__ void d_" << code.gather << "(const K& k, int x, int y, int z)\n{\n"
+          << "    const M m(k);\n    float o[" << nt << "], n[" << nt << "];\n    " << cellfn << "(k, m, x, y, z, o, n);\n"
           << "    const i64 i = IDX(x, y, z);\n";
         for (int q = 0; q < nt; ++q) {
             const std::string t = std::to_string(targets[static_cast<std::size_t>(q)]);
@@ -1634,6 +1646,12 @@ private:
             g << "      mark(k, " << t << ", ch); }\n";
         }
         g << "}\n";
+        for (const bool w32 : {false, true}) {
+            const std::string w = w32 ? "true" : "false";
+            g << "extern \"C\" __global__ void " << code.gather << (w32 ? "_32" : "") << "(K k) { CELL_HEAD\n"
+              << "    if (FE_POW2(k)) d_" << code.gather << "<DirectT<" << w << ", true>>(k, x, y, z);\n"
+              << "    else d_" << code.gather << "<DirectT<" << w << ", false>>(k, x, y, z);\n}\n";
+        }
         out_ << g.str();
         std::map<int, int> reach;
         int cost = 0;

@@ -260,11 +260,13 @@ struct Impl final : HipBackend {
     void launch_gather(const hip::ProgramCode& code, double dt, FusedTotal* fused = nullptr)
     {
         const hip::WindowKernel* w = window_for(code);
-        last_gather = w == nullptr ? code.gather : w->name;
         if (w == nullptr) {
-            launch_cells(code.gather, dt);
+            // the direct gather, with 32-bit reads while the lattice allows them
+            last_gather = shape.cells() < (std::size_t{1} << 31) ? code.gather + "_32" : code.gather;
+            launch_cells(last_gather, dt);
             return;
         }
+        last_gather = w->name;
         int zchunk = zchunk_override > 0 ? zchunk_override : (w->zchunk > 0 ? w->zchunk : code.zchunk);
         int tx = (shape.nx + 64 * w->vx - 1) / (64 * w->vx);
         int ty = (shape.ny + w->ty - 1) / w->ty;
