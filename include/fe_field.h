@@ -115,6 +115,12 @@ void* fe_solver_field_device(fe_solver* s, int field);
  * (checkpoints, adjoint sweeps); call it once the solver has uploaded its fields (a step or refresh_derived). No
  * counterpart in the reference, whose states travel through the host (tools/lawc/main.cpp:310-323). */
 int fe_solver_set_field_device(fe_solver* s, int field, const void* src);
+/* A static field (one no term, learned scheme or derive writes) read from `buffer` (device memory, nx*ny*nz floats)
+ * from now on, without a copy; NULL goes back to the solver's own array. For callers that hold states on the card and
+ * feed them to a solver in turn (adjoint sweeps over checkpointed states). While bound, whatever writes the field's
+ * cells (fe_solver_set_field, a window shift) writes into `buffer`. -1 for a field the law writes. No counterpart in
+ * the reference. */
+int fe_solver_bind_field_device(fe_solver* s, int field, void* buffer);
 /* The window moved by (dx, dy, dz) cells (Solver::shift_window, src/field/Solver.hpp:40-42): every field's host cells
  * slide (what is now at c was at c + d) and the arriving boxes (n_boxes of x0 y0 z0 x1 y1 z1, half-open, inside the
  * lattice) take `arriving`: for each field in order, each box in order, its cells x fastest. Before the next step the

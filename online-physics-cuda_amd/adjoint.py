@@ -4,8 +4,8 @@ The reference's `lawc --vjp` (tools/lawc/main.cpp:274-361) runs the primal forwa
 then the adjoint law (ir::adjoint_law) backward over them, uploading one state a step. Here the primal keeps only a
 checkpoint every `every` steps on the card; the backward pass takes the segments last to first: it restores the
 segment's checkpoint, recomputes the segment's states into a ring on the card, then steps the adjoint law over them
-in reverse, each state copied device to device into the adjoint law's static primal fields. Memory: (steps / every +
-every) states instead of `steps`; work: one extra forward pass.
+in reverse, the adjoint law's static primal fields bound to each state where it lies in the ring (no copy). Memory:
+(steps / every + every) states instead of `steps`; work: one extra forward pass.
 """
 from __future__ import annotations
 
@@ -85,12 +85,14 @@ def vjp(law_text: str, shape, h: float, params: dict, init: dict, dt: float, ste
                 primal.step(dt)
         for j in range(count - 1, -1, -1):
             for arr, name in zip(ring[j], inputs):
-                adj.set_field_device(name, arr)
+                adj.bind_field_device(name, arr)  # the adjoint law reads the state where it lies: no copy
             adj.step(dt)
             for p in pnames:
                 adj.total_accumulate(p + "__adj", pacc[p])
                 adj.fill_field(p + "__adj", 0.0)
     backward_ms = ev.end()
+    for name in inputs:
+        adj.bind_field_device(name, None)  # the ring goes with this call
     grads = {n: adj.get_field(n + "__adj").astype(np.float64) for n in inputs}
     pgrads = {p: float(pacc[p].download().view(np.float64)[0]) for p in pnames}
     return Result(loss, grads, pgrads, forward_ms, backward_ms,

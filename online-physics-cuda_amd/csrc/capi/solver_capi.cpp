@@ -294,6 +294,19 @@ int fe_solver_set_field_device(fe_solver* s, int field, const void* src)
     return 0;
 }
 
+int fe_solver_bind_field_device(fe_solver* s, int field, void* buffer)
+{
+    if (!valid_field(s, field) || s->solver->backend() == nullptr) {
+        return -1;
+    }
+    if (!s->solver->backend()->bind_field(field, buffer)) {
+        s->error = "field " + std::to_string(field) + " is written by the law: only a static field can be bound";
+        return -1;
+    }
+    s->solver->mark_device_written(field);
+    return 0;
+}
+
 int fe_solver_set_net_weights(fe_solver* s, const char* net, const float* weights, size_t count)
 {
     if (s == nullptr || net == nullptr || weights == nullptr) {

@@ -106,6 +106,7 @@ struct Impl final : HipBackend {
     int tile_flux = 0; // FIELD_HIP_TILE named a flux-carrying window: 1 "f4x..", 2 "h4x.."
     bool force_direct = false;
     std::string last_gather;
+    std::vector<void*> own; // per field bound to a caller's buffer: its own array (else null)
     std::vector<std::string> gathers; // per term: the gather kernel its last substep launched
     bool tracking = true; // the gathers mark change flags (the step reads them back)
 
@@ -449,6 +450,7 @@ struct Impl final : HipBackend {
     {
         const std::size_t cells = shape.cells();
         const std::size_t nf = law->fields.size();
+        own.clear(); // bindings end with the arrays they stood in for
         std::vector<char> swapped(nf, 0);
         for (const auto* progs : {&layout.terms, &layout.learned, &layout.derives}) {
             for (const hip::ProgramCode& p : *progs) {
@@ -1806,6 +1808,31 @@ struct Impl final : HipBackend {
         stale[static_cast<std::size_t>(f)] = 1;
         touched(f);
         known[static_cast<std::size_t>(f)] = Bounds{};
+    }
+
+    // A static field read from the caller's buffer (its own array kept for when it is unbound).
+    bool bind_field(int f, void* buffer) override
+    {
+        const std::size_t i = static_cast<std::size_t>(f);
+        for (const auto* progs : {&module.terms, &module.learned, &module.derives}) {
+            for (const ir::Program& p : *progs) {
+                if (std::find(p.writes.begin(), p.writes.end(), f) != p.writes.end()) {
+                    return false;
+                }
+            }
+        }
+        if (own.size() != now.size()) {
+            own.assign(now.size(), nullptr);
+        }
+        if (own[i] == nullptr) {
+            own[i] = now[i];
+        }
+        now[i] = buffer != nullptr ? buffer : own[i];
+        if (buffer == nullptr) {
+            own[i] = nullptr;
+        }
+        device_written(f);
+        return true;
     }
 
     // The field's fixed-order compensated total and total magnitude, from the card (waits for it).

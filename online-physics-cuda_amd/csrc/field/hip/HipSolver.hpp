@@ -48,6 +48,10 @@ public:
     virtual void* device_field(int field) = 0;
     // The caller wrote a field's cells on the card: the host copy is stale.
     virtual void device_written(int field) = 0;
+    // A static field's cells are read from `buffer` (the caller's device memory, a whole lattice of floats) from now
+    // on, without a copy; nullptr goes back to the solver's own array. False (nothing changes) for a field a program
+    // writes. While bound, whatever writes the field's cells (an upload, a window shift) writes into `buffer`.
+    virtual bool bind_field(int field, void* buffer) = 0;
     // A field's fixed-order compensated total and total magnitude, on the card, waited for.
     virtual void field_total(int field, double& sum, double& magnitude) = 0;
     // Adds a field's total to two doubles on the card (a running sum and its compensation), without waiting.

@@ -102,6 +102,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_solver_error": (ctypes.c_char_p, [c_void_p]),
         "fe_solver_field_device": (c_void_p, [c_void_p, c_int]),
         "fe_solver_set_field_device": (c_int, [c_void_p, c_int, c_void_p]),
+        "fe_solver_bind_field_device": (c_int, [c_void_p, c_int, c_void_p]),
         "fe_solver_total": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
         "fe_law_adjoint": (c_void_p, [c_void_p, ctypes.c_char_p, ctypes.c_char_p, c_size_t]),
         "fe_heston_create": (c_void_p, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_char_p,
@@ -509,6 +510,12 @@ class Solver:
         """Copies a whole field from device memory (a DeviceArray or a device pointer) into the solver on the card."""
         ptr = src.ptr if isinstance(src, DeviceArray) else src
         self._check(lib().fe_solver_set_field_device(self.ptr, self._field(field), ptr), "set_field_device")
+
+    def bind_field_device(self, field, buf) -> None:
+        """A static field read from device memory (a DeviceArray, a pointer, or None for the solver's own) without a
+        copy; the caller keeps the memory alive while it is bound."""
+        ptr = buf.ptr if isinstance(buf, DeviceArray) else buf
+        self._check(lib().fe_solver_bind_field_device(self.ptr, self._field(field), ptr), "bind_field_device")
 
 
 class HestonBatch:
