@@ -466,6 +466,77 @@ def hbm_roof(n: int = 512, reps: int = 10):
             "best_read": best_read, "runs": runs}
 
 
+def valley_start(nx: int, ny: int, nz: int):
+    """The valley scene (rock floor, sand bank, fuel with a fire, a pond, ice) laid out in proportion on nx x ny x nz:
+    the golden case's start (tests/golden/cases.py valley_init, 48 x 40 x 24) with every coordinate scaled."""
+    z, y, x = np.meshgrid(np.arange(nz) * (24.0 / nz), np.arange(ny) * (40.0 / ny), np.arange(nx) * (48.0 / nx), indexing="ij")
+    rock = (z < 5).astype(np.float32)
+    sand = ((z >= 5) & (z < 7) & (x < 20)).astype(np.float32)
+    fuel = ((z >= 5) & (z < 8) & (x >= 12) & (x < 30) & (y >= 10) & (y < 30)).astype(np.float32)
+    water = ((z >= 5) & (z < 10) & (x >= 34) & (x < 44) & (y >= 4) & (y < 14)).astype(np.float32)
+    ice = ((z >= 5) & (z < 7) & (x >= 40) & (y >= 30)).astype(np.float32)
+    t = np.full(x.shape, 290.0)
+    t[(x - 20) ** 2 + (y - 20) ** 2 + (z - 7) ** 2 <= 9] = 1400.0
+    t[ice > 0] = 250.0
+    t = t + np.random.default_rng(71).random(x.shape) * 0.5
+    flame = np.where((x - 20) ** 2 + (y - 20) ** 2 + (z - 7) ** 2 <= 4, 0.6, 0.0)
+    out = {"rock": rock, "sand": sand, "water": water, "ice": ice, "fuel": fuel, "T": t, "flame": flame}
+    return {k: np.ascontiguousarray(v, np.float32).ravel() for k, v in out.items()}
+
+
+VALLEY_FIELDS = ["rock", "sand", "water", "ice", "steam", "fuel", "flame", "smoke", "ash", "T", "wet", "near"]
+
+
+def valley_law() -> str:
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import cases  # the law text the golden case holds the card to the reference CPU engine with
+    return cases.VALLEY_LAW
+
+
+def world_valley(n=(144, 144, 48), steps: int = 50, check_steps: int = 3):
+    """A many-term world law (the valley: 14 terms over 12 fields, the shipped world law's op mix) on 144 x 144 x 48
+    (1 M cells, the size the reference times its world law at), h = 0.5, dt = 0.25: held to the reference CPU
+    engine's bits after `check_steps` steps, then timed with contracts checked every step (lazy readback, one
+    fe_solver_step a step, wall clock) and through Solver.run with contracts every 8 steps."""
+    nx, ny, nz = n
+    law = field_hip.Law(valley_law())
+    start = valley_start(nx, ny, nz)
+
+    def fresh():
+        s = field_hip.Solver(law, (nx, ny, nz), 0.5)
+        for f, v in start.items():
+            s.set_field(f, v)
+        s.refresh_derived()
+        s.set_lazy_readback(True)
+        return s
+
+    s = fresh()
+    s.check_contracts_every(1)
+    for _ in range(check_steps):
+        s.step(0.25)
+    rec = reference_check(f"valley_{nx}x{ny}x{nz}", np.concatenate([s.get_field(f) for f in VALLEY_FIELDS]))
+    walls = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        s.step(0.25)
+        walls.append((time.perf_counter() - t) * 1e3)
+    prof = s.profile()
+    idle = sum(1 for t in prof["terms"] if t.get("idle"))
+    r = fresh()
+    r.check_contracts_every(8)
+    r.run(0.25, steps)
+    field_hip.sync()
+    ev = field_hip.Events()
+    ev.begin()
+    r.run(0.25, steps)
+    run_ms = ev.end() / steps
+    out = {"cells": nx * ny * nz, "terms": len(prof["terms"]), "idle_terms_last_step": idle,
+           "ms_per_step_contracts_every_step": float(np.median(walls)), "p99_ms": float(np.percentile(walls, 99)),
+           "ms_per_step_run_contracts_every_8": run_ms, "violations_last_step": len(s.violations())}
+    out.update(rec)
+    return out
+
+
 def reference_check(name: str, final: np.ndarray) -> dict:
     """The card's field against the reference CPU engine's on the same law, start and steps: tests/golden/
     bench_reference.json holds the SHA-256 of the reference's final field bits and its time a step, both taken in the
@@ -676,6 +747,7 @@ def main() -> int:
                 "implicit_lines_256": lambda: implicit_lines(256, 50),
                 "mppi_plan": lambda: mppi_plan(not args.no_cpu),
                 "hbm_roof_512": lambda: hbm_roof(512),
+                "world_valley_144x144x48": lambda: world_valley(),
                 "learned_coarse": lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4),
                 "adjoint_transport": lambda: adjoint_transport(args.adjoint_size, not args.no_cpu)}
     if args.section:

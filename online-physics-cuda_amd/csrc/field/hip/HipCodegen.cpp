@@ -140,24 +140,51 @@ __device__ __forceinline__ void block_sum_bounds(Sum& a, double& lo, double& hi,
     }
 }
 
+// A block's range counts (out of range, not finite) and first bad cell into its record: the lanes' by wave folds, the
+// waves' in LDS, then at most three atomics a block (an atomic a bad cell would queue on one address when a field is
+// broken across the lattice). Every thread of the block calls it.
+__device__ __forceinline__ void range_fold(unsigned nout, unsigned nnan, u64 fb, unsigned* rec, u64* first)
+{
+    __shared__ unsigned rn[16], rb[16];
+    __shared__ u64 rf[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        nout += (unsigned)__shfl_xor((int)nout, o);
+        nnan += (unsigned)__shfl_xor((int)nnan, o);
+        const u64 other = __shfl_xor(fb, o);
+        fb = other < fb ? other : fb;
+    }
+    const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x), w = t >> 6, nw = (int)((blockDim.x * blockDim.y + 63) >> 6);
+    if ((t & 63) == 0) { rn[w] = nout; rb[w] = nnan; rf[w] = fb; }
+    __syncthreads();
+    if (t == 0) {
+        unsigned a = 0, b = 0; u64 f = ~0ull;
+        for (int q = 0; q < nw; ++q) { a += rn[q]; b += rb[q]; f = rf[q] < f ? rf[q] : f; }
+        if (a != 0u) atomicAdd(&rec[0], a);
+        if (b != 0u) atomicAdd(&rec[1], b);
+        if (a != 0u || b != 0u) atomicMin(first, f);
+    }
+}
+
 // A field's range contract (src/field/Solver.cpp:463-511): counts of non-finite and out-of-range cells and the least
 // index of a bad one (integer atomics: the order does not matter). A clamped field's finite strays are clamped instead.
 extern "C" __global__ void k_range(K k, int field, double lo, double hi, int clamp, unsigned* rec, u64* first)
 {
-    CELL_HEAD
-    const i64 i = IDX(x, y, z);
-    const float v = k.now[field][i];
-    const double d = (double)v;
-    const bool fin = finite_d(d);
-    bool bad = !fin;
-    if (fin && (d < lo || d > hi)) {
-        if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
-        else bad = true;
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x); const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = (int)blockIdx.z;
+    unsigned nout = 0, nnan = 0; u64 fb = ~0ull;
+    if (x < k.nx && y < k.ny) {
+        const i64 i = IDX(x, y, z);
+        const float v = k.now[field][i];
+        const double d = (double)v;
+        const bool fin = finite_d(d);
+        bool bad = !fin;
+        if (fin && (d < lo || d > hi)) {
+            if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
+            else bad = true;
+        }
+        if (bad) { if (fin) nout = 1; else nnan = 1; fb = (u64)i; }
     }
-    if (bad) {
-        atomicAdd(&rec[fin ? 0 : 1], 1u);
-        atomicMin(first, (u64)i);
-    }
+    range_fold(nout, nnan, fb, rec, first);
 }
 
 // A clamped field's finite strays into its range, on a step that checks no contract (src/field/Solver.cpp:471-476
@@ -200,6 +227,7 @@ extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 ch
     const i64 begin = (i64)blockIdx.x * chunk;
     i64 end = begin + chunk; if (end > k.cells) end = k.cells;
     Sum s = {0.0, 0.0}; Sum m = {0.0, 0.0};
+    unsigned nout = 0, nnan = 0; u64 fb = ~0ull;
     const float* v = k.now[field];
     // Eight cells' loads (clamped into the range, unconditional) ahead of their compensated sums, which keep their
     // order: the pass waits on memory once in eight cells, not every cell.
@@ -222,13 +250,11 @@ extern "C" __global__ __launch_bounds__(256) void k_total(K k, int field, i64 ch
                     if (clamp) k.now[field][i] = fe_rf(d < lo ? lo : (hi < d ? hi : d));
                     else bad = true;
                 }
-                if (bad) {
-                    atomicAdd(&rec[fin ? 0 : 1], 1u);
-                    atomicMin(first, (u64)i);
-                }
+                if (bad) { if (fin) ++nout; else ++nnan; fb = (u64)i < fb ? (u64)i : fb; }
             }
         }
     }
+    if (rec != nullptr) range_fold(nout, nnan, fb, rec, first);
     block_pairs(s, m, wv);
     if (t == 0) { out[2 * blockIdx.x] = s.s + s.c; out[2 * blockIdx.x + 1] = m.s + m.c; }
 }

@@ -25,7 +25,8 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 
-def run(law: str, grid, init: dict, steps: int, dt: float, field: str, cores: int):
+def run(law: str, grid, init: dict, steps: int, dt: float, field, cores: int):
+    """The reference's final `field` (or, for a list of names, their concatenation) and its median ms a step."""
     nx, ny, nz, h = grid
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "law.law")
@@ -38,7 +39,8 @@ def run(law: str, grid, init: dict, steps: int, dt: float, field: str, cores: in
         subprocess.run([LAWC, "--grid", str(nx), str(ny), str(nz), repr(float(h)), "--init", init_dir, "--steps", str(steps),
                         "--dt", repr(float(dt)), "--dump", out_dir, "--json", js, "--warmup", "1", path],
                        check=True, capture_output=True, env=dict(os.environ, OMP_NUM_THREADS=str(cores)), timeout=1200)
-        final = np.fromfile(os.path.join(out_dir, f"step_{steps}", field + ".bin"), dtype=np.float32)
+        names = [field] if isinstance(field, str) else list(field)
+        final = np.concatenate([np.fromfile(os.path.join(out_dir, f"step_{steps}", f + ".bin"), dtype=np.float32) for f in names])
         return final, json.load(open(js))["median_ms"]
 
 
@@ -68,6 +70,12 @@ def main() -> None:
         out[f"{name}_{n}"] = {"sha256": hashlib.sha256(final.tobytes()).hexdigest(), "steps": 3, "median_ms": ms,
                               "cores": cores, "sample": f"reference lawc (its CPU engine), OpenMP {cores} threads in the "
                               f"build container, {n}^3, 3 steps, median {ms:.1f} ms/step"}
+    nx, ny, nz = 144, 144, 48
+    final, ms = run(bench.valley_law(), (nx, ny, nz, 0.5), bench.valley_start(nx, ny, nz), 3, 0.25, bench.VALLEY_FIELDS, cores)
+    out[f"valley_{nx}x{ny}x{nz}"] = {"sha256": hashlib.sha256(final.tobytes()).hexdigest(), "steps": 3, "median_ms": ms, "cores": cores,
+                                     "sample": f"reference lawc (its CPU engine), OpenMP {cores} threads in the build "
+                                     f"container, the valley law on {nx}x{ny}x{nz}, 3 steps, contracts every step, median "
+                                     f"{ms:.1f} ms/step"}
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bench_reference.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
