@@ -355,6 +355,45 @@ extern "C" __global__ __launch_bounds__(256) void k_reduce_final(K k, int kind, 
     block_sum_bounds(s, a, b, wv);
     if (t == 0) ((double*)k.U)[uslot] = kind == 0 ? s.s + s.c : (kind == 1 ? b : a);
 }
+// The lower envelope of parabolas f(j) + (q - j)^2 along a line (the distance stage's line kernel). push adds the
+// parabola at q (at infinity: never lowest, left out), popping those it hides, by exact cross-multiplied comparisons of
+// the intersections (whole numbers below 2^24, products far below 2^53); the stack's top two entries stay in
+// registers, so LDS is read only when one is popped. start / at then walk the envelope from the left, giving the
+// value of the parabola lowest at q (LDS read only when the walk moves on).
+struct FeEnv {
+    unsigned short* st; float* fs;
+    int kk = -1;
+    int tv = 0, pv = 0; double tF = 0.0, pF = 0.0;
+    int j = 0, v0 = 0, v1 = 0; float f0 = 0.0f; double F0 = 0.0, F1 = 0.0;
+    __device__ __forceinline__ void push(float fq, int q)
+    {
+        if (!(fq < __builtin_inff())) return;
+        const double Fq = (double)fq + (double)q * (double)q;
+        while (kk >= 1 && (Fq - tF) * (double)(tv - pv) <= (tF - pF) * (double)(q - tv)) {
+            --kk; tv = pv; tF = pF;
+            if (kk >= 1) { pv = st[kk - 1]; pF = (double)fs[kk - 1] + (double)pv * (double)pv; }
+        }
+        if (kk >= 0) { pv = tv; pF = tF; }
+        ++kk; st[kk] = (unsigned short)q; fs[kk] = fq; tv = q; tF = Fq;
+    }
+    __device__ __forceinline__ void start()
+    {
+        j = 0;
+        if (kk < 0) return;
+        v0 = st[0]; f0 = fs[0]; F0 = (double)f0 + (double)v0 * (double)v0;
+        if (kk >= 1) { v1 = st[1]; F1 = (double)fs[1] + (double)v1 * (double)v1; }
+    }
+    __device__ __forceinline__ float at(int q)
+    {
+        if (kk < 0) return __builtin_inff();
+        while (j < kk && F1 - F0 < 2.0 * (double)q * (double)(v1 - v0)) {
+            ++j; v0 = v1; F0 = F1; f0 = fs[j];
+            if (j < kk) { v1 = st[j + 1]; F1 = (double)fs[j + 1] + (double)v1 * (double)v1; }
+        }
+        const double dq = (double)(q - v0);
+        return (float)(dq * dq + (double)f0);
+    }
+};
 )HIP";
 
 std::string literal(double v)
@@ -1519,6 +1558,48 @@ private:
              << "        }\n"
              << "    }\n"
              << "    k.A[(i64)to_in * k.cells + here] = bi; k.A[(i64)to_out * k.cells + here] = bo;\n"
+             << "}\n";
+        // The same axis a line a thread: the lower envelope of the parabolas f(j) + (q - j)^2 (Felzenszwalb and
+        // Huttenlocher's 1-D transform) built once along the line, then read at every q, in O(n) a line instead of an
+        // outward search from every cell (which walks whole lines where a set is far or absent). The comparisons are
+        // exact: the f(j) are whole numbers below 2^24 and the intersections are compared cross-multiplied in double
+        // (FeEnv), so each q gets the exact least value, the outward search's to the bit. The stacks
+        // (indices and f) of the block's lines sit in LDS (the launch gives 12 n bytes a line, n the line's length).
+        out_ << "extern \"C\" __global__ void " << s.kernel
+             << "_lines(K k, const unsigned* w, int axis, int from_in, int from_out, int to_in, int to_out) {\n"
+             << "    if (w[0] == 0u) return;\n"
+             << "    extern __shared__ __attribute__((aligned(16))) unsigned char env_lds[];\n"
+             // a block: L lines, both sets (the squared distances to the set's cells and to the others), a thread each
+             << "    const int L = (int)blockDim.x / 2, t = (int)threadIdx.x, out = t >= L ? 1 : 0, lt = t - out * L;\n"
+             << "    const int n = axis == 0 ? k.nx : (axis == 1 ? k.ny : k.nz);\n"
+             << "    const i64 lines = k.cells / n;\n"
+             << "    const i64 line = (i64)blockIdx.x * L + lt;\n"
+             << "    if (line >= lines) return;\n"
+             << "    int x = 0, y = 0, z = 0;\n"
+             << "    if (axis == 0) { y = (int)(line % k.ny); z = (int)(line / k.ny); }\n"
+             << "    else if (axis == 1) { x = (int)(line % k.nx); z = (int)(line / k.nx); }\n"
+             << "    else { x = (int)(line % k.nx); y = (int)(line / k.nx); }\n"
+             << "    const i64 base = IDX(x, y, z), step = axis == 0 ? 1 : (axis == 1 ? (i64)k.nx : k.nxy);\n"
+             << "    FeEnv e;\n"
+             << "    e.st = reinterpret_cast<unsigned short*>(env_lds) + (i64)t * n;\n"
+             << "    e.fs = reinterpret_cast<float*>(env_lds + (i64)4 * L * n) + (i64)t * n;\n"
+             << "    const float inf = __builtin_inff();\n"
+             << "    const int from = from_in < 0 ? " << set << " : (out ? from_out : from_in);\n"
+             << "    const float* f = k.A + (i64)from * k.cells + base;\n"
+             // sixteen cells' loads in flight before their pushes (a push waits on the one before it)
+             << "    for (int q0 = 0; q0 < n; q0 += 16) {\n"
+             << "        float av[16];\n"
+             << "#pragma unroll\n"
+             << "        for (int u = 0; u < 16; ++u) { const i64 q = q0 + u < n ? q0 + u : n - 1; av[u] = f[q * step]; }\n"
+             << "#pragma unroll\n"
+             << "        for (int u = 0; u < 16; ++u) {\n"
+             << "            const int q = q0 + u;\n"
+             << "            if (q < n) e.push(from_in < 0 ? ((av[u] != 0.0f) != (out != 0) ? 0.0f : inf) : av[u], q);\n"
+             << "        }\n"
+             << "    }\n"
+             << "    float* o = k.A + (i64)(out ? to_out : to_in) * k.cells + base;\n"
+             << "    e.start();\n"
+             << "    for (int q = 0; q < n; ++q) o[(i64)q * step] = e.at(q);\n"
              << "}\n";
         out_ << "extern \"C\" __global__ void " << s.kernel << "_signed(K k, unsigned* w) {\n"
              << "    if (w[0] == 0u) return;\n    CELL_HEAD\n"

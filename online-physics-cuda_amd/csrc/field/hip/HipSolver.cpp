@@ -762,7 +762,25 @@ struct Impl final : HipBackend {
             const int passes[3][4] = {{-1, -1, slot + 2, slot + 3}, {slot + 2, slot + 3, slot + 4, slot + 5}, {slot + 4, slot + 5, slot + 2, slot + 3}};
             for (int axis = 0; axis < 3; ++axis) {
                 int a = axis, fi = passes[axis][0], fo = passes[axis][1], ti = passes[axis][2], to = passes[axis][3];
-                launch_cells(s.kernel + "_scan", dt, {&w, &a, &fi, &fo, &ti, &to});
+                // A line and set a thread (the envelopes' stacks, 12 bytes a cell of a line for both sets, in LDS) while
+                // a block's lines fit 64 KB and the squared distances stay below 2^24; the outward search from every
+                // cell otherwise.
+                const int n = axis == 0 ? shape.nx : (axis == 1 ? shape.ny : shape.nz);
+                int lines_a_block = 64;
+                while (lines_a_block > 1 && 12LL * lines_a_block * n > 65536) {
+                    lines_a_block /= 2;
+                }
+                if (n <= 2048 && 12LL * lines_a_block * n <= 65536) {
+                    std::vector<std::uint64_t> k = args(dt);
+                    void* argv[] = {k.data(), &w, &a, &fi, &fo, &ti, &to};
+                    const long long lines = static_cast<long long>(shape.cells()) / n;
+                    check(fe_hip_launch(fn(s.kernel + "_lines"), static_cast<unsigned>((lines + lines_a_block - 1) / lines_a_block), 1, 1,
+                              static_cast<unsigned>(2 * lines_a_block), 1, 1, static_cast<unsigned>(12 * lines_a_block * n), argv),
+                        "distance lines launch");
+                    ++prof.launches;
+                } else {
+                    launch_cells(s.kernel + "_scan", dt, {&w, &a, &fi, &fo, &ti, &to});
+                }
             }
             launch_cells(s.kernel + "_signed", dt, {&w});
             ++prof.launches;
