@@ -574,6 +574,23 @@ __global__ __launch_bounds__(64 * kWavesTrain) void k_backward(const bf16x8* __r
     }
 }
 
+// The first level of k_sum's fixed order: column c of rows [chunk * j, chunk * (j + 1)) summed in row order into
+// out[j][c] (grid: columns x chunks, so the rows' loads spread over many waves instead of one wave a 64 columns).
+__global__ void k_sum_chunks(const float* __restrict__ parts, int rows, int width, int chunk, float* __restrict__ out)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = (int)blockIdx.y;
+    if (c >= width) {
+        return;
+    }
+    const int r1 = (j + 1) * chunk < rows ? (j + 1) * chunk : rows;
+    float s = 0.0f;
+    for (int r = j * chunk; r < r1; ++r) {
+        s += parts[(long long)r * width + c];
+    }
+    out[(long long)j * width + c] = s;
+}
+
 // Sums `parts` rows of `width` floats column by column, in row order (fixed: the same bits every run).
 __global__ void k_sum(const float* __restrict__ parts, int rows, int width, float* __restrict__ out)
 {
@@ -734,7 +751,7 @@ float* noalloc_scratch_here()
 } // namespace
 
 struct fe_lnet {
-    Dev<float> params, grad, m, v, partials, loss_parts, loss, F, E;
+    Dev<float> params, grad, m, v, partials, chunk_sums, loss_parts, loss, F, E;
     Dev<bf16x8> frags;
     Dev<unsigned> flags; // [0] breaches of the last step, [1] fallbacks so far
     float scale = 1.0f;
@@ -953,7 +970,13 @@ int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* 
             static_cast<float>(weight), n->E.p, n->loss_parts.p);
         hipLaunchKernelGGL(k_backward, dim3(fe_lnet::kTrainBlocks), dim3(64 * kWavesTrain), 0, s, n->frags.p, n->params.p, u0, v0, n->E.p, L,
             n->scale, faces, n->partials.p);
-        hipLaunchKernelGGL(k_sum, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, n->partials.p, waves, kParams, n->grad.p);
+        // the waves' partials summed in a fixed order: chunks of 32 waves in wave order, then the chunks in order
+        constexpr int kChunk = 32;
+        const int chunks = (waves + kChunk - 1) / kChunk;
+        n->chunk_sums.alloc(static_cast<std::size_t>(chunks) * kParams);
+        hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(kParams, 256), chunks), dim3(256), 0, s, n->partials.p, waves, kParams, kChunk,
+            n->chunk_sums.p);
+        hipLaunchKernelGGL(k_sum, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, n->chunk_sums.p, chunks, kParams, n->grad.p);
         if (loss_out != nullptr) { // the loss's total only when it is asked for (one block: it is not free)
             hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, n->loss_parts.p, cells, n->loss.p);
         }
