@@ -730,7 +730,7 @@ def main() -> int:
     ap.add_argument("--no-adi", action="store_true", help="skip the batched Heston ADI case")
     ap.add_argument("--adi-options", type=int, default=2048)
     ap.add_argument("--adi-steps", type=int, default=512)
-    ap.add_argument("--adi-sample", type=int, default=4, help="options of each shape checked against FP64")
+    ap.add_argument("--adi-sample", type=int, default=16, help="options of each shape checked against FP64")
     ap.add_argument("--no-extra", action="store_true", help="skip the Dirichlet, Black-Scholes, learned and adjoint cases")
     ap.add_argument("--learned-steps", type=int, default=3000)
     ap.add_argument("--adjoint-size", type=int, default=256)

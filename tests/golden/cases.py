@@ -549,6 +549,11 @@ CASES = [
                "smoke": _uniform(24 * 18 * 12, 44, 0.0, 0.5)}),
     dict(name="op_distance_moving", law=MOVING_LAW, grid=(12, 7, 5, 0.25), dt=1.0, steps=5, params={},
          init=dict(_ops_init(12 * 7 * 5, 150), S=_uniform(12 * 7 * 5, 151, -1.0, 1.0))),
+    # x-lines longer than 2048 cells (the distance stage's outward-search form) beside short y and z lines (its
+    # line-envelope form), a sparse set
+    dict(name="op_distance_long", law=op_law(OPS["distance"]), grid=(2100, 3, 2, 0.5), dt=0.02, steps=2, params={},
+         init={"A": _uniform(2100 * 6, 160, 0.1, 0.9),
+               "B": (np.random.default_rng(161).random(2100 * 6) < 0.01).astype(np.float32)}),
 ] + [
     dict(grid=G_OPS, steps=4, params={}, init=_ops_init(int(np.prod(G_OPS[:3])), 200 + i), **dict(dict(dt=0.02), **case))
     for i, case in enumerate(NET_CASES)
