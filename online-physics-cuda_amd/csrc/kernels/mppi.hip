@@ -31,7 +31,8 @@ typedef unsigned long long u64;
 constexpr int kMaxWidth = 128;        // padded inputs and hidden width
 constexpr int kStride = kMaxWidth + 8; // LDS row stride in halves
 constexpr int kMaxState = 64;         // features, padded
-constexpr int kWaves = 4;             // waves a block: futures tiles in flight per agent
+constexpr int kWaves = 1;             // waves a block: one, so a block's barriers wait for no other wave; an agent's
+                                      // futures spread over samples / 16 blocks
 
 struct Desc {
     int F, in_pad, hid, f_pad, horizon, samples;
@@ -99,9 +100,9 @@ __device__ __forceinline__ void layer(const Desc& d, const _Float16 (*in)[kStrid
     }
 }
 
-// A block an agent, a wave a tile of 16 futures at a time (samples a multiple of 16 * kWaves: every wave runs the same
-// number of tiles, so the block's barriers are reached by all). Actions into acts[agent][k][t][axis], each future's
-// cost into cost[agent][k].
+// A block (agent, part), a wave a tile of 16 futures at a time: part p's waves take tiles p * 16 * kWaves + 16 w, then
+// every 16 * kWaves * parts futures on (samples a multiple of 16 * kWaves: every wave runs the same number of tiles, so
+// the block's barriers are reached by all). Actions into acts[agent][k][t][axis], each future's cost into cost[agent][k].
 __global__ __launch_bounds__(64 * kWaves) void k_mppi_rollout(Desc d, const float* __restrict__ obs, const float* __restrict__ mean,
     const unsigned* __restrict__ ids, float* __restrict__ acts, float* __restrict__ cost, u64 tick, float sigma, float dt)
 {
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mppi_rollout(Desc d, const floa
     const int agent = (int)blockIdx.x;
     const int F = d.F, H = d.horizon, K = d.samples;
     const u64 id = ids[agent];
-    for (int t0 = w * 16; t0 < K; t0 += 16 * kWaves) {
+    for (int t0 = (int)blockIdx.y * 16 * kWaves + w * 16; t0 < K; t0 += 16 * kWaves * (int)gridDim.y) {
         for (int q = lane; q < 16 * F; q += 64) {
             st[w][q / F][q % F] = obs[(long long)agent * F + q % F];
         }
@@ -276,7 +277,7 @@ fe_mppi* fe_mppi_create(int features, int hidden, int horizon, int samples, char
     };
     if (features < 1 || features + 2 > kMaxState || hidden < 16 || hidden > kMaxWidth || hidden % 16 != 0 || horizon < 1 || horizon > 64 ||
         samples < 16 * kWaves || samples > 1024 || samples % (16 * kWaves) != 0) {
-        say("features 1..62, hidden 16..128 (a multiple of 16), horizon 1..64, samples 64..1024 (a multiple of 64)");
+        say("features 1..62, hidden 16..128 (a multiple of 16), horizon 1..64, samples 16..1024 (a multiple of 16)");
         return nullptr;
     }
     fe_hip_device_info info{};
@@ -363,7 +364,8 @@ int fe_mppi_plan(fe_mppi* p, int agents, const float* obs, const uint32_t* ids, 
         p->cost.alloc(static_cast<std::size_t>(agents) * K);
         Desc d{F, p->in_pad, p->hid, p->f_pad, H, K, p->W1.p, p->W2.p, p->W3.p, p->b1.p, p->b2.p, p->b3.p, p->shift.p, p->inscale.p,
             p->scale.p, p->target.p, p->weight.p, p->action_weight};
-        hipLaunchKernelGGL(k_mppi_rollout, dim3(static_cast<unsigned>(agents)), dim3(64 * kWaves), 0, s, d, p->obs.p, p->mean.p, p->ids.p,
+        hipLaunchKernelGGL(k_mppi_rollout, dim3(static_cast<unsigned>(agents), static_cast<unsigned>(K / (16 * kWaves))), dim3(64 * kWaves), 0, s,
+            d, p->obs.p, p->mean.p, p->ids.p,
             p->acts.p, p->cost.p, static_cast<unsigned long long>(tick), sigma, dt);
         hipLaunchKernelGGL(k_mppi_update, dim3(static_cast<unsigned>(agents)), dim3(static_cast<unsigned>(pad16(2 * H))), 0, s, p->cost.p,
             p->acts.p, p->mean.p, K, H);

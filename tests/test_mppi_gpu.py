@@ -64,4 +64,4 @@ def test_refuses_shapes_it_does_not_take(device):
     with pytest.raises(field_hip.FieldHipError):
         field_hip.MppiPlanner(F, 24, H, K)  # hidden not a multiple of 16
     with pytest.raises(field_hip.FieldHipError):
-        field_hip.MppiPlanner(F, HID, H, 100)  # samples not a multiple of 64
+        field_hip.MppiPlanner(F, HID, H, 100)  # samples not a multiple of 16
