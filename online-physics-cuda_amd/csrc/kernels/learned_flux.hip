@@ -605,13 +605,18 @@ __global__ void k_sum(const float* __restrict__ parts, int rows, int width, floa
     out[c] = s;
 }
 
-// Sums n floats in one block: each lane a fixed stride, then a fixed tree.
-__global__ __launch_bounds__(256) void k_total(const float* __restrict__ v, long long n, float* __restrict__ out)
+// Sums n floats in two fixed-order levels: block b its chunk (each lane a fixed stride, then a fixed tree) into
+// parts[b], then k_total_fold the blocks' parts in order (one block over all n waited on memory a lane at a time).
+constexpr int kTotalBlocks = 256;
+__global__ __launch_bounds__(256) void k_total(const float* __restrict__ v, long long n, double* __restrict__ parts)
 {
     __shared__ double s[256];
     const int t = (int)threadIdx.x;
+    const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long long begin = (long long)blockIdx.x * chunk;
+    const long long end = begin + chunk < n ? begin + chunk : n;
     double a = 0.0;
-    for (long long i = t; i < n; i += 256) {
+    for (long long i = begin + t; i < end; i += 256) {
         a += (double)v[i];
     }
     s[t] = a;
@@ -623,7 +628,18 @@ __global__ __launch_bounds__(256) void k_total(const float* __restrict__ v, long
         __syncthreads();
     }
     if (t == 0) {
-        *out = (float)s[0];
+        parts[blockIdx.x] = s[0];
+    }
+}
+
+__global__ void k_total_fold(const double* __restrict__ parts, int count, float* __restrict__ out)
+{
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int b = 0; b < count; ++b) {
+            a += parts[b];
+        }
+        *out = (float)a;
     }
 }
 
@@ -752,6 +768,7 @@ float* noalloc_scratch_here()
 
 struct fe_lnet {
     Dev<float> params, grad, m, v, partials, chunk_sums, loss_parts, loss, F, E;
+    Dev<double> loss_blocks;
     Dev<bf16x8> frags;
     Dev<unsigned> flags; // [0] breaches of the last step, [1] fallbacks so far
     float scale = 1.0f;
@@ -977,8 +994,10 @@ int fe_lnet_gradient(fe_lnet* n, const float* u0, const float* v0, const float* 
         hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(kParams, 256), chunks), dim3(256), 0, s, n->partials.p, waves, kParams, kChunk,
             n->chunk_sums.p);
         hipLaunchKernelGGL(k_sum, dim3(blocks_for(kParams, 256)), dim3(256), 0, s, n->chunk_sums.p, chunks, kParams, n->grad.p);
-        if (loss_out != nullptr) { // the loss's total only when it is asked for (one block: it is not free)
-            hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, n->loss_parts.p, cells, n->loss.p);
+        if (loss_out != nullptr) { // the loss's total only when it is asked for
+            n->loss_blocks.alloc(kTotalBlocks);
+            hipLaunchKernelGGL(k_total, dim3(kTotalBlocks), dim3(256), 0, s, n->loss_parts.p, cells, n->loss_blocks.p);
+            hipLaunchKernelGGL(k_total_fold, dim3(1), dim3(64), 0, s, n->loss_blocks.p, kTotalBlocks, n->loss.p);
         }
         hcheck(hipGetLastError(), "gradient");
         if (grad_out != nullptr) {

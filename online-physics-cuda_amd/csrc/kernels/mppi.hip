@@ -158,23 +158,40 @@ __global__ __launch_bounds__(64 * kWaves) void k_mppi_rollout(Desc d, const floa
 }
 
 // The new plan: a thread an (step, axis); the futures' actions weighted by exp(-(J - J_min) / lambda), lambda a quarter
-// of (mean cost - least cost), at least 1e-4, summed in the futures' order.
+// of (mean cost - least cost), at least 1e-4, summed in the futures' order. The least cost, the sum of costs (in order)
+// and each future's weight are found once a block (in LDS) instead of by every thread.
+constexpr int kMaxSamples = 1024;
 __global__ void k_mppi_update(const float* __restrict__ cost, const float* __restrict__ acts, float* __restrict__ mean, int K, int H)
 {
+    __shared__ float cs[kMaxSamples];
+    __shared__ float lo_s, lam_s;
     const int agent = (int)blockIdx.x, q = (int)threadIdx.x;
+    const float* c = cost + (long long)agent * K;
+    for (int k = q; k < K; k += (int)blockDim.x) {
+        cs[k] = c[k];
+    }
+    __syncthreads();
+    if (q == 0) {
+        float lo = cs[0], sum = 0.0f;
+        for (int k = 0; k < K; ++k) {
+            lo = fminf(lo, cs[k]);
+            sum += cs[k];
+        }
+        lo_s = lo;
+        lam_s = fmaxf(1e-4f, 0.25f * (sum / (float)K - lo));
+    }
+    __syncthreads();
+    const float lo = lo_s, lam = lam_s;
+    for (int k = q; k < K; k += (int)blockDim.x) {
+        cs[k] = expf(-(cs[k] - lo) / lam);
+    }
+    __syncthreads();
     if (q >= 2 * H) {
         return;
     }
-    const float* c = cost + (long long)agent * K;
-    float lo = c[0], sum = 0.0f;
-    for (int k = 0; k < K; ++k) {
-        lo = fminf(lo, c[k]);
-        sum += c[k];
-    }
-    const float lam = fmaxf(1e-4f, 0.25f * (sum / (float)K - lo));
     float wsum = 0.0f, acc = 0.0f;
     for (int k = 0; k < K; ++k) {
-        const float wk = expf(-(c[k] - lo) / lam);
+        const float wk = cs[k];
         wsum += wk;
         acc += wk * acts[((long long)agent * K + k) * 2 * H + q];
     }
