@@ -144,6 +144,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         "fe_mppi_set_dynamics": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
         "fe_mppi_set_cost": (c_int, [c_void_p, c_void_p, c_void_p, c_float]),
         "fe_mppi_plan": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_uint64, c_float, c_float, c_void_p, c_void_p]),
+        "fe_nettrain_create": (c_void_p, [c_void_p, c_int, c_int, ctypes.c_char_p, c_size_t]),
+        "fe_nettrain_free": (None, [c_void_p]),
+        "fe_nettrain_error": (ctypes.c_char_p, [c_void_p]),
+        "fe_nettrain_image_size": (c_int, [c_void_p]),
+        "fe_nettrain_set": (c_int, [c_void_p, c_void_p]),
+        "fe_nettrain_get": (c_int, [c_void_p, c_void_p]),
+        "fe_nettrain_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+        "fe_nettrain_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_float, c_float, c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -740,3 +748,83 @@ class MppiPlanner:
         self._check(lib().fe_mppi_plan(self.ptr, agents, obs.ctypes.data, ids.ctypes.data, int(tick), float(sigma), float(dt),
                                        out.ctypes.data, c.ctypes.data if costs else None), "plan")
         return out, c
+
+
+ACTIVATIONS = ("tanh", "relu", "silu", "linear", "gelu")
+
+
+class NetTrainer:
+    """A law net's trainer on the card (include/fe_nettrain.h): sizes (inputs, hidden widths, outputs), a hidden
+    activation by name, the weights in the .net image layout (in_shift, in_scale, out_scale, then W[out][in] and b[out]
+    per layer); the loss is the mean over rows of 0.5 m |out_scale f(x) - t|^2, Adam clips each gradient first."""
+
+    def __init__(self, sizes, act: str = "tanh"):
+        import numpy as np
+
+        if act not in ACTIVATIONS:
+            raise FieldHipError(f"activation {act!r}: one of {', '.join(ACTIVATIONS)}")
+        self.sizes = [int(v) for v in sizes]
+        self.act = act
+        arr = np.ascontiguousarray(self.sizes, np.int32)
+        why = ctypes.create_string_buffer(1024)
+        self.ptr = lib().fe_nettrain_create(arr.ctypes.data, arr.size, ACTIVATIONS.index(act), why, len(why))
+        if not self.ptr:
+            raise FieldHipError("trainer refused: " + why.value.decode())
+        self.image_size = lib().fe_nettrain_image_size(self.ptr)
+        self.trained = self.image_size - 2 * self.sizes[0] - 1
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fe_nettrain_free(self.ptr)
+            self.ptr = None
+
+    def _check(self, status: int, what: str) -> None:
+        if status != 0:
+            raise FieldHipError(f"{what}: {lib().fe_nettrain_error(self.ptr).decode()}")
+
+    def set_image(self, image) -> None:
+        import numpy as np
+
+        a = np.ascontiguousarray(image, np.float32)
+        if a.size != self.image_size:
+            raise FieldHipError(f"image of {a.size} floats, the net's is {self.image_size}")
+        self._check(lib().fe_nettrain_set(self.ptr, a.ctypes.data), "set")
+
+    def image(self):
+        import numpy as np
+
+        out = np.empty(self.image_size, np.float32)
+        self._check(lib().fe_nettrain_get(self.ptr, out.ctypes.data), "get")
+        return out
+
+    def _rows(self, x, target, m):
+        import numpy as np
+
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        rows = x.shape[0] if x.ndim == 2 else 0
+        if x.ndim != 2 or x.shape[1] != self.sizes[0] or t.shape != (rows, self.sizes[-1]):
+            raise FieldHipError(f"rows: x {x.shape} and targets {t.shape} for sizes {self.sizes}")
+        mm = None if m is None else np.ascontiguousarray(m, np.float32)
+        if mm is not None and mm.shape != (rows,):
+            raise FieldHipError(f"row weights {mm.shape} for {rows} rows")
+        return x, t, mm, rows
+
+    def gradient(self, x, target, m=None):
+        """(loss, gradient of the trained floats) for the rows; no update."""
+        import numpy as np
+
+        x, t, mm, rows = self._rows(x, target, m)
+        g = np.empty(self.trained, np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().fe_nettrain_gradient(self.ptr, x.ctypes.data, t.ctypes.data, None if mm is None else mm.ctypes.data, rows,
+                                               ctypes.byref(loss), g.ctypes.data), "gradient")
+        return loss.value, g
+
+    def step(self, x, target, m=None, lr: float = 2e-3, clip: float = 5.0) -> float:
+        """One Adam step on the rows; the loss before it."""
+        x, t, mm, rows = self._rows(x, target, m)
+        loss = ctypes.c_float()
+        self._check(lib().fe_nettrain_step(self.ptr, x.ctypes.data, t.ctypes.data, None if mm is None else mm.ctypes.data, rows,
+                                           float(lr), float(clip), ctypes.byref(loss)), "step")
+        return loss.value
