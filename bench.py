@@ -431,6 +431,54 @@ def mppi_plan(cpu: bool, plans: int = 20):
     return out
 
 
+def net_train(cpu: bool, steps: int = 50):
+    """A law net's trainer (include/fe_nettrain.h) on the world model's dynamics net, 16 + 2 -> 64 -> 64 -> 16 (tanh):
+    one Adam step a call (it uploads the rows and targets, runs the tiles, the two fixed-order sums and Adam, and reads
+    the loss back), at the world model's batch of 1024 rows and at 65 536 rows; seeded random weights, rows a smooth
+    map of the inputs."""
+    from oracle import nettrain as ref
+
+    F = 16
+    sizes = [F + 2, 64, 64, F]
+    flop_row = 2.0 * (3 * ((F + 2) * 64 + 64 * 64 + 64 * F) - (F + 2) * 64)  # forward, weight gradients, deltas below layer 1
+    rng = np.random.default_rng(31)
+    A = rng.uniform(-1, 1, (F, F + 2)) / np.sqrt(F + 2)
+    image = ref.make_image(sizes, 32, shift=0.0, scale=1.0, out_scale=1.0).astype(np.float32)
+    out = {"sizes": sizes, "activation": "tanh"}
+    for rows in (1024, 65536):
+        x = rng.uniform(-1, 1, (rows, F + 2)).astype(np.float32)
+        t = np.sin(x @ A.T).astype(np.float32)
+        tr = field_hip.NetTrainer(sizes, "tanh")
+        tr.set_image(image)
+        first = tr.step(x, t)
+        for _ in range(3):
+            tr.step(x, t)
+        times = []
+        for _ in range(steps):
+            s = time.perf_counter()
+            last = tr.step(x, t)
+            times.append((time.perf_counter() - s) * 1e3)
+        ms = float(np.median(times))
+        out[f"rows_{rows}"] = {"ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "tflops": flop_row * rows / (ms * 1e9),
+                               "first_loss": first, "loss_after": last}
+    if cpu:
+        rows = 1024
+        x = rng.uniform(-1, 1, (rows, F + 2))
+        t = np.sin(x @ A.T)
+        img, base = image.astype(np.float64), 2 * sizes[0] + 1
+        m1, m2 = np.zeros(img.size - base), np.zeros(img.size - base)
+        n = 0
+        s = time.perf_counter()
+        while time.perf_counter() - s < 5.0:
+            n += 1
+            _, g = ref.gradient(sizes, "tanh", img, x, t)
+            ref.adam(img, g, m1, m2, n, sizes, 2e-3, 5.0)
+        sec = time.perf_counter() - s
+        out["cpu_baseline"] = {"value": n * rows / sec, "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle NumPy restatement (float64), {n} Adam steps of {rows} rows, {sec:.1f} s"}
+    return out
+
+
 def hbm_roof(n: int = 512, reps: int = 10):
     """The attainable HBM rate at n^3 floats (512^3: 512 MiB each way, twice the last-level cache): a copy and a read
     swept over loads in flight a lane (1, 2, 4, 8), ordinary or nontemporal accesses, and 2, 4 or 8 blocks of 256
@@ -746,6 +794,7 @@ def main() -> int:
                 "black_scholes_4096x512": lambda: black_scholes(200),
                 "implicit_lines_256": lambda: implicit_lines(256, 50),
                 "mppi_plan": lambda: mppi_plan(not args.no_cpu),
+                "net_train_world_model": lambda: net_train(not args.no_cpu),
                 "hbm_roof_512": lambda: hbm_roof(512),
                 "world_valley_144x144x48": lambda: world_valley(),
                 "learned_coarse": lambda: learned_coarse(not args.no_cpu, args.learned_steps, 4),

@@ -32,7 +32,8 @@ constexpr int kMaxWidth = 128;  // any layer's width
 constexpr int kRows = 16;       // rows a tile (a wave)
 constexpr int kChunk = 32;      // blocks' partials summed together in the first level
 constexpr int kBlocks = 1024;   // the most blocks a pass (four waves a CU)
-constexpr int kMaxLds = 65536;  // bytes of LDS a tile may use
+constexpr int kMaxLds = 65536;  // bytes of LDS a block may use
+constexpr int kWaves = 4;       // waves a block: a tile's 16-column blocks spread over them
 
 __host__ __device__ inline int pad4(int v) { return (v + 3) / 4 * 4; }
 
@@ -41,13 +42,19 @@ struct Shape {
     int sizes[kMaxLayers + 1] = {};
     int act = 0;
     int w_at[kMaxLayers] = {};  // layer k's W in the image (its b follows W)
-    int a_at[kMaxLayers] = {};  // LDS floats: layer k's inputs, 16 x pad4(sizes[k])
+    int a_at[kMaxLayers] = {};  // LDS floats: layer k's inputs, 16 rows of a_ld(sizes[k])
     int p_at[kMaxLayers] = {};  // LDS floats: hidden layer k's pre-activations, 16 x sizes[k + 1]
-    int d_at[2] = {};           // LDS floats: two delta buffers, 16 x kMaxWidth each
-    int lds = 0;                // LDS floats a tile
+    int d_at[2] = {};           // LDS floats: two delta buffers, 16 rows of d_ld each
+    int d_ld = 0;               // a delta row's stride: the widest layer output, padded to 4, plus 1
+    int loss_at = 0;            // LDS floats: the waves' losses
+    int lds = 0;                // LDS floats a block
     int image = 0;              // image floats
     int trained = 0;            // weights and biases: image - 2 in - 1
 };
+
+// An input row's stride in LDS: padded to 4 (a product's depth steps by 4; the padding stays zero) plus 1, so the 16
+// rows a product reads at once fall in different banks.
+__host__ __device__ inline int a_ld(int n) { return pad4(n) + 1; }
 
 __device__ __forceinline__ float act_f(int a, float x)
 {
@@ -85,19 +92,21 @@ __device__ __forceinline__ float act_d(int a, float x)
 }
 
 // The tiles blockIdx.x, blockIdx.x + gridDim.x, ... of 16 rows each, in that order: forward, loss, backward; the
-// block's gradient summed over its tiles into partial[blockIdx.x][trained] (each lane adds to the same elements for
-// every tile, so the order is fixed), its loss into loss_part[blockIdx.x]. LDS: s.lds floats (the launch gives them).
-__global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict__ w, const float* __restrict__ X,
-    const float* __restrict__ T, const float* __restrict__ M, int rows, float* __restrict__ partial, float* __restrict__ loss_part)
+// block's gradient summed over its tiles into partial[blockIdx.x][0, trained), its loss into column trained (each
+// thread adds to the same elements for every tile, so the order is fixed). The four waves share a tile: each product's
+// 16-column blocks (or the weight gradient's 16 x 16 blocks) go round the waves. LDS: s.lds floats (the launch gives
+// them).
+__global__ __launch_bounds__(64 * kWaves) void k_nt_tile(Shape s, const float* __restrict__ w, const float* __restrict__ X,
+    const float* __restrict__ T, const float* __restrict__ M, int rows, float* __restrict__ partial)
 {
     extern __shared__ __attribute__((aligned(16))) float sh[];
-    const int lane = (int)threadIdx.x, c16 = lane & 15, g4 = lane >> 4;
-    const int in = s.sizes[0], out = s.sizes[s.layers], in4 = pad4(in);
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6, c16 = lane & 15, g4 = lane >> 4;
+    const int in = s.sizes[0], out = s.sizes[s.layers], ld0 = a_ld(in), dl = s.d_ld;
     const int tiles = (rows + kRows - 1) / kRows, base = 2 * in + 1;
     const float scale = w[2 * in];
-    float* P = partial + (long long)blockIdx.x * s.trained;
+    float* P = partial + (long long)blockIdx.x * (s.trained + 1);
     // padding columns stay zero for every tile: only the first sizes[k] columns of a row are written
-    for (int q = lane; q < s.lds; q += 64) {
+    for (int q = tid; q < s.lds; q += 64 * kWaves) {
         sh[q] = 0.0f;
     }
     __syncthreads();
@@ -106,23 +115,23 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
         const int r0 = tile * kRows;
         const bool first = tile == (int)blockIdx.x;
         float* a0 = sh + s.a_at[0];
-        for (int q = lane; q < kRows * in; q += 64) {
+        for (int q = tid; q < kRows * in; q += 64 * kWaves) {
             const int r = q / in, i = q % in, row = r0 + r;
-            a0[r * in4 + i] = row < rows ? (X[(long long)row * in + i] - w[i]) * w[in + i] : 0.0f;
+            a0[r * ld0 + i] = row < rows ? (X[(long long)row * in + i] - w[i]) * w[in + i] : 0.0f;
         }
         __syncthreads();
         for (int k = 0; k < s.layers; ++k) {
-            const int ni = s.sizes[k], no = s.sizes[k + 1], ni4 = pad4(ni), no4 = pad4(no);
+            const int ni = s.sizes[k], no = s.sizes[k + 1], ni4 = pad4(ni), la = a_ld(ni), lo = a_ld(no);
             const float* A = sh + s.a_at[k];
             const float* W = w + s.w_at[k];
             const float* b = W + (long long)no * ni;
             const bool last = k + 1 == s.layers;
-            for (int nb = 0; nb < no; nb += 16) {
+            for (int nb = 16 * wave; nb < no; nb += 16 * kWaves) {
                 const int col = nb + c16;
                 float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 for (int kc = 0; kc < ni4; kc += 4) {
                     const int kk = kc + g4;
-                    const float av = A[c16 * ni4 + kk];
+                    const float av = A[c16 * la + kk];
                     const float bv = col < no && kk < ni ? W[(long long)col * ni + kk] : 0.0f;
                     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
                 }
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
                         const float pre = acc[j] + b[col];
                         if (!last) {
                             sh[s.p_at[k] + r * no + col] = pre;
-                            sh[s.a_at[k + 1] + r * no4 + col] = act_f(s.act, pre);
+                            sh[s.a_at[k + 1] + r * lo + col] = act_f(s.act, pre);
                         } else {
                             const int row = r0 + r;
                             float d = 0.0f;
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
                                 loss += 0.5f * m * e * e;
                                 d = m * e * scale;
                             }
-                            sh[s.d_at[0] + r * kMaxWidth + col] = d;
+                            sh[s.d_at[0] + r * dl + col] = d;
                         }
                     }
                 }
@@ -151,35 +160,37 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
         }
         int cur = 0;
         for (int k = s.layers - 1; k >= 0; --k) {
-            const int ni = s.sizes[k], no = s.sizes[k + 1], ni4 = pad4(ni), no4 = pad4(no);
+            const int ni = s.sizes[k], no = s.sizes[k + 1], la = a_ld(ni);
             const float* A = sh + s.a_at[k];
             const float* D = sh + s.d_at[cur];
             const int g_at = s.w_at[k] - base;
-            // weights: G[o][i] = sum over the tile's rows of D[r][o] A[r][i] (A: o x rows, B: rows x i)
-            for (int ob = 0; ob < no; ob += 16) {
-                for (int ib = 0; ib < ni; ib += 16) {
-                    float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    for (int rc = 0; rc < kRows; rc += 4) {
-                        const int r = rc + g4;
-                        const float av = ob + c16 < no ? D[r * kMaxWidth + ob + c16] : 0.0f;
-                        const float bv = ib + c16 < ni ? A[r * ni4 + ib + c16] : 0.0f;
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-                    }
-                    const int i = ib + c16;
-                    for (int j = 0; j < 4; ++j) {
-                        const int o = ob + 4 * g4 + j;
-                        if (o < no && i < ni) {
-                            float* g = P + g_at + o * ni + i;
-                            *g = first ? acc[j] : *g + acc[j];
-                        }
+            // weights: G[o][i] = sum over the tile's rows of D[r][o] A[r][i] (A: o x rows, B: rows x i), a 16 x 16 block
+            // a wave at a time
+            const int nib = (ni + 15) / 16, blocks = (no + 15) / 16 * nib;
+            for (int q = wave; q < blocks; q += kWaves) {
+                const int ob = q / nib * 16, ib = q % nib * 16;
+                float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int rc = 0; rc < kRows; rc += 4) {
+                    const int r = rc + g4;
+                    const float av = ob + c16 < no ? D[r * dl + ob + c16] : 0.0f;
+                    const float bv = ib + c16 < ni ? A[r * la + ib + c16] : 0.0f;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                }
+                const int i = ib + c16;
+                for (int j = 0; j < 4; ++j) {
+                    const int o = ob + 4 * g4 + j;
+                    if (o < no && i < ni) {
+                        float* g = P + g_at + o * ni + i;
+                        *g = first ? acc[j] : *g + acc[j];
                     }
                 }
             }
             // biases: the rows in order
-            for (int o = lane; o < no; o += 64) {
+            for (int o = tid; o < no; o += 64 * kWaves) {
                 float g = 0.0f;
                 for (int r = 0; r < kRows; ++r) {
-                    g += D[r * kMaxWidth + o];
+                    g += D[r * dl + o];
                 }
                 float* gp = P + g_at + no * ni + o;
                 *gp = first ? g : *gp + g;
@@ -188,19 +199,20 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
                 // the deltas below: (D W)[r][i] times the activation's derivative at the pre-activation
                 const float* W = w + s.w_at[k];
                 float* Dn = sh + s.d_at[cur ^ 1];
-                for (int ib = 0; ib < ni; ib += 16) {
+                const int no4 = pad4(no);
+                for (int ib = 16 * wave; ib < ni; ib += 16 * kWaves) {
                     const int i = ib + c16;
                     float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
                     for (int oc = 0; oc < no4; oc += 4) {
                         const int o = oc + g4;
-                        const float av = o < no ? D[c16 * kMaxWidth + o] : 0.0f;
+                        const float av = o < no ? D[c16 * dl + o] : 0.0f;
                         const float bv = o < no && i < ni ? W[(long long)o * ni + i] : 0.0f;
                         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
                     }
                     if (i < ni) {
                         for (int j = 0; j < 4; ++j) {
                             const int r = 4 * g4 + j;
-                            Dn[r * kMaxWidth + i] = acc[j] * act_d(s.act, sh[s.p_at[k - 1] + r * ni + i]);
+                            Dn[r * dl + i] = acc[j] * act_d(s.act, sh[s.p_at[k - 1] + r * ni + i]);
                         }
                     }
                 }
@@ -209,23 +221,31 @@ __global__ __launch_bounds__(64) void k_nt_tile(Shape s, const float* __restrict
             __syncthreads();
         }
     }
-    // the block's loss: the lanes' shares folded by a fixed butterfly
+    // the block's loss: each wave's lanes folded by a fixed butterfly, then the waves in order
     for (int o = 32; o > 0; o >>= 1) {
         loss += __shfl_xor(loss, o);
     }
     if (lane == 0) {
-        loss_part[blockIdx.x] = loss;
+        sh[s.loss_at + wave] = loss;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float l = 0.0f;
+        for (int v = 0; v < kWaves; ++v) {
+            l += sh[s.loss_at + v];
+        }
+        P[s.trained] = l;
     }
 }
 
 // First level: column c of the blocks' partials [chunk j, chunk j + 1) summed in block order into out[j][c].
-__global__ void k_nt_chunks(const float* __restrict__ parts, int tiles, int width, float* __restrict__ out)
+__global__ void k_nt_chunks(const float* __restrict__ parts, int blocks, int width, float* __restrict__ out)
 {
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y;
     if (c >= width) {
         return;
     }
-    const int t1 = (j + 1) * kChunk < tiles ? (j + 1) * kChunk : tiles;
+    const int t1 = (j + 1) * kChunk < blocks ? (j + 1) * kChunk : blocks;
     float s = 0.0f;
     for (int t = j * kChunk; t < t1; ++t) {
         s += parts[(long long)t * width + c];
@@ -233,9 +253,8 @@ __global__ void k_nt_chunks(const float* __restrict__ parts, int tiles, int widt
     out[(long long)j * width + c] = s;
 }
 
-// Second level: the chunks in order, divided by the rows (the mean); the blocks' losses in order, likewise.
-__global__ void k_nt_fold(const float* __restrict__ chunks, int count, int width, float rows_f, float* __restrict__ grad,
-    const float* __restrict__ loss_part, int tiles, float* __restrict__ loss)
+// Second level: the chunks in order, divided by the rows (the mean); the last column is the loss.
+__global__ void k_nt_fold(const float* __restrict__ chunks, int count, int width, float rows_f, float* __restrict__ grad)
 {
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c < width) {
@@ -244,13 +263,6 @@ __global__ void k_nt_fold(const float* __restrict__ chunks, int count, int width
             s += chunks[(long long)j * width + c];
         }
         grad[c] = s / rows_f;
-    }
-    if (c == 0) {
-        float s = 0.0f;
-        for (int t = 0; t < tiles; ++t) {
-            s += loss_part[t];
-        }
-        *loss = s / rows_f;
     }
 }
 
@@ -314,28 +326,27 @@ unsigned blocks_for(long long n, int per) { return static_cast<unsigned>((n + pe
 
 struct fe_nettrain {
     Shape s;
-    Dev<float> w, m1, m2, x, t, m, partial, chunks, grad, loss_part, loss;
+    Dev<float> w, m1, m2, x, t, m, partial, chunks, grad;
     int adam_t = 0;
     std::string error;
     hipStream_t stream() const { return static_cast<hipStream_t>(fe_hip_stream()); }
+    const float* loss() const { return grad.p + s.trained; }
 
-    // The gradient and loss for the rows already uploaded; leaves the mean gradient in grad, the loss in loss.
+    // The gradient and loss for the rows already uploaded; leaves the mean gradient in grad[0, trained), the mean loss
+    // in grad[trained].
     void gradient(int rows, bool masked)
     {
         const hipStream_t st = stream();
-        const int tiles = std::min((rows + kRows - 1) / kRows, kBlocks);
-        partial.alloc(static_cast<std::size_t>(tiles) * s.trained);
-        loss_part.alloc(static_cast<std::size_t>(tiles));
-        const int nchunks = (tiles + kChunk - 1) / kChunk;
-        chunks.alloc(static_cast<std::size_t>(nchunks) * s.trained);
-        grad.alloc(static_cast<std::size_t>(s.trained));
-        loss.alloc(1);
-        hipLaunchKernelGGL(k_nt_tile, dim3(static_cast<unsigned>(tiles)), dim3(64), static_cast<unsigned>(s.lds) * 4u, st, s, w.p, x.p, t.p,
-            masked ? m.p : nullptr, rows, partial.p, loss_part.p);
-        hipLaunchKernelGGL(k_nt_chunks, dim3(blocks_for(s.trained, 256), static_cast<unsigned>(nchunks)), dim3(256), 0, st, partial.p, tiles,
-            s.trained, chunks.p);
-        hipLaunchKernelGGL(k_nt_fold, dim3(blocks_for(s.trained, 256)), dim3(256), 0, st, chunks.p, nchunks, s.trained, static_cast<float>(rows),
-            grad.p, loss_part.p, tiles, loss.p);
+        const int blocks = std::min((rows + kRows - 1) / kRows, kBlocks), width = s.trained + 1;
+        partial.alloc(static_cast<std::size_t>(blocks) * width);
+        const int nchunks = (blocks + kChunk - 1) / kChunk;
+        chunks.alloc(static_cast<std::size_t>(nchunks) * width);
+        grad.alloc(static_cast<std::size_t>(width));
+        hipLaunchKernelGGL(k_nt_tile, dim3(static_cast<unsigned>(blocks)), dim3(64 * kWaves), static_cast<unsigned>(s.lds) * 4u, st, s, w.p, x.p,
+            t.p, masked ? m.p : nullptr, rows, partial.p);
+        hipLaunchKernelGGL(k_nt_chunks, dim3(blocks_for(width, 256), static_cast<unsigned>(nchunks)), dim3(256), 0, st, partial.p, blocks, width,
+            chunks.p);
+        hipLaunchKernelGGL(k_nt_fold, dim3(blocks_for(width, 256)), dim3(256), 0, st, chunks.p, nchunks, width, static_cast<float>(rows), grad.p);
         hcheck(hipGetLastError(), "gradient launch");
     }
 
@@ -385,20 +396,23 @@ fe_nettrain* fe_nettrain_create(const int* sizes, int count, int act, char* why,
         }
         s.sizes[k] = sizes[k];
     }
-    int at = 2 * sizes[0] + 1, lds = 0;
+    int at = 2 * sizes[0] + 1, lds = 0, widest = 0;
     for (int k = 0; k < s.layers; ++k) {
         s.w_at[k] = at;
         at += s.sizes[k] * s.sizes[k + 1] + s.sizes[k + 1];
         s.a_at[k] = lds;
-        lds += kRows * pad4(s.sizes[k]);
+        lds += kRows * a_ld(s.sizes[k]);
+        widest = std::max(widest, s.sizes[k + 1]);
     }
     for (int k = 0; k + 1 < s.layers; ++k) {
         s.p_at[k] = lds;
         lds += kRows * s.sizes[k + 1];
     }
+    s.d_ld = a_ld(widest);
     s.d_at[0] = lds;
-    s.d_at[1] = lds + kRows * kMaxWidth;
-    s.lds = lds + 2 * kRows * kMaxWidth;
+    s.d_at[1] = lds + kRows * s.d_ld;
+    s.loss_at = lds + 2 * kRows * s.d_ld;
+    s.lds = s.loss_at + kWaves;
     s.image = at;
     s.trained = at - (2 * sizes[0] + 1);
     if (s.lds * 4 > kMaxLds) {
@@ -483,7 +497,7 @@ int fe_nettrain_gradient(fe_nettrain* t, const float* x, const float* target, co
             hcheck(hipMemcpyAsync(grad, t->grad.p, sizeof(float) * static_cast<std::size_t>(t->s.trained), hipMemcpyDeviceToHost, st), "download");
         }
         float l = 0.0f;
-        hcheck(hipMemcpyAsync(&l, t->loss.p, sizeof(float), hipMemcpyDeviceToHost, st), "download");
+        hcheck(hipMemcpyAsync(&l, t->loss(), sizeof(float), hipMemcpyDeviceToHost, st), "download");
         hcheck(hipStreamSynchronize(st), "sync");
         if (loss != nullptr) {
             *loss = l;
@@ -512,7 +526,7 @@ int fe_nettrain_step(fe_nettrain* t, const float* x, const float* target, const 
             t->s.trained, lr, b1, b2, c1, c2, clip);
         hcheck(hipGetLastError(), "adam launch");
         float l = 0.0f;
-        hcheck(hipMemcpyAsync(&l, t->loss.p, sizeof(float), hipMemcpyDeviceToHost, st), "download");
+        hcheck(hipMemcpyAsync(&l, t->loss(), sizeof(float), hipMemcpyDeviceToHost, st), "download");
         hcheck(hipStreamSynchronize(st), "sync");
         if (loss != nullptr) {
             *loss = l;
