@@ -35,7 +35,6 @@ constexpr int kBlocks = 1024;   // the most blocks a pass (four waves a CU)
 constexpr int kMaxLds = 65536;  // bytes of LDS a block may use
 constexpr int kWaves = 4;       // waves a block: a tile's 16-column blocks spread over them
 
-__host__ __device__ inline int pad4(int v) { return (v + 3) / 4 * 4; }
 
 struct Shape {
     int layers = 0;
@@ -52,9 +51,11 @@ struct Shape {
     int trained = 0;            // weights and biases: image - 2 in - 1
 };
 
-// An input row's stride in LDS: padded to 4 (a product's depth steps by 4; the padding stays zero) plus 1, so the 16
-// rows a product reads at once fall in different banks.
-__host__ __device__ inline int a_ld(int n) { return pad4(n) + 1; }
+__host__ __device__ inline int pad16(int v) { return (v + 15) / 16 * 16; }
+
+// An input row's stride in LDS: padded to 16 (a product's depth goes 16 at a time, four steps of 4 whose loads are
+// issued together; the padding stays zero) plus 1, so the 16 rows a product reads at once fall in different banks.
+__host__ __device__ inline int a_ld(int n) { return pad16(n) + 1; }
 
 __device__ __forceinline__ float act_f(int a, float x)
 {
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_nt_tile(Shape s, const float* _
         }
         __syncthreads();
         for (int k = 0; k < s.layers; ++k) {
-            const int ni = s.sizes[k], no = s.sizes[k + 1], ni4 = pad4(ni), la = a_ld(ni), lo = a_ld(no);
+            const int ni = s.sizes[k], no = s.sizes[k + 1], ni16 = pad16(ni), la = a_ld(ni), lo = a_ld(no);
             const float* A = sh + s.a_at[k];
             const float* W = w + s.w_at[k];
             const float* b = W + (long long)no * ni;
@@ -129,11 +130,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_nt_tile(Shape s, const float* _
             for (int nb = 16 * wave; nb < no; nb += 16 * kWaves) {
                 const int col = nb + c16;
                 float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int kc = 0; kc < ni4; kc += 4) {
-                    const int kk = kc + g4;
-                    const float av = A[c16 * la + kk];
-                    const float bv = col < no && kk < ni ? W[(long long)col * ni + kk] : 0.0f;
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                for (int kc = 0; kc < ni16; kc += 16) {
+                    float av[4], bv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = kc + 4 * u + g4;
+                        av[u] = A[c16 * la + kk];
+                        bv[u] = col < no && kk < ni ? W[(long long)col * ni + kk] : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+                    }
                 }
                 if (col < no) {
                     for (int j = 0; j < 4; ++j) {
@@ -199,15 +207,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_nt_tile(Shape s, const float* _
                 // the deltas below: (D W)[r][i] times the activation's derivative at the pre-activation
                 const float* W = w + s.w_at[k];
                 float* Dn = sh + s.d_at[cur ^ 1];
-                const int no4 = pad4(no);
+                const int no16 = pad16(no);
                 for (int ib = 16 * wave; ib < ni; ib += 16 * kWaves) {
                     const int i = ib + c16;
                     float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    for (int oc = 0; oc < no4; oc += 4) {
-                        const int o = oc + g4;
-                        const float av = o < no ? D[c16 * dl + o] : 0.0f;
-                        const float bv = o < no && i < ni ? W[(long long)o * ni + i] : 0.0f;
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                    for (int oc = 0; oc < no16; oc += 16) {
+                        float av[4], bv[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int o = oc + 4 * u + g4;
+                            av[u] = o < no ? D[c16 * dl + o] : 0.0f;
+                            bv[u] = o < no && i < ni ? W[(long long)o * ni + i] : 0.0f;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+                        }
                     }
                     if (i < ni) {
                         for (int j = 0; j < 4; ++j) {
