@@ -29,10 +29,7 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 typedef unsigned long long u64;
 
 constexpr int kMaxWidth = 128;        // padded inputs and hidden width
-constexpr int kStride = kMaxWidth + 8; // LDS row stride in halves
 constexpr int kMaxState = 64;         // features, padded
-constexpr int kWaves = 1;             // waves a block: one, so a block's barriers wait for no other wave; an agent's
-                                      // futures spread over samples / 16 blocks
 
 struct Desc {
     int F, in_pad, hid, f_pad, horizon, samples;
@@ -61,27 +58,52 @@ __host__ __device__ inline u64 mix64(u64 h)
     h ^= h >> 31;
     return h;
 }
-__host__ __device__ inline u64 noise_key(u64 id, u64 tick, u64 k, u64 t, u64 d)
+// The part of a key that is the same for every future and step of one agent's plan; the key of (future k, step t,
+// axis d) is noise_base ^ (k << 24) ^ (t << 8) ^ d.
+__host__ __device__ inline u64 noise_base(u64 id, u64 tick)
 {
-    return mix64(id * 0x9e3779b97f4a7c15ull + 1) ^ mix64(tick * 0xd1b54a32d192ed03ull + 2) ^ (k << 24) ^ (t << 8) ^ d;
+    return mix64(id * 0x9e3779b97f4a7c15ull + 1) ^ mix64(tick * 0xd1b54a32d192ed03ull + 2);
 }
+// The hardware's logarithm, square root and cosine (v_log_f32, v_sqrt_f32, v_cos_f32, the last taking turns, so
+// cos(2 pi u2) is one instruction): a few ulp, where the library's range reductions were a large share of a step.
 __device__ inline float gauss(u64 key)
 {
     const u64 a = mix64(key), b = mix64(a ^ 0x632be59bd9b4e019ull);
     const float u1 = ((float)(a >> 40) + 0.5f) * (1.0f / 16777216.0f);
     const float u2 = ((float)(b >> 40) + 0.5f) * (1.0f / 16777216.0f);
-    return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+    return __builtin_amdgcn_sqrtf(-2.0f * 0.6931471805599453f * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
 }
 
+// tanh u = 1 - 2 / (1 + e^(2u)) through the hardware exponential and reciprocal: a few ulp of fp32, below the fp16
+// rounding of the activation it becomes; the library tanhf's long polynomial sequence was most of a layer's issue slots.
+// Saturates to +-1 (e^(2u) overflowing to inf, or to 0).
+__device__ __forceinline__ float fast_tanh(float u)
+{
+    const float e = __expf(2.0f * u);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+// The block's LDS, sized to the net (a block is one wave): two fp16 activation tiles of 16 rows at a stride of ts
+// halves (the wider of the padded inputs and the hidden width, plus 8 so the 16 rows a product reads start in
+// different banks), the 16 futures' states at a stride of ss floats (odd), and their actions for two steps.
+struct Lds {
+    int ts, ss;
+    __host__ __device__ static Lds of(const Desc& d)
+    {
+        return Lds{(d.in_pad > d.hid ? d.in_pad : d.hid) + 8, d.F | 1};
+    }
+    __host__ __device__ int bytes() const { return 2 * 16 * ts * 2 + 16 * ss * 4 + 2 * 16 * 2 * 4; }
+};
+
 // One layer for a wave's 16 futures: in (K halves a row) -> out (N outputs), tanh or (last) the rates into the states.
-__device__ __forceinline__ void layer(const Desc& d, const _Float16 (*in)[kStride], _Float16 (*out)[kStride], int K, int N, const _Float16* W,
-    const float* b, bool last, float (*st)[kMaxState], float dt, int lane)
+__device__ __forceinline__ void layer(const Desc& d, const _Float16* in, _Float16* out, int ts, int K, int N, const _Float16* W,
+    const float* b, bool last, float* st, int ss, float dt, int lane)
 {
     const int r = lane & 15, g = lane >> 4;
     for (int nb = 0; nb < N; nb += 16) {
         float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int kc = 0; kc < K; kc += 16) {
-            const half4 a = *reinterpret_cast<const half4*>(&in[r][kc + 4 * g]);
+            const half4 a = *reinterpret_cast<const half4*>(in + r * ts + kc + 4 * g);
             const half4 w = *reinterpret_cast<const half4*>(W + (long long)(nb + r) * K + kc + 4 * g);
             acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, w, acc, 0, 0, 0);
         }
@@ -92,61 +114,72 @@ __device__ __forceinline__ void layer(const Desc& d, const _Float16 (*in)[kStrid
             const int row = 4 * g + j;
             const float v = acc[j] + bias;
             if (!last) {
-                out[row][col] = (_Float16)tanhf(v);
+                out[row * ts + col] = (_Float16)fast_tanh(v);
             } else if (col < d.F) {
-                st[row][col] += dt * v * d.scale[col];
+                st[row * ss + col] += dt * v * d.scale[col];
             }
         }
     }
 }
 
-// A block (agent, part), a wave a tile of 16 futures at a time: part p's waves take tiles p * 16 * kWaves + 16 w, then
-// every 16 * kWaves * parts futures on (samples a multiple of 16 * kWaves: every wave runs the same number of tiles, so
-// the block's barriers are reached by all). Actions into acts[agent][k][t][axis], each future's cost into cost[agent][k].
-__global__ __launch_bounds__(64 * kWaves) void k_mppi_rollout(Desc d, const float* __restrict__ obs, const float* __restrict__ mean,
+// A block (agent, part), a wave a tile of 16 futures at a time: part p takes tiles 16 p, then every 16 * parts futures
+// on. Actions into acts[agent][k][t][axis], each future's cost into cost[agent][k]. LDS: Lds::of(d).bytes(). The noise
+// of two steps is drawn at once (lane l: future l / 2 % 16, axis l % 2, step t + l / 32), so every lane draws.
+__global__ __launch_bounds__(64) void k_mppi_rollout(Desc d, const float* __restrict__ obs, const float* __restrict__ mean,
     const unsigned* __restrict__ ids, float* __restrict__ acts, float* __restrict__ cost, u64 tick, float sigma, float dt)
 {
-    __shared__ __attribute__((aligned(16))) _Float16 tile[kWaves][2][16][kStride];
-    __shared__ float st[kWaves][16][kMaxState];
-    __shared__ float act[kWaves][16][2];
-    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const Lds L = Lds::of(d);
+    const int ts = L.ts, ss = L.ss;
+    _Float16* tile0 = reinterpret_cast<_Float16*>(lds);
+    _Float16* tile1 = tile0 + 16 * ts;
+    float* st = reinterpret_cast<float*>(tile1 + 16 * ts);
+    float* act = st + 16 * ss; // [2 steps][16 futures][2 axes]
+    const int lane = (int)threadIdx.x, r16 = lane & 15, g4 = lane >> 4;
     const int agent = (int)blockIdx.x;
     const int F = d.F, H = d.horizon, K = d.samples;
-    const u64 id = ids[agent];
-    for (int t0 = (int)blockIdx.y * 16 * kWaves + w * 16; t0 < K; t0 += 16 * kWaves * (int)gridDim.y) {
+    const u64 base = noise_base(ids[agent], tick);
+    for (int t0 = (int)blockIdx.y * 16; t0 < K; t0 += 16 * (int)gridDim.y) {
         for (int q = lane; q < 16 * F; q += 64) {
-            st[w][q / F][q % F] = obs[(long long)agent * F + q % F];
+            st[(q / F) * ss + q % F] = obs[(long long)agent * F + q % F];
         }
         float J = 0.0f; // lanes 0..15: future t0 + lane
         for (int t = 0; t < H; ++t) {
-            if (lane < 32) {
-                const int row = lane >> 1, axis = lane & 1, k = t0 + row;
-                const float e = k == 0 ? 0.0f : gauss(noise_key(id, tick, (u64)k, (u64)t, (u64)axis));
-                const float a = fminf(fmaxf(mean[((long long)agent * H + t) * 2 + axis] + sigma * e, -1.0f), 1.0f);
-                act[w][row][axis] = a;
-                acts[(((long long)agent * K + k) * H + t) * 2 + axis] = a;
-            }
-            __syncthreads();
-            // the inputs (state, then action), normalised, in fp16; zero past them
-            for (int q = lane; q < 16 * d.in_pad; q += 64) {
-                const int row = q / d.in_pad, f = q % d.in_pad;
-                const float x = f < F ? st[w][row][f] : (f < F + 2 ? act[w][row][f - F] : 0.0f);
-                tile[w][0][row][f] = (_Float16)(f < F + 2 ? (x - d.shift[f]) * d.inscale[f] : 0.0f);
-            }
-            __syncthreads();
-            layer(d, tile[w][0], tile[w][1], d.in_pad, d.hid, d.W1, d.b1, false, st[w], dt, lane);
-            __syncthreads();
-            layer(d, tile[w][1], tile[w][0], d.hid, d.hid, d.W2, d.b2, false, st[w], dt, lane);
-            __syncthreads();
-            layer(d, tile[w][0], tile[w][1], d.hid, d.f_pad, d.W3, d.b3, true, st[w], dt, lane);
-            __syncthreads();
-            if (lane < 16) {
-                float c = 0.0f;
-                for (int f = 0; f < F; ++f) {
-                    const float e = st[w][lane][f] - d.target[f];
-                    c += d.weight[f] * e * e;
+            if ((t & 1) == 0) {
+                const int row = (lane >> 1) & 15, axis = lane & 1, step = t + (lane >> 5), k = t0 + row;
+                if (step < H) {
+                    const float e = k == 0 ? 0.0f : gauss(base ^ ((u64)k << 24) ^ ((u64)step << 8) ^ (u64)axis);
+                    const float a = fminf(fmaxf(mean[((long long)agent * H + step) * 2 + axis] + sigma * e, -1.0f), 1.0f);
+                    act[lane] = a;
+                    acts[(((long long)agent * K + k) * H + step) * 2 + axis] = a;
                 }
-                const float ax = act[w][lane][0], ay = act[w][lane][1];
+            }
+            __syncthreads();
+            const float* at = act + 32 * (t & 1);
+            // the inputs (state, then action), normalised, in fp16; zero past them: lane l row l % 16, features l / 16,
+            // l / 16 + 4, ...
+            for (int f = g4; f < d.in_pad; f += 4) {
+                const float x = f < F ? st[r16 * ss + f] : (f < F + 2 ? at[r16 * 2 + f - F] : 0.0f);
+                tile0[r16 * ts + f] = (_Float16)(f < F + 2 ? (x - d.shift[f]) * d.inscale[f] : 0.0f);
+            }
+            __syncthreads();
+            layer(d, tile0, tile1, ts, d.in_pad, d.hid, d.W1, d.b1, false, st, ss, dt, lane);
+            __syncthreads();
+            layer(d, tile1, tile0, ts, d.hid, d.hid, d.W2, d.b2, false, st, ss, dt, lane);
+            __syncthreads();
+            layer(d, tile0, tile1, ts, d.hid, d.f_pad, d.W3, d.b3, true, st, ss, dt, lane);
+            __syncthreads();
+            // the step's cost: lane l the features l / 16, l / 16 + 4, ... of future l % 16, then the four lane groups
+            // added in a fixed order
+            float c = 0.0f;
+            for (int f = g4; f < F; f += 4) {
+                const float e = st[r16 * ss + f] - d.target[f];
+                c += d.weight[f] * e * e;
+            }
+            c += __shfl_xor(c, 16);
+            c += __shfl_xor(c, 32);
+            if (lane < 16) {
+                const float ax = at[lane * 2], ay = at[lane * 2 + 1];
                 J += c + d.action_weight * (ax * ax + ay * ay);
             }
             __syncthreads();
@@ -242,7 +275,7 @@ int pad16(int v) { return (v + 15) / 16 * 16; }
 } // namespace
 
 struct fe_mppi {
-    int F = 0, hid = 0, horizon = 0, samples = 0, in_pad = 0, f_pad = 0;
+    int F = 0, hid = 0, horizon = 0, samples = 0, in_pad = 0, f_pad = 0, cus = 0;
     bool dynamics = false, costed = false;
     Dev<_Float16> W1, W2, W3;
     Dev<float> b1, b2, b3, shift, inscale, scale, target, weight;
@@ -253,6 +286,19 @@ struct fe_mppi {
 };
 
 namespace {
+
+// Blocks an agent's futures spread over: one tile of 16 a block while every tile's block fits the card at once; past
+// that each block walks a few tiles in turn, so the launch is one round (a second, part-filled round of one-tile
+// blocks took longer than the first: 824 against 777 us for 256 agents x 512 futures). A CU holds as many blocks as
+// its 160 KB of LDS allows, at most 28 (seven waves a SIMD at the rollout's registers).
+int rollout_parts(const fe_mppi& p, const Desc& d, int agents)
+{
+    const int tiles = p.samples / 16;
+    const int per_cu = std::min(160 * 1024 / Lds::of(d).bytes(), 28);
+    const long long fit = static_cast<long long>(std::max(p.cus, 1)) * per_cu;
+    const long long rounds = (static_cast<long long>(agents) * tiles + fit - 1) / fit;
+    return static_cast<int>((tiles + rounds - 1) / rounds);
+}
 
 int fail(fe_mppi* p, const std::string& what)
 {
@@ -293,7 +339,7 @@ fe_mppi* fe_mppi_create(int features, int hidden, int horizon, int samples, char
         }
     };
     if (features < 1 || features + 2 > kMaxState || hidden < 16 || hidden > kMaxWidth || hidden % 16 != 0 || horizon < 1 || horizon > 64 ||
-        samples < 16 * kWaves || samples > 1024 || samples % (16 * kWaves) != 0) {
+        samples < 16 || samples > 1024 || samples % 16 != 0) {
         say("features 1..62, hidden 16..128 (a multiple of 16), horizon 1..64, samples 16..1024 (a multiple of 16)");
         return nullptr;
     }
@@ -303,6 +349,7 @@ fe_mppi* fe_mppi_create(int features, int hidden, int horizon, int samples, char
         return nullptr;
     }
     auto p = std::make_unique<fe_mppi>();
+    p->cus = info.compute_units;
     p->F = features;
     p->hid = hidden;
     p->horizon = horizon;
@@ -381,7 +428,9 @@ int fe_mppi_plan(fe_mppi* p, int agents, const float* obs, const uint32_t* ids, 
         p->cost.alloc(static_cast<std::size_t>(agents) * K);
         Desc d{F, p->in_pad, p->hid, p->f_pad, H, K, p->W1.p, p->W2.p, p->W3.p, p->b1.p, p->b2.p, p->b3.p, p->shift.p, p->inscale.p,
             p->scale.p, p->target.p, p->weight.p, p->action_weight};
-        hipLaunchKernelGGL(k_mppi_rollout, dim3(static_cast<unsigned>(agents), static_cast<unsigned>(K / (16 * kWaves))), dim3(64 * kWaves), 0, s,
+        // a block is one wave (its barriers wait for no other wave), an agent's futures spread over samples / 16 blocks
+        hipLaunchKernelGGL(k_mppi_rollout, dim3(static_cast<unsigned>(agents), static_cast<unsigned>(rollout_parts(*p, d, agents))), dim3(64),
+            static_cast<unsigned>(Lds::of(d).bytes()), s,
             d, p->obs.p, p->mean.p, p->ids.p,
             p->acts.p, p->cost.p, static_cast<unsigned long long>(tick), sigma, dt);
         hipLaunchKernelGGL(k_mppi_update, dim3(static_cast<unsigned>(agents)), dim3(static_cast<unsigned>(pad16(2 * H))), 0, s, p->cost.p,
