@@ -47,7 +47,13 @@ struct HOpt {
     int nx, ny;
     int row;       // its first entry in the row table
     float h, dt, thdt, r;
+    int part8, part16; // its partitioned column solve's coefficients in the part table, 8 or 16 rows a thread (-1: none)
 };
+
+// The part table (part_table): a row's m, a, C (downward sweep), Q, R (after the upward sweep), then a chunk's C_(M-1)
+// (the last row's downward C), al, m and c' (its equation's factorisation).
+constexpr int kPartRow = 5;
+constexpr int kPartChunk = 4;
 
 struct HRow {
     float lx, ux, ly, uy; // A1, A2 coefficients of the row
@@ -425,34 +431,34 @@ __global__ __launch_bounds__(64) void k_cols_lds(const HOpt* __restrict__ opts, 
 // downward and an upward sweep express every unknown in the chunk's last unknown and the one before the chunk); the P
 // equations of a column in those unknowns are solved down the column by one thread (Thomas through LDS; measured
 // faster than parallel cyclic reduction over the waves, whose extra barriers and divisions cost more than the idle
-// waves), and each thread substitutes back. HBM traffic is the pass's input and output once: 8 bytes a cell (16 with the update of U),
-// against 16 (24) for k_cols. The arithmetic differs from the Thomas sweep's, so its bits do too (both fp32).
+// waves), and each thread substitutes back. (I - th dt A2) is the same matrix for every column of an option, so every
+// coefficient of that elimination (the sweeps' multipliers, the chunk equations and their factorisation) is the
+// option's own, computed once on the host in double (part_table) and read here as scalars; a column carries only its
+// right-hand side through them: three multiply-adds a cell and no division. HBM traffic is the pass's input and output
+// once: 8 bytes a cell (16 with the update of U), against 16 (24) for k_cols. The arithmetic differs from the Thomas
+// sweep's, so its bits do too (both fp32).
 template <int M, bool FINAL>
-__global__ __launch_bounds__(1024) void k_cols_part(const HOpt* __restrict__ opts, const HRow* __restrict__ rows,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_cols_part(const HOpt* __restrict__ opts, const float* __restrict__ ptab,
     const Task* __restrict__ tasks, float* __restrict__ U, float* __restrict__ D2, const float* __restrict__ T1)
 {
     static_assert(M >= 2, "a chunk holds two rows at least");
-    extern __shared__ float red[]; // [8][P][64]: first-row terms P0 Q0 R0, the chunk equations al be ga de, u
+    extern __shared__ float red[]; // [3][P][64]: a chunk's first unknown P0, the chunk equations' right sides, u
     const Task t = tasks[blockIdx.x];
     const HOpt o = opts[t.opt];
-    // blockDim.x is the wave: p is the same across a wave, so the row coefficients are scalar loads
+    // blockDim.x is the wave: p is the same across a wave, so the table's entries are scalar loads
     const int c = (int)threadIdx.x, p = __builtin_amdgcn_readfirstlane((int)threadIdx.y), P = (int)blockDim.y;
     const int i = t.first + c;
     const bool on = i < o.nx;
     const int ic = on ? i : o.nx - 1;
     const int ny = o.ny;
     const long long nx = o.nx;
-    const HRow* cf = rows + o.row;
+    const float* rt = ptab + (M == 8 ? o.part8 : o.part16); // [P M rows][kPartRow], then [P chunks][kPartChunk]
+    const float* ct = rt + P * M * kPartRow;
     const int j0 = p * M;
     const int stride = P * 64;
     float* sP = red;
-    float* sQ = red + stride;
-    float* sR = red + 2 * stride;
-    float* sAl = red + 3 * stride;
-    float* sBe = red + 4 * stride;
-    float* sGa = red + 5 * stride;
-    float* sDe = red + 6 * stride;
-    float* sU = red + 7 * stride;
+    float* sDe = red + stride;
+    float* sU = red + 2 * stride;
     const int me = p * 64 + c;
     // every load unconditional (clamped rows and column), padding selected afterwards
     float x[M];
@@ -461,65 +467,37 @@ __global__ __launch_bounds__(1024) void k_cols_part(const HOpt* __restrict__ opt
         const int j = min(j0 + q, ny - 1);
         x[q] = T1[o.off + j * nx + ic];
     }
-    float C[M], A[M];
+    // downward: x_q = (d_q - a_q x_(q-1)) m_q
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-        const int j = j0 + q;
-        const bool real = j < ny;
-        const HRow r = cf[min(j, ny - 1)];
-        const float a = real && j > 0 ? r.ya : 0.0f;
-        const float b = real ? r.yb : 1.0f;
-        const float cc = real ? r.yc : 0.0f;
-        const float d = real ? x[q] : 0.0f;
-        if (q == 0) {
-            const float m = 1.0f / b;
-            C[0] = cc * m;
-            A[0] = a * m;
-            x[0] = d * m;
-        } else {
-            const float m = 1.0f / (b - a * C[q - 1]);
-            C[q] = cc * m;
-            A[q] = -a * A[q - 1] * m;
-            x[q] = (d - a * x[q - 1]) * m;
-        }
+        const float* e = rt + (j0 + q) * kPartRow;
+        const float d = j0 + q < ny ? x[q] : 0.0f;
+        x[q] = q == 0 ? d * e[0] : (d - e[1] * x[q - 1]) * e[0];
     }
-    // upward: x_q = x[q] - C[q] x_e - A[q] x_(s-1) for q < M - 1 (x[] then holds P, C holds Q, A holds R)
-    const float ce = C[M - 1], ae = A[M - 1], de = x[M - 1];
+    // upward: x_q -= C_q x_(q+1), C the downward sweep's
 #pragma unroll
     for (int q = M - 3; q >= 0; --q) {
-        x[q] = x[q] - C[q] * x[q + 1];
-        A[q] = A[q] - C[q] * A[q + 1];
-        C[q] = -C[q] * C[q + 1];
+        x[q] = x[q] - rt[(j0 + q) * kPartRow + 2] * x[q + 1];
     }
     sP[me] = x[0];
-    sQ[me] = C[0];
-    sR[me] = A[0];
     __syncthreads();
-    {
-        const bool tail = p == P - 1;
-        const float p1 = tail ? 0.0f : sP[me + 64], q1 = tail ? 0.0f : sQ[me + 64], r1 = tail ? 0.0f : sR[me + 64];
-        sAl[me] = ae;
-        sBe[me] = 1.0f - ce * r1;
-        sGa[me] = -ce * q1;
-        sDe[me] = de - ce * p1;
-    }
+    // the chunk's equation in (its last unknown, the next chunk's): right side x_(M-1) - C_(M-1) P0 of the next chunk
+    sDe[me] = x[M - 1] - ct[p * kPartChunk] * (p == P - 1 ? 0.0f : sP[me + 64]);
     __syncthreads();
     if (p == 0) {
-        // Thomas down the column's P chunk equations (al u_(k-1) + be u_k + ga u_(k+1) = de), c' and d' kept in place
-        float cp = 0.0f, dp = 0.0f;
+        // Thomas down the column's P chunk equations, the factorisation the table's: d'_k = (de_k - al_k d'_(k-1)) m_k,
+        // then u_k = d'_k - c'_k u_(k+1)
+        float dp = 0.0f;
         for (int k = 0; k < P; ++k) {
+            const float* e = ct + k * kPartChunk;
             const int s = k * 64 + c;
-            const float al = k > 0 ? sAl[s] : 0.0f;
-            const float m = 1.0f / (sBe[s] - al * cp);
-            cp = sGa[s] * m;
-            dp = (sDe[s] - al * dp) * m;
-            sGa[s] = cp;
+            dp = (sDe[s] - e[1] * dp) * e[2];
             sDe[s] = dp;
         }
         float u = 0.0f;
         for (int k = P - 1; k >= 0; --k) {
             const int s = k * 64 + c;
-            u = sDe[s] - sGa[s] * u;
+            u = sDe[s] - ct[k * kPartChunk + 3] * u;
             sU[s] = u;
         }
     }
@@ -529,10 +507,10 @@ __global__ __launch_bounds__(1024) void k_cols_part(const HOpt* __restrict__ opt
     x[M - 1] = u;
 #pragma unroll
     for (int q = 0; q < M - 1; ++q) {
-        x[q] = x[q] - C[q] * u - A[q] * ul;
+        const float* e = rt + (j0 + q) * kPartRow;
+        x[q] = x[q] - e[3] * u - e[4] * ul;
     }
     if (FINAL) {
-        // (no loads moved above the substitution: with them in flight beside x, C and A a 1024-thread block spills)
         __builtin_amdgcn_sched_barrier(0);
         float a[M], b[M];
 #pragma unroll
@@ -553,6 +531,60 @@ __global__ __launch_bounds__(1024) void k_cols_part(const HOpt* __restrict__ opt
             if (on && j0 + q < ny) {
                 D2[o.off + (j0 + q) * nx + i] = x[q];
             }
+        }
+    }
+}
+
+// The coefficients k_cols_part needs for one option, P chunks of M rows (rows past ny the x = 0 equations), in double
+// from the row table's (I - th dt A2): per row the downward sweep's multiplier m = 1 / (b - a C_(q-1)) (1 / b on a
+// chunk's first row), its a and C = c m, and after the upward sweep the coefficients Q, R of the chunk's last unknown and
+// of the one before the chunk; per chunk C_(M-1), and the chunk equations al u_(k-1) + be u_k + ga u_(k+1) = de
+// (al = A_(M-1), be = 1 - C_(M-1) R_0', ga = -C_(M-1) Q_0', ' the next chunk's) factorised: al, m = 1 / (be - al c'),
+// c' = ga m.
+void part_table(const HRow* cf, int ny, int P, int M, std::vector<float>& out)
+{
+    const int R = P * M;
+    std::vector<double> m(R), a(R), C(R), A(R), Q(R), Rr(R);
+    for (int j = 0; j < R; ++j) {
+        const int q = j % M;
+        const bool real = j < ny;
+        const HRow& r = cf[std::min(j, ny - 1)];
+        const double aj = real && j > 0 ? r.ya : 0.0;
+        const double bj = real ? r.yb : 1.0;
+        const double cj = real ? r.yc : 0.0;
+        m[j] = q == 0 ? 1.0 / bj : 1.0 / (bj - aj * C[j - 1]);
+        C[j] = cj * m[j];
+        A[j] = q == 0 ? aj * m[j] : -aj * A[j - 1] * m[j];
+        a[j] = aj;
+    }
+    for (int p = 0; p < P; ++p) {
+        const int j0 = p * M;
+        for (int q = 0; q < M; ++q) {
+            Q[j0 + q] = C[j0 + q];
+            Rr[j0 + q] = A[j0 + q];
+        }
+        for (int q = M - 3; q >= 0; --q) {
+            const int j = j0 + q;
+            Rr[j] = Rr[j] - C[j] * Rr[j + 1];
+            Q[j] = -C[j] * Q[j + 1];
+        }
+    }
+    for (int j = 0; j < R; ++j) {
+        for (const double v : {m[j], a[j], C[j], Q[j], Rr[j]}) {
+            out.push_back(static_cast<float>(v));
+        }
+    }
+    double cp = 0.0;
+    for (int p = 0; p < P; ++p) {
+        const int last = p * M + M - 1;
+        const bool tail = p == P - 1;
+        const double ce = C[last], al = p > 0 ? A[last] : 0.0;
+        const double r1 = tail ? 0.0 : Rr[(p + 1) * M], q1 = tail ? 0.0 : Q[(p + 1) * M];
+        const double be = 1.0 - ce * r1, ga = -ce * q1;
+        const double mm = 1.0 / (be - al * cp);
+        cp = ga * mm;
+        for (const double v : {ce, al, mm, cp}) {
+            out.push_back(static_cast<float>(v));
         }
     }
 }
@@ -689,6 +721,7 @@ struct fe_heston {
     std::vector<Class> classes;
     Dev<HOpt> opts;
     Dev<HRow> rows;
+    Dev<float> ptab; // the partitioned column solves' coefficients (part_table)
     Dev<Stencil> stencils;
     Dev<double> strikes;
     Dev<double> prices;
@@ -733,18 +766,18 @@ struct fe_heston {
     {
         const int pw = part_waves(c, c.col_part);
         const dim3 block(64, static_cast<unsigned>(pw));
-        const unsigned shm = 8u * static_cast<unsigned>(pw) * 64u * 4u;
+        const unsigned shm = 3u * static_cast<unsigned>(pw) * 64u * 4u;
         const dim3 grid(static_cast<unsigned>(c.n_col_tasks));
         if (c.col_part == 8) {
             if (final_pass) {
-                hipLaunchKernelGGL((k_cols_part<8, true>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+                hipLaunchKernelGGL((k_cols_part<8, true>), grid, block, shm, s, opts.p, ptab.p, c.col_tasks.p, U.p, D2.p, T1.p);
             } else {
-                hipLaunchKernelGGL((k_cols_part<8, false>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+                hipLaunchKernelGGL((k_cols_part<8, false>), grid, block, shm, s, opts.p, ptab.p, c.col_tasks.p, U.p, D2.p, T1.p);
             }
         } else if (final_pass) {
-            hipLaunchKernelGGL((k_cols_part<16, true>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+            hipLaunchKernelGGL((k_cols_part<16, true>), grid, block, shm, s, opts.p, ptab.p, c.col_tasks.p, U.p, D2.p, T1.p);
         } else {
-            hipLaunchKernelGGL((k_cols_part<16, false>), grid, block, shm, s, opts.p, rows.p, c.col_tasks.p, U.p, D2.p, T1.p);
+            hipLaunchKernelGGL((k_cols_part<16, false>), grid, block, shm, s, opts.p, ptab.p, c.col_tasks.p, U.p, D2.p, T1.p);
         }
     }
 
@@ -1031,8 +1064,24 @@ fe_heston* fe_heston_create(int n, const int* nx, const int* ny, const double* o
             cl.ny_max = std::max(cl.ny_max, Y);
         }
         b->cells = off;
+        std::vector<float> ptab;
+        for (fe_heston::Class& c : b->classes) {
+            for (const int o : c.members) {
+                HOpt& ho = b->host_opts[static_cast<std::size_t>(o)];
+                const HRow* cf = rows.data() + ho.row;
+                for (const int m : {8, 16}) {
+                    int& at = m == 8 ? ho.part8 : ho.part16;
+                    at = -1;
+                    if (fe_heston::part_fits(c, m)) {
+                        at = static_cast<int>(ptab.size());
+                        part_table(cf, ho.ny, fe_heston::part_waves(c, m), m, ptab);
+                    }
+                }
+            }
+        }
         b->opts.upload(b->host_opts);
         b->rows.upload(rows);
+        b->ptab.upload(ptab.empty() ? std::vector<float>(1, 0.0f) : ptab);
         b->stencils.upload(st);
         b->strikes.upload(strikes);
         b->prices.alloc(static_cast<std::size_t>(n));
