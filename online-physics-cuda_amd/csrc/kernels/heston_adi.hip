@@ -159,7 +159,10 @@ __device__ __forceinline__ void store_row(float* p, const float (&v)[K])
 }
 
 // Solves (b on the diagonal, a below, c above; the first row's a and the last row's c folded onto the diagonal) for
-// the row whose right-hand side the L lanes of a group hold, K cells a lane; the answer replaces d.
+// the row whose right-hand side the L lanes of a group hold, K cells a lane; the answer replaces d. Quotients are taken
+// through the hardware reciprocal (v_rcp_f32, 1 ulp): an IEEE division is a sequence of about ten instructions, and the
+// row passes are bound by their instruction issue; the systems are diagonally dominant, so the extra half ulp a quotient
+// does not grow.
 template <int K, int L>
 __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float c, int p)
 {
@@ -169,7 +172,7 @@ __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float
     {
         const bool first = p == 0;
         const float bi = b + (first ? a : 0.0f);
-        const float m = 1.0f / bi;
+        const float m = __builtin_amdgcn_rcpf(bi);
         C[0] = c * m;
         A[0] = first ? 0.0f : a * m;
         D[0] = d[0] * m;
@@ -178,7 +181,7 @@ __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float
     for (int q = 1; q < K; ++q) {
         const bool last = p == L - 1 && q == K - 1;
         const float bi = b + (last ? c : 0.0f);
-        const float m = 1.0f / (bi - a * C[q - 1]);
+        const float m = __builtin_amdgcn_rcpf(bi - a * C[q - 1]);
         C[q] = last ? 0.0f : c * m;
         A[q] = -a * A[q - 1] * m;
         D[q] = (d[q] - a * D[q - 1]) * m;
@@ -209,8 +212,8 @@ __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float
         const float alp = __shfl_down(al, s, L), bep = __shfl_down(be, s, L), gap = __shfl_down(ga, s, L),
                     dep = __shfl_down(de, s, L);
         const bool has_m = p >= s, has_p = p + s < L;
-        const float k1 = has_m ? al / bem : 0.0f;
-        const float k2 = has_p ? ga / bep : 0.0f;
+        const float k1 = has_m ? al * __builtin_amdgcn_rcpf(bem) : 0.0f;
+        const float k2 = has_p ? ga * __builtin_amdgcn_rcpf(bep) : 0.0f;
         const float nal = has_m ? -k1 * alm : 0.0f;
         const float nga = has_p ? -k2 * gap : 0.0f;
         const float nbe = be - (has_m ? k1 * gam : 0.0f) - (has_p ? k2 * alp : 0.0f);
@@ -220,7 +223,7 @@ __device__ __forceinline__ void solve_row(float (&d)[K], float a, float b, float
         ga = nga;
         de = nde;
     }
-    const float u = de / be;
+    const float u = de * __builtin_amdgcn_rcpf(be);
     const float ul = from_left<L>(u, 0.0f, p); // lane 0 has no unknown before it (its R terms are 0)
     d[K - 1] = u;
 #pragma unroll
