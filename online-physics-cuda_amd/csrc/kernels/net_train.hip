@@ -1,15 +1,15 @@
 // A law net's trainer on gfx950 (include/fe_nettrain.h): the loss and gradient of a regression over rows, and a
 // clipped Adam step.
 //
-// A one-wave block takes a tile of 16 rows. Its inputs are normalised into LDS; each layer's product runs on the matrix
-// cores in fp32 (v_mfma_f32_16x16x4f32: A 16 x 4, lane l holds A[l % 16][l / 16]; B 4 x 16, lane l holds
-// B[l / 16][l % 16]; D 16 x 16, lane l holds D[4 (l / 16) + j][l % 16], j = 0..3), the pre-activations and activations
-// kept in LDS for the way back. The output layer gives each row's error e = out_scale * y - t and delta m e out_scale;
-// going back, each layer's weight gradient (deltas^T x inputs, the 16 rows as the product's depth) and bias gradient
-// (the rows summed in order) go to the tile's own slot of a partial array, and the deltas move down through W^T and
-// the activation's derivative. A block walks its tiles in a fixed order, adding each tile's gradient into its own slot of
-// a partial array; the blocks' partials are then summed in two fixed-order levels (chunks of 32 blocks in order, then
-// the chunks), so the same rows give the same bits. Adam follows on the summed mean.
+// A block of four waves takes tiles of 16 rows. A tile's inputs are normalised into LDS; each layer's product runs on
+// the matrix cores in fp32 (v_mfma_f32_16x16x4f32: A 16 x 4, lane l holds A[l % 16][l / 16]; B 4 x 16, lane l holds
+// B[l / 16][l % 16]; D 16 x 16, lane l holds D[4 (l / 16) + j][l % 16], j = 0..3), its 16-column blocks spread over the
+// waves, the pre-activations and activations kept in LDS for the way back. The output layer gives each row's error
+// e = out_scale * y - t and delta m e out_scale; going back, each layer's weight gradient (deltas^T x inputs, the 16 rows
+// as the product's depth) and bias gradient (the rows summed in order) are added into the block's own slot of a partial
+// array, and the deltas move down through W^T and the activation's derivative. A block walks its tiles in a fixed order
+// and the blocks' partials are summed in a fixed order (in one level up to 64 blocks, past that in chunks of 32 first),
+// so the same rows give the same bits; the last level's pass also takes the Adam step.
 #include "fe_nettrain.h"
 #include "fe_hip.h"
 
@@ -30,7 +30,8 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 constexpr int kMaxLayers = 8;   // weight matrices
 constexpr int kMaxWidth = 128;  // any layer's width
 constexpr int kRows = 16;       // rows a tile (a wave)
-constexpr int kChunk = 32;      // blocks' partials summed together in the first level
+constexpr int kChunk = 32;      // blocks' partials summed together in the first level (past kOneLevel blocks)
+constexpr int kOneLevel = 64;   // up to this many blocks their partials are added in one level
 constexpr int kBlocks = 1024;   // the most blocks a pass (four waves a CU)
 constexpr int kMaxLds = 65536;  // bytes of LDS a block may use
 constexpr int kWaves = 4;       // waves a block: a tile's 16-column blocks spread over them
@@ -262,38 +263,49 @@ __global__ void k_nt_chunks(const float* __restrict__ parts, int blocks, int wid
     }
     const int t1 = (j + 1) * kChunk < blocks ? (j + 1) * kChunk : blocks;
     float s = 0.0f;
+#pragma unroll 8
     for (int t = j * kChunk; t < t1; ++t) {
         s += parts[(long long)t * width + c];
     }
     out[(long long)j * width + c] = s;
 }
 
-// Second level: the chunks in order, divided by the rows (the mean); the last column is the loss.
-__global__ void k_nt_fold(const float* __restrict__ chunks, int count, int width, float rows_f, float* __restrict__ grad)
+// Adam on the trained part of the image (the reference's form, src/field/cuda/CudaNet.cpp:356-366): each gradient
+// clipped to +-clip, then the bias-corrected step.
+struct Adam {
+    float* w; // the image's trained part
+    float* m1;
+    float* m2;
+    int n; // 0: no step
+    float lr, b1, b2, c1, c2, clip;
+};
+
+__device__ __forceinline__ void adam_at(const Adam& a, int p, float grad)
+{
+    const float g = fminf(fmaxf(grad, -a.clip), a.clip);
+    const float m = a.b1 * a.m1[p] + (1.0f - a.b1) * g, v = a.b2 * a.m2[p] + (1.0f - a.b2) * g * g;
+    a.m1[p] = m;
+    a.m2[p] = v;
+    a.w[p] -= a.lr * (m / a.c1) / (sqrtf(v / a.c2) + 1e-8f);
+}
+
+// Last level: `count` rows of partial sums (the blocks' own, or the chunks') added in order, divided by the rows (the
+// mean; the last column is the loss), and, when adam.n > 0, the Adam step on each weight and bias with its mean.
+__global__ void k_nt_fold(const float* __restrict__ sums, int count, int width, float rows_f, float* __restrict__ grad, Adam adam)
 {
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c < width) {
         float s = 0.0f;
+#pragma unroll 8
         for (int j = 0; j < count; ++j) {
-            s += chunks[(long long)j * width + c];
+            s += sums[(long long)j * width + c];
         }
-        grad[c] = s / rows_f;
+        const float g = s / rows_f;
+        grad[c] = g;
+        if (c < adam.n) {
+            adam_at(adam, c, g);
+        }
     }
-}
-
-// Adam on the trained part of the image (the reference's form, src/field/cuda/CudaNet.cpp:356-366).
-__global__ void k_nt_adam(float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m1, float* __restrict__ m2, int n,
-    float lr, float b1, float b2, float c1, float c2, float clip)
-{
-    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= n) {
-        return;
-    }
-    const float g = fminf(fmaxf(grad[p], -clip), clip);
-    const float m = b1 * m1[p] + (1.0f - b1) * g, v = b2 * m2[p] + (1.0f - b2) * g * g;
-    m1[p] = m;
-    m2[p] = v;
-    w[p] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
 }
 
 void hcheck(hipError_t e, const char* what)
@@ -348,20 +360,26 @@ struct fe_nettrain {
     const float* loss() const { return grad.p + s.trained; }
 
     // The gradient and loss for the rows already uploaded; leaves the mean gradient in grad[0, trained), the mean loss
-    // in grad[trained].
-    void gradient(int rows, bool masked)
+    // in grad[trained], and takes the Adam step when adam.n > 0. Up to kOneLevel blocks their partials are added in one
+    // level, past it in chunks of kChunk first.
+    void gradient(int rows, bool masked, const Adam& adam)
     {
         const hipStream_t st = stream();
         const int blocks = std::min((rows + kRows - 1) / kRows, kBlocks), width = s.trained + 1;
         partial.alloc(static_cast<std::size_t>(blocks) * width);
-        const int nchunks = (blocks + kChunk - 1) / kChunk;
-        chunks.alloc(static_cast<std::size_t>(nchunks) * width);
         grad.alloc(static_cast<std::size_t>(width));
         hipLaunchKernelGGL(k_nt_tile, dim3(static_cast<unsigned>(blocks)), dim3(64 * kWaves), static_cast<unsigned>(s.lds) * 4u, st, s, w.p, x.p,
             t.p, masked ? m.p : nullptr, rows, partial.p);
-        hipLaunchKernelGGL(k_nt_chunks, dim3(blocks_for(width, 256), static_cast<unsigned>(nchunks)), dim3(256), 0, st, partial.p, blocks, width,
-            chunks.p);
-        hipLaunchKernelGGL(k_nt_fold, dim3(blocks_for(width, 256)), dim3(256), 0, st, chunks.p, nchunks, width, static_cast<float>(rows), grad.p);
+        const float* sums = partial.p;
+        int count = blocks;
+        if (blocks > kOneLevel) {
+            count = (blocks + kChunk - 1) / kChunk;
+            chunks.alloc(static_cast<std::size_t>(count) * width);
+            hipLaunchKernelGGL(k_nt_chunks, dim3(blocks_for(width, 256), static_cast<unsigned>(count)), dim3(256), 0, st, partial.p, blocks, width,
+                chunks.p);
+            sums = chunks.p;
+        }
+        hipLaunchKernelGGL(k_nt_fold, dim3(blocks_for(width, 256)), dim3(256), 0, st, sums, count, width, static_cast<float>(rows), grad.p, adam);
         hcheck(hipGetLastError(), "gradient launch");
     }
 
@@ -506,7 +524,7 @@ int fe_nettrain_gradient(fe_nettrain* t, const float* x, const float* target, co
     }
     try {
         t->upload(x, target, m, rows);
-        t->gradient(rows, m != nullptr);
+        t->gradient(rows, m != nullptr, Adam{});
         const hipStream_t st = t->stream();
         if (grad != nullptr) {
             hcheck(hipMemcpyAsync(grad, t->grad.p, sizeof(float) * static_cast<std::size_t>(t->s.trained), hipMemcpyDeviceToHost, st), "download");
@@ -530,16 +548,14 @@ int fe_nettrain_step(fe_nettrain* t, const float* x, const float* target, const 
     }
     try {
         t->upload(x, target, m, rows);
-        t->gradient(rows, m != nullptr);
         const float b1 = 0.9f, b2 = 0.999f;
-        ++t->adam_t;
-        const float c1 = 1.0f - std::pow(b1, static_cast<float>(t->adam_t));
-        const float c2 = 1.0f - std::pow(b2, static_cast<float>(t->adam_t));
-        const hipStream_t st = t->stream();
+        const int step = t->adam_t + 1;
+        const float c1 = 1.0f - std::pow(b1, static_cast<float>(step));
+        const float c2 = 1.0f - std::pow(b2, static_cast<float>(step));
         const int base = 2 * t->s.sizes[0] + 1;
-        hipLaunchKernelGGL(k_nt_adam, dim3(blocks_for(t->s.trained, 256)), dim3(256), 0, st, t->w.p + base, t->grad.p, t->m1.p, t->m2.p,
-            t->s.trained, lr, b1, b2, c1, c2, clip);
-        hcheck(hipGetLastError(), "adam launch");
+        t->gradient(rows, m != nullptr, Adam{t->w.p + base, t->m1.p, t->m2.p, t->s.trained, lr, b1, b2, c1, c2, clip});
+        t->adam_t = step;
+        const hipStream_t st = t->stream();
         float l = 0.0f;
         hcheck(hipMemcpyAsync(&l, t->loss(), sizeof(float), hipMemcpyDeviceToHost, st), "download");
         hcheck(hipStreamSynchronize(st), "sync");
