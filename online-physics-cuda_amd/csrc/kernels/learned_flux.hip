@@ -585,7 +585,8 @@ __global__ void k_sum_chunks(const float* __restrict__ parts, int rows, int widt
     }
     const int r1 = (j + 1) * chunk < rows ? (j + 1) * chunk : rows;
     float s = 0.0f;
-    for (int r = j * chunk; r < r1; ++r) {
+#pragma unroll 8
+    for (int r = j * chunk; r < r1; ++r) { // eight loads in flight, added in order
         s += parts[(long long)r * width + c];
     }
     out[(long long)j * width + c] = s;
@@ -599,7 +600,8 @@ __global__ void k_sum(const float* __restrict__ parts, int rows, int width, floa
         return;
     }
     float s = 0.0f;
-    for (int r = 0; r < rows; ++r) {
+#pragma unroll 8
+    for (int r = 0; r < rows; ++r) { // eight loads in flight, added in order
         s += parts[(long long)r * width + c];
     }
     out[c] = s;

@@ -206,7 +206,8 @@ __global__ void k_mppi_update(const float* __restrict__ cost, const float* __res
     __syncthreads();
     if (q == 0) {
         float lo = cs[0], sum = 0.0f;
-        for (int k = 0; k < K; ++k) {
+#pragma unroll 8
+        for (int k = 0; k < K; ++k) { // eight reads in flight, added in order
             lo = fminf(lo, cs[k]);
             sum += cs[k];
         }
@@ -223,7 +224,8 @@ __global__ void k_mppi_update(const float* __restrict__ cost, const float* __res
         return;
     }
     float wsum = 0.0f, acc = 0.0f;
-    for (int k = 0; k < K; ++k) {
+#pragma unroll 8
+    for (int k = 0; k < K; ++k) { // eight loads in flight, added in order
         const float wk = cs[k];
         wsum += wk;
         acc += wk * acts[((long long)agent * K + k) * 2 * H + q];
