@@ -465,7 +465,7 @@ def net_train(cpu: bool, steps: int = 50):
         rows = 1024
         x = rng.uniform(-1, 1, (rows, F + 2))
         t = np.sin(x @ A.T)
-        img, base = image.astype(np.float64), 2 * sizes[0] + 1
+        img, base = image.astype(np.float64), ref.first_weight(sizes)
         m1, m2 = np.zeros(img.size - base), np.zeros(img.size - base)
         n = 0
         s = time.perf_counter()

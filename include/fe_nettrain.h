@@ -12,9 +12,9 @@
  * gradients) on the matrix cores in fp32 (v_mfma_f32_16x16x4f32), the activations and deltas of a tile in LDS, the
  * tiles' gradients summed in a fixed order (the same rows give the same bits). fp32 throughout, as the reference's.
  *
- * The weights travel in the .net image layout (the one law nets load): in_shift[in], in_scale[in], out_scale, then per
- * layer W[out][in] row-major and b[out]. Entry points return 0, or -1 with the reason in fe_nettrain_error(); they wait
- * for the card. */
+ * The weights travel in the reference's GPU net image (net_layout / net_image, src/field/cuda/CudaNet.cpp:97-136):
+ * in_shift[in], in_scale[in], out_scale, zeros up to the next multiple of 4 floats, then per layer W[out][in] row-major
+ * and b[out]. Entry points return 0, or -1 with the reason in fe_nettrain_error(); they wait for the card. */
 #ifndef FE_NETTRAIN_H
 #define FE_NETTRAIN_H
 
@@ -32,12 +32,13 @@ typedef struct fe_nettrain fe_nettrain;
 fe_nettrain* fe_nettrain_create(const int* sizes, int count, int act, char* why, size_t why_bytes);
 void fe_nettrain_free(fe_nettrain* t);
 const char* fe_nettrain_error(fe_nettrain* t);
-/* Floats of the image (2 in + 1 + the weights and biases). */
+/* Floats of the image (2 in + 1 padded to a multiple of 4, then the weights and biases). */
 int fe_nettrain_image_size(fe_nettrain* t);
 int fe_nettrain_set(fe_nettrain* t, const float* image);
 int fe_nettrain_get(fe_nettrain* t, float* image);
-/* The loss (mean over rows) and, when grad is not NULL, the gradient of the weights and biases (image_size - 2 in - 1
- * floats, in the image's order) for rows x (rows x in), t (rows x out), m (rows, or NULL for 1 each). No update. */
+/* The loss (mean over rows) and, when grad is not NULL, the gradient of the weights and biases (the image's floats
+ * from the first weight on, in its order) for rows x (rows x in), t (rows x out), m (rows, or NULL for 1 each). No
+ * update. */
 int fe_nettrain_gradient(fe_nettrain* t, const float* x, const float* target, const float* m, int rows, float* loss, float* grad);
 /* One Adam step on those rows (the step count carries over calls); the loss before the step into *loss (may be NULL). */
 int fe_nettrain_step(fe_nettrain* t, const float* x, const float* target, const float* m, int rows, float lr, float clip, float* loss);

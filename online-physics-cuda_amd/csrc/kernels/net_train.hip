@@ -48,8 +48,9 @@ struct Shape {
     int d_ld = 0;               // a delta row's stride: the widest layer output, padded to 4, plus 1
     int loss_at = 0;            // LDS floats: the waves' losses
     int lds = 0;                // LDS floats a block
+    int base = 0;               // the first weight: 2 in + 1 padded to a multiple of 4
     int image = 0;              // image floats
-    int trained = 0;            // weights and biases: image - 2 in - 1
+    int trained = 0;            // weights and biases: image - base
 };
 
 __host__ __device__ inline int pad16(int v) { return (v + 15) / 16 * 16; }
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_nt_tile(Shape s, const float* _
     extern __shared__ __attribute__((aligned(16))) float sh[];
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6, c16 = lane & 15, g4 = lane >> 4;
     const int in = s.sizes[0], out = s.sizes[s.layers], ld0 = a_ld(in), dl = s.d_ld;
-    const int tiles = (rows + kRows - 1) / kRows, base = 2 * in + 1;
+    const int tiles = (rows + kRows - 1) / kRows, base = s.base;
     const float scale = w[2 * in];
     float* P = partial + (long long)blockIdx.x * (s.trained + 1);
     // padding columns stay zero for every tile: only the first sizes[k] columns of a row are written
@@ -429,7 +430,8 @@ fe_nettrain* fe_nettrain_create(const int* sizes, int count, int act, char* why,
         }
         s.sizes[k] = sizes[k];
     }
-    int at = 2 * sizes[0] + 1, lds = 0, widest = 0;
+    s.base = (2 * sizes[0] + 1 + 3) / 4 * 4;
+    int at = s.base, lds = 0, widest = 0;
     for (int k = 0; k < s.layers; ++k) {
         s.w_at[k] = at;
         at += s.sizes[k] * s.sizes[k + 1] + s.sizes[k + 1];
@@ -447,7 +449,7 @@ fe_nettrain* fe_nettrain_create(const int* sizes, int count, int act, char* why,
     s.loss_at = lds + 2 * kRows * s.d_ld;
     s.lds = s.loss_at + kWaves;
     s.image = at;
-    s.trained = at - (2 * sizes[0] + 1);
+    s.trained = at - s.base;
     if (s.lds * 4 > kMaxLds) {
         say("the net's tile needs " + std::to_string(s.lds * 4) + " bytes of LDS, more than 65536");
         return nullptr;
@@ -552,8 +554,7 @@ int fe_nettrain_step(fe_nettrain* t, const float* x, const float* target, const 
         const int step = t->adam_t + 1;
         const float c1 = 1.0f - std::pow(b1, static_cast<float>(step));
         const float c2 = 1.0f - std::pow(b2, static_cast<float>(step));
-        const int base = 2 * t->s.sizes[0] + 1;
-        t->gradient(rows, m != nullptr, Adam{t->w.p + base, t->m1.p, t->m2.p, t->s.trained, lr, b1, b2, c1, c2, clip});
+        t->gradient(rows, m != nullptr, Adam{t->w.p + t->s.base, t->m1.p, t->m2.p, t->s.trained, lr, b1, b2, c1, c2, clip});
         t->adam_t = step;
         const hipStream_t st = t->stream();
         float l = 0.0f;

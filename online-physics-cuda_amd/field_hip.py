@@ -755,8 +755,9 @@ ACTIVATIONS = ("tanh", "relu", "silu", "linear", "gelu")
 
 class NetTrainer:
     """A law net's trainer on the card (include/fe_nettrain.h): sizes (inputs, hidden widths, outputs), a hidden
-    activation by name, the weights in the .net image layout (in_shift, in_scale, out_scale, then W[out][in] and b[out]
-    per layer); the loss is the mean over rows of 0.5 m |out_scale f(x) - t|^2, Adam clips each gradient first."""
+    activation by name, the weights in the reference's GPU net image (in_shift, in_scale, out_scale, zeros to a multiple
+    of 4 floats, then W[out][in] and b[out] per layer); the loss is the mean over rows of 0.5 m |out_scale f(x) - t|^2,
+    Adam clips each gradient first."""
 
     def __init__(self, sizes, act: str = "tanh"):
         import numpy as np
@@ -771,7 +772,8 @@ class NetTrainer:
         if not self.ptr:
             raise FieldHipError("trainer refused: " + why.value.decode())
         self.image_size = lib().fe_nettrain_image_size(self.ptr)
-        self.trained = self.image_size - 2 * self.sizes[0] - 1
+        self.base = (2 * self.sizes[0] + 1 + 3) // 4 * 4  # the first weight
+        self.trained = self.image_size - self.base
 
     def __del__(self):
         if getattr(self, "ptr", None):

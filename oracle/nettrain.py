@@ -48,9 +48,14 @@ def act_grad(name: str, x):
     return np.ones_like(x)
 
 
+def first_weight(sizes) -> int:
+    """The image's first weight: 2 in + 1 floats of normalisation padded to a multiple of 4 (CudaNet.cpp:103)."""
+    return (2 * sizes[0] + 1 + 3) // 4 * 4
+
+
 def layout(sizes):
-    """[(W offset, b offset)] per layer in the image, and the image's length."""
-    at = 2 * sizes[0] + 1
+    """[(W offset, b offset)] per layer in the image (CudaNet.cpp:97-110), and the image's length."""
+    at = first_weight(sizes)
     out = []
     for k in range(len(sizes) - 1):
         ni, no = sizes[k], sizes[k + 1]
@@ -110,7 +115,7 @@ def adam(image, grad, m1, m2, step, sizes, lr, clip, b1=0.9, b2=0.999):
     m1[:] = b1 * m1 + (1 - b1) * g
     m2[:] = b2 * m2 + (1 - b2) * g * g
     c1, c2 = 1 - b1**step, 1 - b2**step
-    base = 2 * sizes[0] + 1
+    base = first_weight(sizes)
     image[base:] -= lr * (m1 / c1) / (np.sqrt(m2 / c2) + 1e-8)
 
 

@@ -57,12 +57,32 @@ def test_wide_deep_and_many_rows(device, sizes, n):
     check_gradient(field_hip, sizes, "silu", n, 31)
 
 
+def test_image_padding_is_not_read_nor_written(device):
+    """The reference's GPU image pads the normalisation to a multiple of 4 floats (CudaNet.cpp:103): what lies there
+    changes nothing, and a step leaves it and the normalisation as they were."""
+    import field_hip
+
+    sizes = [5, 9, 4]
+    image = nettrain.make_image(sizes, 51).astype(np.float32)
+    padded = image.copy()
+    padded[11] = 1e30
+    x, t, m = rows_for(sizes, 50, 52)
+    a, b = field_hip.NetTrainer(sizes, "relu"), field_hip.NetTrainer(sizes, "relu")
+    a.set_image(image)
+    b.set_image(padded)
+    la, ga = a.gradient(x, t, m)
+    lb, gb = b.gradient(x, t, m)
+    assert la == lb and np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+    b.step(x, t, m)
+    assert np.array_equal(b.image()[:12].view(np.uint32), padded[:12].view(np.uint32))
+
+
 def test_adam_steps_follow_the_oracle(device):
     import field_hip
 
     sizes = [6, 32, 32, 4]
     tr, image, x, t, m, _ = check_gradient(field_hip, sizes, "tanh", 200, 41)
-    base = 2 * sizes[0] + 1
+    base = nettrain.first_weight(sizes)
     ref = image.astype(np.float64)
     m1, m2 = np.zeros(ref.size - base), np.zeros(ref.size - base)
     lr = 1e-3

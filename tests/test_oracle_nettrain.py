@@ -13,8 +13,9 @@ from oracle import nettrain
 
 def test_known_answer_of_a_one_one_one_net():
     # x 0.5, shift 0.1, scale 2 -> 0.8; W1 0.5, b1 0.1 -> 0.5; tanh -> 0.46211715726000974; W2 2, b2 -0.3 ->
-    # 0.6242343145200195; out_scale 0.8 -> 0.4993874516160156; t 0.25 -> e 0.2493874516160156
-    image = np.array([0.1, 2.0, 0.8, 0.5, 0.1, 2.0, -0.3])
+    # 0.6242343145200195; out_scale 0.8 -> 0.4993874516160156; t 0.25 -> e 0.2493874516160156. The image: shift,
+    # scale, out_scale, one float of padding (the first weight at 4), W1, b1, W2, b2.
+    image = np.array([0.1, 2.0, 0.8, 0.0, 0.5, 0.1, 2.0, -0.3])
     y = nettrain.forward([1, 1, 1], "tanh", image, np.array([[0.5]]))
     assert y[0, 0] == pytest.approx(0.4993874516160156, rel=1e-14)
     loss, g = nettrain.gradient([1, 1, 1], "tanh", image, np.array([[0.5]]), np.array([[0.25]]), np.array([2.0]))
@@ -34,7 +35,7 @@ def test_gradient_is_the_loss_derivative(act):
     t = rng.uniform(-1, 1, (9, 3))
     m = rng.uniform(0.2, 2.0, 9)
     _, g = nettrain.gradient(sizes, act, image, x, t, m)
-    base = 2 * sizes[0] + 1
+    base = nettrain.first_weight(sizes)
     h = 1e-6
     for p in range(base, image.size):
         up, dn = image.copy(), image.copy()
@@ -61,7 +62,7 @@ def test_activations_are_the_textbook_ones():
 def test_adams_first_step_is_lr_times_the_sign_and_clips():
     sizes = [2, 3]
     image = nettrain.make_image(sizes, 3)
-    base = 2 * sizes[0] + 1
+    base = nettrain.first_weight(sizes)
     n = image.size - base
     g = np.linspace(-50, 50, n)
     g[n // 2] = 1e-3
@@ -85,7 +86,7 @@ def test_adam_fits_a_linear_map():
     image = np.zeros(nettrain.layout(sizes)[1])
     image[3:6] = 1.0
     image[6] = 1.0
-    base = 7
+    base = 8  # 2 x 3 + 1 floats of normalisation, padded to 8
     m1, m2 = np.zeros(image.size - base), np.zeros(image.size - base)
     first, _ = nettrain.gradient(sizes, "linear", image, x, t)
     for s in range(1, 3001):
@@ -96,3 +97,18 @@ def test_adam_fits_a_linear_map():
     W = image[base:base + 6].reshape(2, 3)
     np.testing.assert_allclose(W, A, atol=1e-4)
     np.testing.assert_allclose(image[base + 6:], c, atol=1e-4)
+
+
+def test_the_image_is_the_reference_gpu_layout():
+    """CudaNet.cpp:97-110: the first weight at 2 in + 1 padded to a multiple of 4, then per layer (in + 1) x out floats."""
+    for sizes, first, total in [([1, 1, 1], 4, 4 + 2 + 2), ([18, 64, 64, 2], 40, 40 + 19 * 64 + 65 * 64 + 65 * 2),
+                                ([24, 64, 64, 22], 52, 52 + 25 * 64 + 65 * 64 + 65 * 22)]:
+        assert nettrain.first_weight(sizes) == first
+        assert nettrain.layout(sizes)[1] == total
+    # the padding is not read: changing it changes nothing
+    sizes = [5, 7, 3]
+    image = nettrain.make_image(sizes, 9)
+    x = np.random.default_rng(1).uniform(-1, 1, (4, 5))
+    other = image.copy()
+    other[11] = 123.0
+    np.testing.assert_array_equal(nettrain.forward(sizes, "tanh", image, x), nettrain.forward(sizes, "tanh", other, x))
