@@ -67,27 +67,28 @@ __device__ __host__ __forceinline__ unsigned short to_bf16_bits(float f)
 __device__ __forceinline__ __bf16 bf(float f) { return static_cast<__bf16>(f); }
 __device__ __forceinline__ float unbf(__bf16 b) { return __builtin_bit_cast(float, static_cast<unsigned int>(__builtin_bit_cast(unsigned short, b)) << 16); }
 
-// tanh u = 1 - 2 / (1 + e^(2u)) through the hardware exponential and reciprocal (a few ulp; the activations it feeds
-// are rounded to bf16, 8 bits): the library tanhf is a long polynomial sequence, and the net's forward and backward
-// passes were bound by it, not by the matrix cores. Saturates to +-1 (e^(2u) overflowing to inf, or to 0).
-__device__ __forceinline__ float fast_tanh(float u)
+// GELU, tanh form: 0.5 z (1 + tanh u), u = sqrt(2 / pi) (z + 0.044715 z^3), computed as z s with s = 1 / (1 + e^(-2u))
+// (the same function: 0.5 (1 + tanh u) is the logistic of 2u), e^(-2u) as 2^(z (a + b z^2)) with the constants folded,
+// through the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32: a few ulp; the activations it feeds are rounded
+// to bf16, 8 bits). Eight instructions where the library tanhf is a long polynomial sequence: the net's forward and
+// backward passes are bound by these, not by the matrix cores. Saturates to z (e^(-2u) -> 0) and to -0 (-> inf).
+constexpr float kGeluA = -2.0f * 0.7978845608028654f * 1.4426950408889634f;             // -2 sqrt(2/pi) log2(e)
+constexpr float kGeluB = -2.0f * 0.7978845608028654f * 0.044715f * 1.4426950408889634f; // the same times 0.044715
+
+__device__ __forceinline__ float gelu_logistic(float z)
 {
-    const float e = __expf(2.0f * u);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+    const float e = __builtin_amdgcn_exp2f(z * (kGeluA + kGeluB * (z * z)));
+    return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-__device__ __forceinline__ float gelu(float z)
-{
-    const float k0 = 0.7978845608028654f; // sqrt(2 / pi)
-    const float t = fast_tanh(k0 * (z + 0.044715f * z * z * z));
-    return 0.5f * z * (1.0f + t);
-}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_logistic(z); }
 
+// d/dz z s(z) = s + z s (1 - s) 2 u'(z), u' = sqrt(2 / pi) (1 + 3 0.044715 z^2).
 __device__ __forceinline__ float gelu_grad(float z)
 {
     const float k0 = 0.7978845608028654f;
-    const float t = fast_tanh(k0 * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k0 * (1.0f + 3.0f * 0.044715f * z * z);
+    const float sg = gelu_logistic(z);
+    return sg + z * sg * (1.0f - sg) * (2.0f * k0) * (1.0f + 3.0f * 0.044715f * (z * z));
 }
 
 __device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const floatx4& c)
