@@ -79,7 +79,7 @@ __device__ inline float gauss(u64 key)
 // Saturates to +-1 (e^(2u) overflowing to inf, or to 0).
 __device__ __forceinline__ float fast_tanh(float u)
 {
-    const float e = __expf(2.0f * u);
+    const float e = __builtin_amdgcn_exp2f((2.0f * 1.4426950408889634f) * u); // e^(2u), 2 log2(e) folded
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
 }
 

@@ -59,16 +59,26 @@ __host__ __device__ inline int pad16(int v) { return (v + 15) / 16 * 16; }
 // issued together; the padding stays zero) plus 1, so the 16 rows a product reads at once fall in different banks.
 __host__ __device__ inline int a_ld(int n) { return pad16(n) + 1; }
 
+// The activations through the hardware base-2 exponential and reciprocal (v_exp_f32, v_rcp_f32, a few ulp of fp32;
+// near zero tanh's 1 - 2 / (1 + e^(2x)) is within an ulp of 1 absolute): tanh x = 1 - 2 / (1 + e^(2x)),
+// silu x = x / (1 + e^(-x)), gelu (tanh form) z / (1 + e^(-2u)), u = sqrt(2 / pi) (z + 0.044715 z^3).
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kGeluK0 = 0.7978845608028654f; // sqrt(2 / pi)
+
+__device__ __forceinline__ float logistic2(float y) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y)); } // 1 / (1 + 2^y)
+
+__device__ __forceinline__ float gelu_s(float x) // 1 / (1 + e^(-2u))
+{
+    return logistic2(x * ((-2.0f * kGeluK0 * kLog2e) + (-2.0f * kGeluK0 * 0.044715f * kLog2e) * (x * x)));
+}
+
 __device__ __forceinline__ float act_f(int a, float x)
 {
     switch (a) {
-    case 0: return tanhf(x);
+    case 0: return 1.0f - 2.0f * logistic2((2.0f * kLog2e) * x);
     case 1: return x > 0.0f ? x : 0.0f;
-    case 2: return x / (1.0f + expf(-x));
-    case 4: {
-        const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-        return 0.5f * x * (1.0f + tanhf(u));
-    }
+    case 2: return x * logistic2(-kLog2e * x);
+    case 4: return x * gelu_s(x);
     default: return x;
     }
 }
@@ -77,18 +87,17 @@ __device__ __forceinline__ float act_d(int a, float x)
 {
     switch (a) {
     case 0: {
-        const float t = tanhf(x);
+        const float t = 1.0f - 2.0f * logistic2((2.0f * kLog2e) * x);
         return 1.0f - t * t;
     }
     case 1: return x > 0.0f ? 1.0f : 0.0f;
     case 2: {
-        const float s = 1.0f / (1.0f + expf(-x));
+        const float s = logistic2(-kLog2e * x);
         return s + x * s * (1.0f - s);
     }
     case 4: {
-        const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-        const float t = tanhf(u);
-        return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+        const float s = gelu_s(x);
+        return s + x * s * (1.0f - s) * (2.0f * kGeluK0) * (1.0f + 3.0f * 0.044715f * (x * x));
     }
     default: return 1.0f;
     }
