@@ -32,10 +32,9 @@ constexpr int kMaxWidth = 128;  // any layer's width
 constexpr int kRows = 16;       // rows a tile (a wave)
 constexpr int kChunk = 32;      // blocks' partials summed together in the first level (past kOneLevel blocks)
 constexpr int kOneLevel = 64;   // up to this many blocks their partials are added in one level
-constexpr int kBlocks = 1024;   // the most blocks a pass (four waves a CU)
+constexpr int kBlocks = 1024;   // the most blocks a pass (four blocks of four waves a CU)
 constexpr int kMaxLds = 65536;  // bytes of LDS a block may use
 constexpr int kWaves = 4;       // waves a block: a tile's 16-column blocks spread over them
-
 
 struct Shape {
     int layers = 0;
@@ -45,7 +44,7 @@ struct Shape {
     int a_at[kMaxLayers] = {};  // LDS floats: layer k's inputs, 16 rows of a_ld(sizes[k])
     int p_at[kMaxLayers] = {};  // LDS floats: hidden layer k's pre-activations, 16 x sizes[k + 1]
     int d_at[2] = {};           // LDS floats: two delta buffers, 16 rows of d_ld each
-    int d_ld = 0;               // a delta row's stride: the widest layer output, padded to 4, plus 1
+    int d_ld = 0;               // a delta row's stride: a_ld of the widest layer output
     int loss_at = 0;            // LDS floats: the waves' losses
     int lds = 0;                // LDS floats a block
     int base = 0;               // the first weight: 2 in + 1 padded to a multiple of 4
