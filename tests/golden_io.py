@@ -53,3 +53,23 @@ def same(meta, got, want) -> bool:
     if atol is None:
         return got.shape == want.shape and bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
     return got.shape == want.shape and bool(np.allclose(got, want, rtol=0.0, atol=atol))
+
+
+def nettrain_reference_cases():
+    """The vectors the reference's own CPU net code wrote for the net trainer (tests/golden/make_nettrain_golden.py): per
+    case (name, sizes, activation, the image in the reference's GPU layout as float64, x, t, m, the rows' mean loss,
+    the mean gradient)."""
+    from oracle import nettrain
+
+    z = np.load(os.path.join(GOLDEN, "nettrain", "ref_cases.npz"), allow_pickle=False)
+    names = sorted({k[: -len("_sizes")] for k in z.files if k.endswith("_sizes")})
+    for name in names:
+        sizes = [int(v) for v in z[f"{name}_sizes"]]
+        image = np.zeros(nettrain.layout(sizes)[1])
+        n = sizes[0]
+        image[:n] = z[f"{name}_shift"]
+        image[n:2 * n] = z[f"{name}_scale"]
+        image[2 * n] = z[f"{name}_out_scale"]
+        image[nettrain.first_weight(sizes):] = z[f"{name}_weights"]
+        yield (name, sizes, nettrain.ACTS[int(z[f"{name}_act"])], image, z[f"{name}_x"], z[f"{name}_t"], z[f"{name}_m"],
+               float(z[f"{name}_loss"]), z[f"{name}_grad"])

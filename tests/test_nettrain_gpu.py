@@ -135,3 +135,20 @@ def test_refusals(device):
         tr.gradient(np.zeros((4, 2), np.float32), np.zeros((4, 2), np.float32))
     with pytest.raises(field_hip.FieldHipError, match="no rows"):
         tr.gradient(np.zeros((0, 3), np.float32), np.zeros((0, 2), np.float32))
+
+
+def test_matches_the_reference_cpu_net_code(device):
+    """The card against the vectors the reference's own CPU net code wrote (net_forward_taped / net_backward, fp32,
+    tests/golden/nettrain/ref_cases.npz): loss within 1e-4 relative, gradient within 2e-4 of its largest element."""
+    import field_hip
+    import golden_io
+
+    seen = 0
+    for name, sizes, act, image, x, t, m, loss, grad in golden_io.nettrain_reference_cases():
+        tr = field_hip.NetTrainer(sizes, act)
+        tr.set_image(image.astype(np.float32))
+        got_loss, got = tr.gradient(x, t, m)
+        assert got_loss == pytest.approx(loss, rel=1e-4), name
+        assert np.max(np.abs(got - grad)) <= 2e-4 * np.max(np.abs(grad)), name
+        seen += 1
+    assert seen == 5

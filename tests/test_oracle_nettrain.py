@@ -1,9 +1,11 @@
 """The net trainer's oracle (oracle/nettrain.py) pinned: a hand-computed known answer for the forward pass and the loss,
 the analytic gradient against central differences of the loss for every activation (a dropped term, a transposed
 operand or a wrong derivative fails it), the activations against their textbook definitions, Adam's first step
-(lr sign(g) whatever g's size) and clipping, and a linear regression that Adam fits."""
+(lr sign(g) whatever g's size) and clipping, a linear regression that Adam fits, and the reference's own CPU net code
+(net_forward_taped / net_backward) on five nets (tests/golden/nettrain/ref_cases.npz)."""
 import math
 
+import golden_io
 import numpy as np
 import pytest
 from scipy.special import erf, expit
@@ -112,3 +114,15 @@ def test_the_image_is_the_reference_gpu_layout():
     other = image.copy()
     other[11] = 123.0
     np.testing.assert_array_equal(nettrain.forward(sizes, "tanh", image, x), nettrain.forward(sizes, "tanh", other, x))
+
+
+def test_matches_the_reference_cpu_net_code():
+    """The oracle (float64) against the reference's net_forward_taped / net_backward (fp32): loss to 1e-6 relative,
+    gradient to 1e-5 of its largest element."""
+    seen = 0
+    for name, sizes, act, image, x, t, m, loss, grad in golden_io.nettrain_reference_cases():
+        wl, wg = nettrain.gradient(sizes, act, image, x.astype(np.float64), t.astype(np.float64), m.astype(np.float64))
+        assert wl == pytest.approx(loss, rel=1e-6), name
+        assert np.max(np.abs(wg - grad)) <= 1e-5 * np.max(np.abs(wg)), name
+        seen += 1
+    assert seen == 5
