@@ -352,6 +352,14 @@ struct Impl final : HipBackend {
             }
             return w.vx == 4 ? 1 : 0;
         };
+        // A window's tile is 64 vx cells wide; a row that fills its tiles badly (144 cells in a 256-wide tile, or in three
+        // 64-wide ones) leaves lanes idle in every plane of every tile, and the direct gather was faster there
+        // (the valley law on 144 x 144 x 48: 0.66 ms a step against 0.82 through 256x2 and 64x6 tiles), so a window is
+        // taken only when the row fills at least 90% of its tiles' width.
+        auto fills = [&](const hip::WindowKernel& w) {
+            const long long width = 64LL * w.vx, tiles_x = (shape.nx + width - 1) / width;
+            return 10LL * shape.nx >= 9LL * width * tiles_x;
+        };
         for (const hip::WindowKernel& w : code.windows) {
             if (w.vx == 4 && shape.nx % 4 != 0) {
                 continue;
@@ -362,7 +370,7 @@ struct Impl final : HipBackend {
                 }
                 continue;
             }
-            if (heavy.count(w.name) != 0) {
+            if (heavy.count(w.name) != 0 || !fills(w)) {
                 continue;
             }
             const int zc = zchunk_override > 0 ? zchunk_override : (w.zchunk > 0 ? w.zchunk : code.zchunk);
