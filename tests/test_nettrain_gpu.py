@@ -152,3 +152,20 @@ def test_matches_the_reference_cpu_net_code(device):
         assert np.max(np.abs(got - grad)) <= 2e-4 * np.max(np.abs(grad)), name
         seen += 1
     assert seen == 5
+
+
+def test_rows_weighted_zero_give_zero_and_adam_leaves_the_weights(device):
+    """Every row weighted 0: the loss and every gradient are exactly 0, and a step moves nothing (Adam's update of a
+    zero gradient is 0 / (0 + eps))."""
+    import field_hip
+
+    sizes = [7, 20, 5]
+    image = nettrain.make_image(sizes, 61).astype(np.float32)
+    tr = field_hip.NetTrainer(sizes, "gelu")
+    tr.set_image(image)
+    x, t, _ = rows_for(sizes, 333, 62)
+    m = np.zeros(333, np.float32)
+    loss, g = tr.gradient(x, t, m)
+    assert loss == 0.0 and not np.any(g)
+    tr.step(x, t, m)
+    assert np.array_equal(tr.image().view(np.uint32), image.view(np.uint32))
