@@ -391,3 +391,19 @@ def test_many_term_world_law_runs_as_the_reference(device):
             assert golden_io.same(meta, got, expected), f"{field}: max |diff| {np.max(np.abs(got - expected))}"
         digests.append(s.digest())
     assert digests[0] == digests[1]
+
+
+@pytest.mark.parametrize("nx,window", [(144, False), (192, True), (256, True), (132, False)])
+def test_a_window_only_where_rows_fill_its_tiles(device, monkeypatch, nx, window):
+    """The gather a term gets: a window only when a row fills at least 90% of its tiles' width (64 cells with a cell a
+    lane, 256 with four), the direct gather otherwise; both give the reference's bits, so this pins the choice itself."""
+    import field_hip
+
+    for key in ("FIELD_HIP_DIRECT", "FIELD_HIP_TILE", "FIELD_HIP_ZCHUNK", "FIELD_HIP_REMAP", "FIELD_HIP_STREAM"):
+        monkeypatch.delenv(key, raising=False)
+    law = field_hip.Law("field T init 0\nfield k init 1 static\nterm t\n  d T = diffuse(k, T) + 0.1 * lap(T)\n")
+    s = field_hip.Solver(law, (nx, 12, 10), 0.5)
+    s.set_field("T", np.random.default_rng(nx).random(nx * 120, dtype=np.float32))
+    s.step(0.01)
+    kernel = s.profile()["terms"][0]["kernel"]
+    assert ("_w" in kernel) == window, (nx, kernel)
