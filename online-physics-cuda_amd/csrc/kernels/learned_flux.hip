@@ -227,7 +227,7 @@ __device__ __forceinline__ void forward16(const bf16x8* __restrict__ frags, cons
 
 // Face fluxes of every face evaluation: F[fe][field] = s (g_direct - g_mirror) / 2 (0 across the lattice's edge).
 // A wave takes 16 rows (8 faces) at a time.
-__global__ __launch_bounds__(256) void k_faces(const bf16x8* __restrict__ frags, const float* __restrict__ params, const float* __restrict__ u,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_faces(const bf16x8* __restrict__ frags, const float* __restrict__ params, const float* __restrict__ u,
     const float* __restrict__ v, Lattice L, float scale, long long faces, float* __restrict__ F)
 {
     const int l = (int)(threadIdx.x & 63u);
